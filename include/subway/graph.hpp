@@ -1,0 +1,126 @@
+// graph.hpp -- Graph<E> / GraphPR<E>: host CSR plus per-vertex state, the
+// drop-in replacement of shared/graph.cuh:7-59.
+//
+// Public fields keep the reference's names and meaning (host arrays without
+// prefix, device arrays with d_).  Differences a caller can see:
+//  * offsets are 64-bit (nodePointer is ull[num_nodes+1], num_edges is ull),
+//    so graphs with 2^32 or more edges load (the legacy u32 file header is
+//    still read and written byte-exact);
+//  * outDegree is correct for every vertex (the reference never writes
+//    outDegree[num_nodes-2], shared/graph.cu:177-181);
+//  * the pinned edge list is also mapped into the device address space
+//    (d_edgeList) so kernels can gather from it without a staging copy;
+//  * d_nodePointer mirrors nodePointer on the device;
+//  * destructors release every allocation.
+#ifndef SUBWAY_GRAPH_HPP
+#define SUBWAY_GRAPH_HPP
+
+#include <memory>
+#include <string>
+
+#include "globals.hpp"
+
+namespace subway {
+
+// Host CSR with pinned, device-mapped edges.  Shared between graph objects
+// so that several runs (or a Graph and a GraphPR) reuse one loaded file.
+template <class E>
+struct HostCsr {
+    ull n = 0;
+    ull m = 0;
+    ull *ptr = nullptr;     // n+1 offsets (ordinary host memory)
+    E *edges = nullptr;     // m edges, hipHostMalloc'd (pinned + mapped)
+    E *d_edges = nullptr;   // device address of `edges`
+    uint *degree = nullptr; // n out-degrees
+    uint maxDegree = 0;
+    bool large = false;     // read from / destined for the 64-bit file variant
+    bool pinned = false;    // edges live in pinned, device-mapped memory
+    HostCsr() = default;
+    HostCsr(const HostCsr &) = delete;
+    HostCsr &operator=(const HostCsr &) = delete;
+    ~HostCsr();
+    void allocate(ull n, ull m);   // allocates ptr, edges, degree
+    void finalize();               // degrees, maxDegree, device mapping
+};
+
+template <class E>
+std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path);
+template <class E>
+void write_csr_file(const HostCsr<E> &csr, const std::string &path);
+
+std::string file_extension(const std::string &fileName);
+
+}  // namespace subway
+
+template <class E>
+class Graph {
+public:
+    std::string graphFilePath;
+    bool isWeighted;
+    bool isLarge;
+    uint num_nodes;
+    ull num_edges;
+    ull *nodePointer;
+    E *edgeList;
+    uint *outDegree;
+    bool *label1;
+    bool *label2;
+    uint *value;
+    uint *d_outDegree;
+    uint *d_value;
+    bool *d_label1;
+    bool *d_label2;
+    ull *d_nodePointer;
+    E *d_edgeList;          // device-mapped view of the pinned edgeList
+    std::string graphFormat;
+    std::shared_ptr<subway::HostCsr<E>> csr;
+
+    Graph(std::string graphFilePath, bool isWeighted);
+    // Wrap an already-built CSR (no file involved).
+    explicit Graph(std::shared_ptr<subway::HostCsr<E>> csr);
+    ~Graph();
+    Graph(const Graph &) = delete;
+    Graph &operator=(const Graph &) = delete;
+    std::string GetFileExtension(std::string fileName);
+    void AssignW8(uint w8, uint index);
+    void ReadGraph();
+
+private:
+    void attach();
+};
+
+template <class E>
+class GraphPR {
+public:
+    std::string graphFilePath;
+    bool isWeighted;
+    bool isLarge;
+    uint num_nodes;
+    ull num_edges;
+    ull *nodePointer;
+    E *edgeList;
+    uint *outDegree;
+    float *value;
+    float *delta;
+    uint *d_outDegree;
+    float *d_value;
+    float *d_delta;
+    ull *d_nodePointer;
+    E *d_edgeList;
+    std::string graphFormat;
+    std::shared_ptr<subway::HostCsr<E>> csr;
+
+    GraphPR(std::string graphFilePath, bool isWeighted);
+    explicit GraphPR(std::shared_ptr<subway::HostCsr<E>> csr);
+    ~GraphPR();
+    GraphPR(const GraphPR &) = delete;
+    GraphPR &operator=(const GraphPR &) = delete;
+    std::string GetFileExtension(std::string fileName);
+    void AssignW8(uint w8, uint index);
+    void ReadGraph();
+
+private:
+    void attach();
+};
+
+#endif  // SUBWAY_GRAPH_HPP

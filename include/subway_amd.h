@@ -1,0 +1,152 @@
+/*
+ * subway_amd.h -- C ABI of the MI355X-native Subway pipeline.
+ *
+ * The reference has no C ABI: its boundary is a C++ template library linked
+ * into ten application binaries (Makefile:12, the subway/ apps).  These entry
+ * points expose the same operations with plain pointers and sizes so that a
+ * harness (ctypes, cgo, JNI, ...) can drive the path without C++.  Every
+ * function returns 0 on success or a negative subway status / positive HIP
+ * error code; subway_last_error() returns the message of the calling
+ * thread's last failure.  Nothing here exits the process (the reference's
+ * gpuErrorcheck does, shared/gpu_error_check.cuh:8-16; the C++ API keeps
+ * that behaviour).
+ */
+#ifndef SUBWAY_AMD_H
+#define SUBWAY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (negative); positive values are hipError_t codes */
+#define SUBWAY_OK 0
+#define SUBWAY_ERR_ARG (-1)
+#define SUBWAY_ERR_IO (-2)
+#define SUBWAY_ERR_FORMAT (-3)
+#define SUBWAY_ERR_BUDGET (-4)
+#define SUBWAY_ERR_NOMEM (-5)
+#define SUBWAY_ERR_INTERNAL (-6)
+
+/* algorithms and modes (the ten apps are algo x mode) */
+#define SUBWAY_BFS 0
+#define SUBWAY_CC 1
+#define SUBWAY_SSSP 2
+#define SUBWAY_SSWP 3
+#define SUBWAY_PR 4
+#define SUBWAY_SYNC 0
+#define SUBWAY_ASYNC 1
+
+/* shared/globals.hpp:27 */
+#define SUBWAY_DIST_INFINITY 4294967294u
+
+typedef struct subway_graph subway_graph;
+
+typedef struct subway_options {
+    int32_t algo;           /* SUBWAY_BFS .. SUBWAY_PR */
+    int32_t mode;           /* SUBWAY_SYNC / SUBWAY_ASYNC */
+    uint32_t source;        /* --source (default 0) */
+    int32_t num_buffers;    /* 1 or 2 device edge buffers (2 overlaps copy/compute) */
+    uint64_t budget_bytes;  /* device edge budget; 0 = reference sizing */
+    float acc;              /* PR threshold (default 0.01) */
+    float init_pr;          /* PR initial delta (default 0.15) */
+    int32_t transfer;       /* 0 auto, 1 GPU zero-copy gather, 2 host-thread gather */
+    int32_t policy;         /* 0 plain */
+    int32_t verbose;        /* print per-partition progress lines */
+    int32_t advance_grid;   /* persistent workgroups of the advance kernel; 0 = auto */
+    int32_t profile;        /* 1: time every kernel launch and transfer with HIP events */
+    int32_t reserved[5];
+} subway_options;
+
+typedef struct subway_stats {
+    double ms_total;        /* first generate .. values on host */
+    double ms_transfer;     /* copy-stream busy time (profile = 1) */
+    double ms_advance;      /* advance kernel time (profile = 1) */
+    double ms_frontier;     /* frontier kernel time (profile = 1) */
+    double ms_generate;     /* generator + partition kernels (profile = 1) */
+    uint64_t h2d_bytes;     /* edge bytes moved host -> device */
+    uint64_t h2d_dma_bytes;
+    uint64_t h2d_gather_bytes;
+    uint64_t inner_rounds;  /* frontier launches that selected work */
+    uint64_t partitions;
+    uint64_t updates;       /* successful relaxations / PR pushes */
+    uint64_t advance_launches;
+    uint64_t frontier_launches;
+    uint64_t transfer_ops;
+    uint64_t work_edges;    /* edges relaxed over all rounds */
+    uint64_t work_entries;  /* vertices that pushed over all rounds */
+    uint32_t global_iterations;
+    uint32_t trace_len;     /* records written to the trace array */
+} subway_stats;
+
+typedef struct subway_iter_record {
+    uint32_t iteration;
+    uint32_t partitions;
+    uint64_t active_nodes;
+    uint64_t active_edges;
+    uint64_t h2d_bytes;
+    double ms_iteration;
+} subway_iter_record;
+
+/* -------- library */
+int32_t subway_version(void);
+const char *subway_last_error(void);
+void subway_default_options(subway_options *opt);
+/* number of visible HIP devices (0 without a GPU; never fails) */
+int32_t subway_device_count(void);
+
+/* -------- graphs (host CSR in pinned, device-mapped memory) */
+/* Graph<E>(path, isWeighted).ReadGraph(): shared/graph.cu:32-197.
+ * .bcsr/.bwcsr (u32 header), .bcsr64/.bwcsr64 (u64 header), .el/.wel text. */
+int32_t subway_load(const char *path, int32_t weighted, subway_graph **out);
+/* Build from caller arrays: node_pointer[n+1], ends[m], weights[m] or NULL. */
+int32_t subway_graph_from_csr(uint64_t n, uint64_t m, const uint64_t *node_pointer, const uint32_t *ends,
+                              const uint32_t *weights, subway_graph **out);
+/* Reproducible R-MAT (a, b, c; d = 1-a-b-c), self-loops dropped. */
+int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                          int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax,
+                          subway_graph **out);
+/* tools/converter.cpp:88-93,157-162 file layouts; extension picks the variant */
+int32_t subway_graph_save(const subway_graph *g, const char *path);
+int32_t subway_graph_info(const subway_graph *g, uint64_t *n, uint64_t *m, int32_t *weighted,
+                          uint32_t *max_degree, uint32_t *max_degree_vertex);
+/* copy the CSR out: node_pointer[n+1], ends[m], weights[m] (may be NULL) */
+int32_t subway_graph_export(const subway_graph *g, uint64_t *node_pointer, uint32_t *ends, uint32_t *weights);
+void subway_graph_free(subway_graph *g);
+
+/* -------- the path: one application run to its fixpoint.
+ * values: n x uint32 (BFS/CC/SSSP/SSWP) or n x float (PR).
+ * trace (may be NULL): per-global-iteration records, up to trace_cap. */
+int32_t subway_run(const subway_graph *g, const subway_options *opt, void *values, subway_stats *stats,
+                   subway_iter_record *trace, uint32_t trace_cap);
+
+/* -------- single stages on host arrays (each runs the GPU kernel) */
+/* SubgraphGenerator::generate: shared/subgraph_generator.cu:128-162.
+ * active_nodes[n], active_nodes_pointer[n+1]. */
+int32_t subway_generate(uint64_t n, const uint8_t *label1, const uint8_t *label2, const uint32_t *out_degree,
+                        uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active);
+/* SubgraphGenerator::generate(GraphPR&, ..., acc): shared/subgraph_generator.cu:209-243 */
+int32_t subway_generate_pr(uint64_t n, const float *delta, float acc, const uint32_t *out_degree,
+                           uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active);
+/* Partitioner::partition: shared/partitioner.cu:12-91.  rows: 6 x u64 per
+ * partition (fromNode, nodeCount, fromEdge, edgeCount, firstId, lastId). */
+int32_t subway_partition(uint64_t num_active, const uint64_t *active_nodes_pointer, const uint32_t *active_nodes,
+                         uint64_t budget_edges, uint64_t max_rows, uint64_t *rows, uint64_t *num_rows);
+/* Gather of the active vertices' adjacency (the host-thread copy of
+ * shared/subgraph_generator.cu:63-97) by the GPU from mapped pinned memory:
+ * out receives sum(degree) edges of sizeof(edge) bytes. */
+int32_t subway_gather(const subway_graph *g, uint64_t num_active, const uint32_t *active_nodes, void *out,
+                      uint64_t *num_edges);
+
+/* -------- measurement */
+/* Pinned-copy peak: hipMemcpyAsync of `bytes` from hipHostMalloc memory to
+ * the device, best of `reps`, in GB/s (10^9 B/s).  No reference counterpart;
+ * the denominator of the achieved host-to-device rate. */
+int32_t subway_h2d_peak(uint64_t bytes, int32_t reps, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUBWAY_AMD_H */

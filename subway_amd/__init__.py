@@ -1,0 +1,283 @@
+"""subway_amd -- Python host mirror of the MI355X-native Subway pipeline.
+
+Everything here calls the C ABI of ``lib/libsubway_amd.so`` (declared in
+``include/subway_amd.h``) through ctypes; there is no Python or CPU fallback
+for any GPU stage.  Importing works without a GPU (graph construction,
+format conversion and R-MAT generation are host code); running a stage
+without a GPU raises :class:`SubwayError`.
+
+The entry points mirror the reference's applications
+(``subway/{bfs,cc,sssp,sswp,pr}-{sync,async}.cu``): :func:`run` takes the
+algorithm, the mode and the reference's ``--source`` meaning, plus the
+device edge budget that drives partitioning.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import lib, LIB_PATH, SubwayError, check
+
+__all__ = [
+    "Graph", "RunStats", "run", "generate", "generate_pr", "partition", "gather",
+    "device_count", "DIST_INFINITY", "ALGOS", "LIB_PATH", "SubwayError", "APPS", "h2d_peak",
+]
+
+DIST_INFINITY = 4294967294  # shared/globals.hpp:27
+ALGOS = {"bfs": 0, "cc": 1, "sssp": 2, "sswp": 3, "pr": 4}
+MODES = {"sync": 0, "async": 1}
+TRANSFERS = {"auto": 0, "zerocopy": 1, "host": 2}
+POLICIES = {"plain": 0, "learned": 1}
+# the ten applications of the reference (subway/*.cu)
+APPS = [f"{a}-{m}" for a in ("bfs", "cc", "sssp", "sswp", "pr") for m in ("sync", "async")]
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("algo", ctypes.c_int32), ("mode", ctypes.c_int32), ("source", ctypes.c_uint32),
+        ("num_buffers", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64), ("acc", ctypes.c_float),
+        ("init_pr", ctypes.c_float), ("transfer", ctypes.c_int32), ("policy", ctypes.c_int32),
+        ("verbose", ctypes.c_int32), ("advance_grid", ctypes.c_int32), ("profile", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double), ("ms_transfer", ctypes.c_double), ("ms_advance", ctypes.c_double),
+        ("ms_frontier", ctypes.c_double), ("ms_generate", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
+        ("h2d_dma_bytes", ctypes.c_uint64), ("h2d_gather_bytes", ctypes.c_uint64),
+        ("inner_rounds", ctypes.c_uint64), ("partitions", ctypes.c_uint64), ("updates", ctypes.c_uint64),
+        ("advance_launches", ctypes.c_uint64), ("frontier_launches", ctypes.c_uint64),
+        ("transfer_ops", ctypes.c_uint64), ("work_edges", ctypes.c_uint64), ("work_entries", ctypes.c_uint64),
+        ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
+    ]
+
+
+class _IterRecord(ctypes.Structure):
+    _fields_ = [
+        ("iteration", ctypes.c_uint32), ("partitions", ctypes.c_uint32), ("active_nodes", ctypes.c_uint64),
+        ("active_edges", ctypes.c_uint64), ("h2d_bytes", ctypes.c_uint64), ("ms_iteration", ctypes.c_double),
+    ]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def device_count() -> int:
+    """Number of visible HIP devices (0 when there is no GPU)."""
+    return int(lib.subway_device_count())
+
+
+class Graph:
+    """Host CSR in pinned, device-mapped memory (Graph<E> / GraphPR<E>).
+
+    ``weighted`` graphs hold OutEdgeWeighted {end, w8} edges (.bwcsr),
+    unweighted ones OutEdge {end} (.bcsr).
+    """
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+        n, m = ctypes.c_uint64(), ctypes.c_uint64()
+        w, md, mv = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib.subway_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(w),
+                                    ctypes.byref(md), ctypes.byref(mv)))
+        self.n, self.m, self.weighted = int(n.value), int(m.value), bool(w.value)
+        self.max_degree, self.max_degree_vertex = int(md.value), int(mv.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.subway_graph_free(h)
+            self._h = None
+
+    # --- constructors
+    @classmethod
+    def load(cls, path: str, weighted: bool | None = None) -> "Graph":
+        """Read .bcsr/.bwcsr (and 64-bit variants) or .el/.wel text."""
+        if weighted is None:
+            weighted = os.path.splitext(path)[1] in (".bwcsr", ".bwcsr64", ".wel")
+        h = ctypes.c_void_p()
+        check(lib.subway_load(path.encode(), int(bool(weighted)), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, node_pointer, ends, weights=None) -> "Graph":
+        ptr = np.ascontiguousarray(node_pointer, dtype=np.uint64)
+        e = np.ascontiguousarray(ends, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
+        n = len(ptr) - 1
+        if n < 1:
+            raise SubwayError(-1, "a graph needs at least one vertex")
+        h = ctypes.c_void_p()
+        check(lib.subway_graph_from_csr(n, len(e), _ptr(ptr, _u64p), _ptr(e, _u32p), _ptr(w, _u32p),
+                                        ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n: int, src, dst, weights=None) -> "Graph":
+        """CSR by a stable sort on the source (tools/converter.cpp:64-83)."""
+        src = np.asarray(src, dtype=np.uint32)
+        dst = np.asarray(dst, dtype=np.uint32)
+        order = np.argsort(src, kind="stable")
+        deg = np.bincount(src, minlength=n).astype(np.uint64)
+        ptr = np.zeros(n + 1, dtype=np.uint64)
+        np.cumsum(deg, out=ptr[1:])
+        w = None if weights is None else np.asarray(weights, dtype=np.uint32)[order]
+        return cls.from_csr(ptr, dst[order], w)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 42, weighted: bool = False, symmetrize: bool = False, wmin: int = 1,
+             wmax: int = 64) -> "Graph":
+        h = ctypes.c_void_p()
+        check(lib.subway_graph_rmat(scale, edge_factor, a, b, c, seed, int(weighted), int(symmetrize),
+                                    wmin, wmax, ctypes.byref(h)))
+        return cls(h)
+
+    # --- accessors
+    def save(self, path: str) -> None:
+        check(lib.subway_graph_save(self._h, path.encode()))
+
+    def csr(self):
+        """(node_pointer[n+1] u64, ends[m] u32, weights[m] u32 or None)."""
+        ptr = np.empty(self.n + 1, dtype=np.uint64)
+        ends = np.empty(self.m, dtype=np.uint32)
+        w = np.empty(self.m, dtype=np.uint32) if self.weighted else None
+        check(lib.subway_graph_export(self._h, _ptr(ptr, _u64p), _ptr(ends, _u32p), _ptr(w, _u32p)))
+        return ptr, ends, w
+
+    def out_degree(self):
+        ptr, _, _ = self.csr()
+        return np.diff(ptr).astype(np.uint32)
+
+
+@dataclass
+class RunStats:
+    ms_total: float
+    ms_transfer: float
+    ms_advance: float
+    ms_frontier: float
+    ms_generate: float
+    h2d_bytes: int
+    h2d_dma_bytes: int
+    h2d_gather_bytes: int
+    inner_rounds: int
+    partitions: int
+    updates: int
+    advance_launches: int
+    frontier_launches: int
+    transfer_ops: int
+    work_edges: int
+    work_entries: int
+    global_iterations: int
+    trace: list = field(default_factory=list)
+
+
+def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
+        threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
+        verbose: bool = False, advance_grid: int = 0, profile: bool = False):
+    """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
+
+    Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
+    float32 (PR).  ``budget`` is the device edge budget in bytes (0 = 90% of
+    device memory, the reference's sizing).
+    """
+    o = _Options()
+    lib.subway_default_options(ctypes.byref(o))
+    o.algo = ALGOS[algo]
+    o.mode = MODES[mode]
+    o.source = source
+    o.budget_bytes = budget
+    o.num_buffers = buffers
+    o.acc = threshold
+    o.init_pr = init_pr
+    o.transfer = TRANSFERS[transfer]
+    o.policy = POLICIES[policy]
+    o.verbose = int(verbose)
+    o.advance_grid = advance_grid
+    o.profile = int(profile)
+    vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
+    st = _Stats()
+    cap = 4096
+    rec = (_IterRecord * cap)()
+    check(lib.subway_run(graph._h, ctypes.byref(o), vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st),
+                         rec, cap))
+    trace = [dict(iteration=r.iteration, partitions=r.partitions, active_nodes=r.active_nodes,
+                  active_edges=r.active_edges, h2d_bytes=r.h2d_bytes, ms=r.ms_iteration)
+             for r in rec[: st.trace_len]]
+    names = [f.name for f in RunStats.__dataclass_fields__.values() if f.name != "trace"]
+    return vals, RunStats(**{k: getattr(st, k) for k in names}, trace=trace)
+
+
+def generate(label1, label2, out_degree):
+    """Subgraph generation on the GPU: (activeNodes, activeNodesPointer[A+1])."""
+    l1 = np.ascontiguousarray(label1, dtype=np.uint8)
+    l2 = np.ascontiguousarray(label2, dtype=np.uint8)
+    deg = np.ascontiguousarray(out_degree, dtype=np.uint32)
+    n = len(deg)
+    an = np.empty(n, dtype=np.uint32)
+    ap = np.empty(n + 1, dtype=np.uint64)
+    cnt = ctypes.c_uint64()
+    check(lib.subway_generate(n, _ptr(l1, _u8p), _ptr(l2, _u8p), _ptr(deg, _u32p), _ptr(an, _u32p),
+                              _ptr(ap, _u64p), ctypes.byref(cnt)))
+    a = int(cnt.value)
+    return an[:a].copy(), ap[: a + 1].copy()
+
+
+def generate_pr(delta, acc, out_degree):
+    d = np.ascontiguousarray(delta, dtype=np.float32)
+    deg = np.ascontiguousarray(out_degree, dtype=np.uint32)
+    n = len(deg)
+    an = np.empty(n, dtype=np.uint32)
+    ap = np.empty(n + 1, dtype=np.uint64)
+    cnt = ctypes.c_uint64()
+    check(lib.subway_generate_pr(n, _ptr(d, _f32p), ctypes.c_float(acc), _ptr(deg, _u32p), _ptr(an, _u32p),
+                                 _ptr(ap, _u64p), ctypes.byref(cnt)))
+    a = int(cnt.value)
+    return an[:a].copy(), ap[: a + 1].copy()
+
+
+def partition(active_nodes_pointer, active_nodes, budget_edges: int):
+    """Device partitioner: rows of (fromNode, nodeCount, fromEdge, edgeCount, firstId, lastId)."""
+    ap = np.ascontiguousarray(active_nodes_pointer, dtype=np.uint64)
+    an = np.ascontiguousarray(active_nodes, dtype=np.uint32)
+    a = len(an)
+    cap = max(16, a + 1)
+    rows = np.zeros((cap, 6), dtype=np.uint64)
+    nr = ctypes.c_uint64()
+    check(lib.subway_partition(a, _ptr(ap, _u64p), _ptr(an, _u32p), budget_edges, cap, _ptr(rows, _u64p),
+                               ctypes.byref(nr)))
+    return rows[: int(nr.value)].copy()
+
+
+def gather(graph: Graph, active_nodes):
+    """GPU gather of the active vertices' adjacency (ends, weights or None)."""
+    an = np.ascontiguousarray(active_nodes, dtype=np.uint32)
+    ne = ctypes.c_uint64()
+    check(lib.subway_gather(graph._h, len(an), _ptr(an, _u32p), None, ctypes.byref(ne)))
+    m = int(ne.value)
+    width = 2 if graph.weighted else 1
+    out = np.empty(m * width, dtype=np.uint32)
+    if m:
+        check(lib.subway_gather(graph._h, len(an), _ptr(an, _u32p), out.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.byref(ne)))
+    if graph.weighted:
+        return out[0::2].copy(), out[1::2].copy()
+    return out, None
+
+
+def h2d_peak(nbytes: int = 256 << 20, reps: int = 10) -> float:
+    """Measured pinned host-to-device copy peak in GB/s (best of reps)."""
+    out = ctypes.c_double()
+    check(lib.subway_h2d_peak(nbytes, reps, ctypes.byref(out)))
+    return float(out.value)

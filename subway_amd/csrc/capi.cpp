@@ -1,0 +1,449 @@
+// capi.cpp -- extern "C" facade over the C++ classes (see subway_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "subway/engine.hpp"
+#include "subway/gpu_error_check.hpp"
+#include "subway/graph.hpp"
+#include "subway_amd.h"
+
+namespace subway {
+template <class E>
+std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
+                                     bool symmetrize, uint32_t wmin, uint32_t wmax);
+}
+
+using namespace subway;
+
+struct subway_graph {
+    bool weighted = false;
+    std::shared_ptr<HostCsr<OutEdge>> u;
+    std::shared_ptr<HostCsr<OutEdgeWeighted>> w;
+    ull n() const { return weighted ? w->n : u->n; }
+    ull m() const { return weighted ? w->m : u->m; }
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int32_t guard(F &&f)
+{
+    try {
+        f();
+        return SUBWAY_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code ? e.code : SUBWAY_ERR_INTERNAL;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return SUBWAY_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return SUBWAY_ERR_INTERNAL;
+    }
+}
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    explicit DevBuf(size_t n)
+    {
+        SUBWAY_CHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+extern "C" {
+
+int32_t subway_version(void) { return 100; }
+
+const char *subway_last_error(void) { return g_last_error.c_str(); }
+
+void subway_default_options(subway_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->algo = SUBWAY_BFS;
+    o->mode = SUBWAY_SYNC;
+    o->num_buffers = 2;
+    o->acc = 0.01f;
+    o->init_pr = 0.15f;
+}
+
+int32_t subway_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int32_t subway_load(const char *path, int32_t weighted, subway_graph **out)
+{
+    return guard([&] {
+        if (!path || !out) throw Error(ERR_ARG, "null argument");
+        std::unique_ptr<subway_graph> g(new subway_graph);
+        g->weighted = weighted != 0;
+        if (g->weighted) g->w = read_csr_file<OutEdgeWeighted>(path);
+        else g->u = read_csr_file<OutEdge>(path);
+        *out = g.release();
+    });
+}
+
+}  // extern "C"
+
+template <class E>
+static std::shared_ptr<HostCsr<E>> csr_from_arrays(uint64_t n, uint64_t m, const uint64_t *ptr, const uint32_t *ends,
+                                                   const uint32_t *weights)
+{
+    auto c = std::make_shared<HostCsr<E>>();
+    c->allocate(n, m);
+    if (ptr[0] != 0 || ptr[n] != m) throw Error(ERR_FORMAT, "node_pointer must start at 0 and end at m");
+    for (uint64_t i = 0; i <= n; i++) c->ptr[i] = ptr[i];
+    for (uint64_t e = 0; e < m; e++) {
+        if (ends[e] >= n) throw Error(ERR_FORMAT, "edge endpoint out of range");
+        c->edges[e].end = ends[e];
+        if constexpr (sizeof(E) == 8) reinterpret_cast<uint32_t *>(&c->edges[e])[1] = weights[e];
+    }
+    c->finalize();
+    return c;
+}
+
+extern "C" {
+
+int32_t subway_graph_from_csr(uint64_t n, uint64_t m, const uint64_t *node_pointer, const uint32_t *ends,
+                              const uint32_t *weights, subway_graph **out)
+{
+    return guard([&] {
+        if (!node_pointer || !out || (m && !ends)) throw Error(ERR_ARG, "null argument");
+        std::unique_ptr<subway_graph> g(new subway_graph);
+        g->weighted = weights != nullptr;
+        if (g->weighted) g->w = csr_from_arrays<OutEdgeWeighted>(n, m, node_pointer, ends, weights);
+        else g->u = csr_from_arrays<OutEdge>(n, m, node_pointer, ends, nullptr);
+        *out = g.release();
+    });
+}
+
+int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                          int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax, subway_graph **out)
+{
+    return guard([&] {
+        if (!out) throw Error(ERR_ARG, "null argument");
+        std::unique_ptr<subway_graph> g(new subway_graph);
+        g->weighted = weighted != 0;
+        if (g->weighted)
+            g->w = rmat_csr<OutEdgeWeighted>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax);
+        else
+            g->u = rmat_csr<OutEdge>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax);
+        *out = g.release();
+    });
+}
+
+int32_t subway_graph_save(const subway_graph *g, const char *path)
+{
+    return guard([&] {
+        if (!g || !path) throw Error(ERR_ARG, "null argument");
+        if (g->weighted) write_csr_file(*g->w, path);
+        else write_csr_file(*g->u, path);
+    });
+}
+
+int32_t subway_graph_info(const subway_graph *g, uint64_t *n, uint64_t *m, int32_t *weighted, uint32_t *max_degree,
+                          uint32_t *max_degree_vertex)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        if (n) *n = g->n();
+        if (m) *m = g->m();
+        if (weighted) *weighted = g->weighted;
+        const uint *deg = g->weighted ? g->w->degree : g->u->degree;
+        uint md = g->weighted ? g->w->maxDegree : g->u->maxDegree;
+        if (max_degree) *max_degree = md;
+        if (max_degree_vertex) {
+            uint64_t at = 0;
+            for (uint64_t i = 0; i < g->n(); i++)
+                if (deg[i] == md) {
+                    at = i;
+                    break;
+                }
+            *max_degree_vertex = (uint32_t)at;
+        }
+    });
+}
+
+int32_t subway_graph_export(const subway_graph *g, uint64_t *node_pointer, uint32_t *ends, uint32_t *weights)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        const ull n = g->n(), m = g->m();
+        const ull *ptr = g->weighted ? g->w->ptr : g->u->ptr;
+        if (node_pointer) std::memcpy(node_pointer, ptr, (n + 1) * sizeof(ull));
+        for (ull e = 0; e < m; e++) {
+            if (g->weighted) {
+                if (ends) ends[e] = g->w->edges[e].end;
+                if (weights) weights[e] = g->w->edges[e].w8;
+            } else {
+                if (ends) ends[e] = g->u->edges[e].end;
+                if (weights) weights[e] = 1;
+            }
+        }
+    });
+}
+
+void subway_graph_free(subway_graph *g) { delete g; }
+
+static RunOptions to_options(const subway_options *o)
+{
+    RunOptions r;
+    r.algo = o->algo;
+    r.mode = o->mode;
+    r.source = o->source;
+    r.budgetBytes = o->budget_bytes;
+    r.numBuffers = o->num_buffers ? o->num_buffers : 2;
+    r.acc = o->acc;
+    r.initPR = o->init_pr;
+    r.transfer = o->transfer;
+    r.policy = o->policy;
+    r.verbose = o->verbose;
+    r.advanceGrid = o->advance_grid;
+    r.profile = o->profile;
+    if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
+    if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
+    return r;
+}
+
+}  // extern "C"
+
+template <class E>
+static RunStats run_on(const std::shared_ptr<HostCsr<E>> &csr, const RunOptions &o, void *values)
+{
+    if (o.algo == PR) {
+        GraphPR<E> g(csr);
+        RunStats s = runPageRank(g, o);
+        if (values) std::memcpy(values, g.value, (size_t)g.num_nodes * sizeof(float));
+        return s;
+    }
+    Graph<E> g(csr);
+    RunStats s = runLabel(g, o);
+    if (values) std::memcpy(values, g.value, (size_t)g.num_nodes * sizeof(uint));
+    return s;
+}
+
+extern "C" {
+
+int32_t subway_run(const subway_graph *g, const subway_options *opt, void *values, subway_stats *stats,
+                   subway_iter_record *trace, uint32_t trace_cap)
+{
+    return guard([&] {
+        if (!g || !opt) throw Error(ERR_ARG, "null argument");
+        RunOptions o = to_options(opt);
+        if ((o.algo == SSSP || o.algo == SSWP) && !g->weighted)
+            throw Error(ERR_ARG, "SSSP/SSWP need a weighted graph");
+        RunStats s = g->weighted ? run_on(g->w, o, values) : run_on(g->u, o, values);
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->ms_total = s.msTotal;
+            stats->ms_transfer = s.msTransfer;
+            stats->ms_advance = s.msAdvance;
+            stats->ms_frontier = s.msFrontier;
+            stats->ms_generate = s.msGenerate;
+            stats->advance_launches = s.advanceLaunches;
+            stats->frontier_launches = s.frontierLaunches;
+            stats->transfer_ops = s.transferOps;
+            stats->work_edges = s.workEdges;
+            stats->work_entries = s.workEntries;
+            stats->h2d_bytes = s.h2dBytes;
+            stats->h2d_dma_bytes = s.h2dDmaBytes;
+            stats->h2d_gather_bytes = s.h2dGatherBytes;
+            stats->inner_rounds = s.innerRounds;
+            stats->partitions = s.partitions;
+            stats->updates = s.updates;
+            stats->global_iterations = s.globalIterations;
+        }
+        uint32_t k = 0;
+        if (trace) {
+            for (const IterRecord &r : s.trace) {
+                if (k >= trace_cap) break;
+                trace[k].iteration = r.iteration;
+                trace[k].partitions = r.partitions;
+                trace[k].active_nodes = r.activeNodes;
+                trace[k].active_edges = r.activeEdges;
+                trace[k].h2d_bytes = r.h2dBytes;
+                trace[k].ms_iteration = r.msIteration;
+                k++;
+            }
+        }
+        if (stats) stats->trace_len = k;
+    });
+}
+
+int32_t subway_generate(uint64_t n, const uint8_t *label1, const uint8_t *label2, const uint32_t *out_degree,
+                        uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active)
+{
+    return guard([&] {
+        if (!n || !label1 || !label2 || !out_degree || !active_nodes || !active_nodes_pointer)
+            throw Error(ERR_ARG, "null argument or empty graph");
+        DevBuf<uint8_t> l1(n), l2(n);
+        DevBuf<uint32_t> deg(n), an(n);
+        DevBuf<ull> ap(n + 1);
+        SUBWAY_CHECK(hipMemcpy(l1.p, label1, n, hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(l2.p, label2, n, hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(deg.p, out_degree, n * 4, hipMemcpyHostToDevice));
+        ScanScratch s;
+        alloc_scan_scratch(s, (n + GEN_TILE - 1) / GEN_TILE);
+        struct F { ScanScratch &s; ~F() { free_scan_scratch(s); } } f{s};
+        launch_generate_labels(n, l1.p, l2.p, deg.p, an.p, ap.p, s, nullptr);
+        ScanCtl ctl;
+        SUBWAY_CHECK(hipMemcpy(&ctl, s.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        SUBWAY_CHECK(hipMemcpy(active_nodes, an.p, ctl.count * 4, hipMemcpyDeviceToHost));
+        SUBWAY_CHECK(hipMemcpy(active_nodes_pointer, ap.p, (ctl.count + 1) * 8, hipMemcpyDeviceToHost));
+        if (num_active) *num_active = ctl.count;
+    });
+}
+
+int32_t subway_generate_pr(uint64_t n, const float *delta, float acc, const uint32_t *out_degree,
+                           uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active)
+{
+    return guard([&] {
+        if (!n || !delta || !out_degree || !active_nodes || !active_nodes_pointer)
+            throw Error(ERR_ARG, "null argument or empty graph");
+        DevBuf<float> dl(n);
+        DevBuf<uint32_t> deg(n), an(n);
+        DevBuf<ull> ap(n + 1);
+        SUBWAY_CHECK(hipMemcpy(dl.p, delta, n * 4, hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(deg.p, out_degree, n * 4, hipMemcpyHostToDevice));
+        ScanScratch s;
+        alloc_scan_scratch(s, (n + GEN_TILE - 1) / GEN_TILE);
+        struct F { ScanScratch &s; ~F() { free_scan_scratch(s); } } f{s};
+        launch_generate_pr(n, dl.p, acc, deg.p, an.p, ap.p, s, nullptr);
+        ScanCtl ctl;
+        SUBWAY_CHECK(hipMemcpy(&ctl, s.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        SUBWAY_CHECK(hipMemcpy(active_nodes, an.p, ctl.count * 4, hipMemcpyDeviceToHost));
+        SUBWAY_CHECK(hipMemcpy(active_nodes_pointer, ap.p, (ctl.count + 1) * 8, hipMemcpyDeviceToHost));
+        if (num_active) *num_active = ctl.count;
+    });
+}
+
+int32_t subway_partition(uint64_t num_active, const uint64_t *active_nodes_pointer, const uint32_t *active_nodes,
+                         uint64_t budget_edges, uint64_t max_rows, uint64_t *rows, uint64_t *num_rows)
+{
+    return guard([&] {
+        if (!active_nodes_pointer || !active_nodes || !rows || !num_rows) throw Error(ERR_ARG, "null argument");
+        if (!budget_edges) throw Error(ERR_BUDGET, "budget of zero edges");
+        *num_rows = 0;
+        if (!num_active) return;
+        DevBuf<ull> ap(num_active + 1);
+        DevBuf<uint32_t> an(num_active);
+        SUBWAY_CHECK(hipMemcpy(ap.p, active_nodes_pointer, (num_active + 1) * 8, hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(an.p, active_nodes, num_active * 4, hipMemcpyHostToDevice));
+        const uint32_t chunk = 4096;
+        DevBuf<PartRow> dr(chunk);
+        DevBuf<ull> dout(4);
+        std::vector<PartRow> hr(chunk);
+        uint64_t start = 0, total = 0;
+        for (;;) {
+            launch_partition(ap.p, an.p, nullptr, num_active, budget_edges, start, chunk, dr.p, dout.p, nullptr);
+            ull out[4];
+            SUBWAY_CHECK(hipMemcpy(out, dout.p, sizeof(out), hipMemcpyDeviceToHost));
+            SUBWAY_CHECK(hipMemcpy(hr.data(), dr.p, out[0] * sizeof(PartRow), hipMemcpyDeviceToHost));
+            for (uint64_t r = 0; r < out[0]; r++, total++) {
+                if (total < max_rows) {
+                    uint64_t *o = rows + 6 * total;
+                    o[0] = hr[r].fromNode;
+                    o[1] = hr[r].nodeCount;
+                    o[2] = hr[r].fromEdge;
+                    o[3] = hr[r].edgeCount;
+                    o[4] = hr[r].firstId;
+                    o[5] = hr[r].lastId;
+                }
+            }
+            if (out[0] < chunk) break;
+            start = out[1];
+        }
+        *num_rows = total;
+    });
+}
+
+}  // extern "C"
+
+template <class E>
+static void gather_on(const HostCsr<E> &csr, uint64_t A, const uint32_t *nodes, void *out, uint64_t *num_edges)
+{
+    std::vector<ull> ptr(A + 1);
+    uint64_t run = 0;
+    for (uint64_t i = 0; i < A; i++) {
+        if (nodes[i] >= csr.n) throw Error(ERR_ARG, "active vertex out of range");
+        if (i && nodes[i] <= nodes[i - 1]) throw Error(ERR_ARG, "active list must be strictly ascending");
+        ptr[i] = run;
+        run += csr.degree[nodes[i]];
+    }
+    ptr[A] = run;
+    if (num_edges) *num_edges = run;
+    if (!run || !out) return;
+    DevBuf<ull> dp(A + 1), np(csr.n + 1);
+    DevBuf<uint32_t> dn(A);
+    DevBuf<E> dst(run);
+    SUBWAY_CHECK(hipMemcpy(dp.p, ptr.data(), (A + 1) * 8, hipMemcpyHostToDevice));
+    SUBWAY_CHECK(hipMemcpy(dn.p, nodes, A * 4, hipMemcpyHostToDevice));
+    SUBWAY_CHECK(hipMemcpy(np.p, csr.ptr, (csr.n + 1) * 8, hipMemcpyHostToDevice));
+    launch_gather((int)sizeof(E), dn.p, dp.p, 0, A, np.p, csr.d_edges, dst.p, 0, run, nullptr);
+    SUBWAY_CHECK(hipDeviceSynchronize());
+    SUBWAY_CHECK(hipMemcpy(out, dst.p, run * sizeof(E), hipMemcpyDeviceToHost));
+}
+
+extern "C" {
+
+int32_t subway_gather(const subway_graph *g, uint64_t num_active, const uint32_t *active_nodes, void *out,
+                      uint64_t *num_edges)
+{
+    return guard([&] {
+        if (!g || (num_active && !active_nodes)) throw Error(ERR_ARG, "null argument");
+        if (!num_active) {
+            if (num_edges) *num_edges = 0;
+            return;
+        }
+        if (g->weighted) gather_on(*g->w, num_active, active_nodes, out, num_edges);
+        else gather_on(*g->u, num_active, active_nodes, out, num_edges);
+    });
+}
+
+int32_t subway_h2d_peak(uint64_t bytes, int32_t reps, double *gbps)
+{
+    return guard([&] {
+        if (!bytes || reps < 1 || !gbps) throw Error(ERR_ARG, "bad argument");
+        void *h = nullptr, *d = nullptr;
+        SUBWAY_CHECK(hipHostMalloc(&h, bytes, 0));
+        struct Free { void *&h, *&d; ~Free() { if (h) (void)hipHostFree(h); if (d) (void)hipFree(d); } } fr{h, d};
+        SUBWAY_CHECK(hipMalloc(&d, bytes));
+        std::memset(h, 1, bytes);
+        hipStream_t s;
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t a, b;
+        SUBWAY_CHECK(hipEventCreate(&a));
+        SUBWAY_CHECK(hipEventCreate(&b));
+        float best = 1e30f;
+        for (int r = 0; r < reps; r++) {
+            SUBWAY_CHECK(hipEventRecord(a, s));
+            SUBWAY_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+            SUBWAY_CHECK(hipEventRecord(b, s));
+            SUBWAY_CHECK(hipEventSynchronize(b));
+            float ms = 0;
+            SUBWAY_CHECK(hipEventElapsedTime(&ms, a, b));
+            best = std::min(best, ms);
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipStreamDestroy(s);
+        *gbps = (double)bytes / (best * 1e-3) / 1e9;
+    });
+}
+
+}  // extern "C"

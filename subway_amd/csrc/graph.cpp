@@ -1,0 +1,462 @@
+// graph.cpp -- CSR storage, the on-disk formats and the Graph / GraphPR
+// classes (reference: shared/graph.cu, tools/converter.cpp).
+//
+// Formats (extension decides, as in shared/graph.cu:36-52):
+//   .bcsr / .bwcsr     [u32 n][u32 m][u32 nodePointer x n][E x m]
+//                      (shared/graph.cu:39-51, tools/converter.cpp:88-93)
+//   .bcsr64 / .bwcsr64 [u64 n][u64 m][u64 nodePointer x n][E x m]
+//                      64-bit variant for m >= 2^32 (same layout, wider ints)
+//   .el / .wel         text, one "source end [w8]" per line; n = max id + 1;
+//                      stable counting sort by source (shared/graph.cu:53-170)
+// OutEdge is {u32 end}; OutEdgeWeighted is {u32 end, u32 w8}.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <thread>
+#include <vector>
+
+#include "subway/globals.hpp"
+#include "subway/gpu_error_check.hpp"
+#include "subway/graph.hpp"
+#include "host_util.hpp"
+
+namespace subway {
+
+template <class E>
+constexpr bool is_weighted_edge() { return sizeof(E) == 8; }
+
+std::string file_extension(const std::string &fileName)
+{
+    size_t p = fileName.find_last_of('.');
+    if (p != std::string::npos) return fileName.substr(p + 1);
+    return "";
+}
+
+template <class E>
+HostCsr<E>::~HostCsr()
+{
+    delete[] ptr;
+    delete[] degree;
+    if (edges) {
+        if (pinned) (void)hipHostFree(edges);
+        else std::free(edges);
+    }
+}
+
+template <class E>
+void HostCsr<E>::allocate(ull n_, ull m_)
+{
+    if (n_ == 0) throw Error(ERR_FORMAT, "graph has no vertices");
+    if (n_ > 0xFFFFFFFFull) throw Error(ERR_FORMAT, "vertex ids are 32-bit; graph has too many vertices");
+    n = n_;
+    m = m_;
+    try {
+        ptr = new ull[n + 1];
+        degree = new uint[n];
+    } catch (const std::bad_alloc &) {
+        throw Error(ERR_NOMEM, "host allocation of the CSR offsets failed");
+    }
+    size_t bytes = (size_t)(m ? m : 1) * sizeof(E);
+    hipError_t e = hipHostMalloc((void **)&edges, bytes, hipHostMallocMapped | hipHostMallocPortable);
+    if (e != hipSuccess) {
+        edges = nullptr;
+        int devices = 0;
+        if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+            // No GPU in this process: the CSR can still be built, read and
+            // written (file conversion, generation); running the pipeline on
+            // it is refused because nothing is mapped for the device.
+            (void)hipGetLastError();
+            edges = static_cast<E *>(std::malloc(bytes));
+            if (!edges) throw Error(ERR_NOMEM, "host allocation of the edge list failed");
+            pinned = false;
+            return;
+        }
+        throw Error(ERR_NOMEM, "pinned allocation of " + std::to_string(bytes) +
+                                   " bytes for the edge list failed: " + hipGetErrorString(e));
+    }
+    pinned = true;
+}
+
+template <class E>
+void HostCsr<E>::finalize()
+{
+    ptr[n] = m;
+    uint mx = 0;
+    for (ull i = 0; i < n; i++) {
+        if (ptr[i + 1] < ptr[i]) throw Error(ERR_FORMAT, "node pointers are not non-decreasing");
+        ull d = ptr[i + 1] - ptr[i];
+        if (d > 0xFFFFFFFFull) throw Error(ERR_FORMAT, "a vertex has 2^32 or more edges");
+        degree[i] = (uint)d;
+        mx = std::max(mx, (uint)d);
+    }
+    maxDegree = mx;
+    large = large || (m > 0xFFFFFFFFull);
+    if (pinned) SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_edges, edges, 0));
+}
+
+// ------------------------------------------------------------ binary I/O
+static void read_exact(FILE *f, void *dst, size_t bytes, const std::string &path)
+{
+    char *p = (char *)dst;
+    while (bytes) {
+        size_t chunk = std::min(bytes, (size_t)1 << 30);
+        size_t got = fread(p, 1, chunk, f);
+        if (got != chunk) throw Error(ERR_IO, "truncated graph file: " + path);
+        p += got;
+        bytes -= got;
+    }
+}
+
+static void write_exact(FILE *f, const void *src, size_t bytes, const std::string &path)
+{
+    const char *p = (const char *)src;
+    while (bytes) {
+        size_t chunk = std::min(bytes, (size_t)1 << 30);
+        if (fwrite(p, 1, chunk, f) != chunk) throw Error(ERR_IO, "write failed: " + path);
+        p += chunk;
+        bytes -= chunk;
+    }
+}
+
+static ull file_size(FILE *f)
+{
+    fseeko(f, 0, SEEK_END);
+    ull s = (ull)ftello(f);
+    fseeko(f, 0, SEEK_SET);
+    return s;
+}
+
+template <class E>
+static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) throw Error(ERR_IO, "cannot open graph file: " + path + " (" + strerror(errno) + ")");
+    struct Closer { FILE *f; ~Closer() { fclose(f); } } closer{f};
+    ull size = file_size(f);
+    ull n, m;
+    size_t hdr = wide ? 16 : 8;
+    if (size < hdr) throw Error(ERR_IO, "truncated graph file header: " + path);
+    if (wide) {
+        uint64_t h[2];
+        read_exact(f, h, 16, path);
+        n = h[0];
+        m = h[1];
+    } else {
+        uint32_t h[2];
+        read_exact(f, h, 8, path);
+        n = h[0];
+        m = h[1];
+    }
+    ull ptrBytes = n * (wide ? 8ull : 4ull);
+    ull expect = hdr + ptrBytes + m * sizeof(E);
+    if (size != expect)
+        throw Error(ERR_FORMAT, "graph file size " + std::to_string(size) + " does not match header (n=" +
+                                    std::to_string(n) + ", m=" + std::to_string(m) + ", expected " +
+                                    std::to_string(expect) + " bytes): " + path);
+    csr.allocate(n, m);
+    csr.large = wide;
+    if (wide) {
+        read_exact(f, csr.ptr, n * 8, path);
+    } else {
+        std::vector<uint32_t> p32(n);
+        read_exact(f, p32.data(), n * 4, path);
+        for (ull i = 0; i < n; i++) csr.ptr[i] = p32[i];
+    }
+    read_exact(f, csr.edges, m * sizeof(E), path);
+    for (ull i = 0; i < n; i++)
+        if (csr.ptr[i] > m) throw Error(ERR_FORMAT, "node pointer beyond edge count: " + path);
+    csr.finalize();
+    for (ull e = 0; e < m; e++)
+        if (csr.edges[e].end >= n) throw Error(ERR_FORMAT, "edge endpoint out of range: " + path);
+}
+
+// ------------------------------------------------------------- text I/O
+static bool parse_uint(const char *&p, const char *end, uint64_t &out)
+{
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',')) p++;
+    if (p >= end || *p < '0' || *p > '9') return false;
+    uint64_t v = 0;
+    while (p < end && *p >= '0' && *p <= '9') {
+        v = v * 10 + (uint64_t)(*p - '0');
+        if (v > 0xFFFFFFFFull) return false;
+        p++;
+    }
+    out = v;
+    return true;
+}
+
+template <class E>
+static void read_text(HostCsr<E> &csr, const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw Error(ERR_IO, "cannot open graph file: " + path);
+    std::vector<uint32_t> src, dst, wt;
+    std::string line;
+    uint64_t mx = 0;
+    bool any = false;
+    unsigned long long lineNo = 0;
+    while (std::getline(in, line)) {
+        lineNo++;
+        const char *p = line.data(), *end = p + line.size();
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) p++;
+        if (p == end || *p == '#' || *p == '%') continue;  // blank or comment line
+        uint64_t s, d, w = 0;
+        if (!parse_uint(p, end, s) || !parse_uint(p, end, d))
+            throw Error(ERR_FORMAT, path + ":" + std::to_string(lineNo) + ": expected 'source end'");
+        if (is_weighted_edge<E>() && !parse_uint(p, end, w))
+            throw Error(ERR_FORMAT, path + ":" + std::to_string(lineNo) + ": expected a weight");
+        src.push_back((uint32_t)s);
+        dst.push_back((uint32_t)d);
+        if (is_weighted_edge<E>()) wt.push_back((uint32_t)w);
+        mx = std::max(mx, std::max(s, d));
+        any = true;
+    }
+    if (!any) throw Error(ERR_FORMAT, "graph file has no edges: " + path);
+    ull n = mx + 1, m = src.size();
+    csr.allocate(n, m);
+    std::vector<ull> cursor(n, 0);
+    for (ull e = 0; e < m; e++) cursor[src[e]]++;
+    ull run = 0;
+    for (ull v = 0; v < n; v++) {
+        csr.ptr[v] = run;
+        ull d = cursor[v];
+        cursor[v] = run;
+        run += d;
+    }
+    for (ull e = 0; e < m; e++) {
+        ull at = cursor[src[e]]++;
+        csr.edges[at].end = dst[e];
+        if constexpr (sizeof(E) == 8) reinterpret_cast<uint32_t *>(&csr.edges[at])[1] = wt[e];
+    }
+    csr.finalize();
+}
+
+template <class E>
+std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path)
+{
+    auto csr = std::make_shared<HostCsr<E>>();
+    std::string ext = file_extension(path);
+    bool weighted = is_weighted_edge<E>();
+    if (ext == "bcsr" || ext == "bcsr64") {
+        if (weighted) throw Error(ERR_FORMAT, "a weighted graph needs a .bwcsr file, got " + path);
+        read_binary(*csr, path, ext == "bcsr64");
+    } else if (ext == "bwcsr" || ext == "bwcsr64") {
+        if (!weighted) throw Error(ERR_FORMAT, "an unweighted graph needs a .bcsr file, got " + path);
+        read_binary(*csr, path, ext == "bwcsr64");
+    } else if (ext == "el" || ext == "wel") {
+        read_text(*csr, path);
+    } else {
+        throw Error(ERR_FORMAT, "The graph format is not supported: " + path);
+    }
+    return csr;
+}
+
+template <class E>
+void write_csr_file(const HostCsr<E> &csr, const std::string &path)
+{
+    std::string ext = file_extension(path);
+    bool wide;
+    if (ext == "bcsr" || ext == "bwcsr") wide = false;
+    else if (ext == "bcsr64" || ext == "bwcsr64") wide = true;
+    else throw Error(ERR_FORMAT, "output must be .bcsr/.bwcsr or the 64-bit variants: " + path);
+    bool wantWeighted = (ext == "bwcsr" || ext == "bwcsr64");
+    if (wantWeighted != is_weighted_edge<E>())
+        throw Error(ERR_FORMAT, "file extension does not match the edge type: " + path);
+    if (!wide && (csr.m > 0xFFFFFFFFull || csr.n > 0xFFFFFFFFull))
+        throw Error(ERR_FORMAT, "graph too large for the 32-bit header; use the 64-bit variant");
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) throw Error(ERR_IO, "cannot create " + path);
+    struct Closer { FILE *f; ~Closer() { fclose(f); } } closer{f};
+    if (wide) {
+        uint64_t h[2] = {csr.n, csr.m};
+        write_exact(f, h, 16, path);
+        write_exact(f, csr.ptr, csr.n * 8, path);
+    } else {
+        uint32_t h[2] = {(uint32_t)csr.n, (uint32_t)csr.m};
+        write_exact(f, h, 8, path);
+        std::vector<uint32_t> p32(csr.n);
+        for (ull i = 0; i < csr.n; i++) p32[i] = (uint32_t)csr.ptr[i];
+        write_exact(f, p32.data(), csr.n * 4, path);
+    }
+    write_exact(f, csr.edges, csr.m * sizeof(E), path);
+}
+
+template struct HostCsr<OutEdge>;
+template struct HostCsr<OutEdgeWeighted>;
+template std::shared_ptr<HostCsr<OutEdge>> read_csr_file<OutEdge>(const std::string &);
+template std::shared_ptr<HostCsr<OutEdgeWeighted>> read_csr_file<OutEdgeWeighted>(const std::string &);
+template void write_csr_file<OutEdge>(const HostCsr<OutEdge> &, const std::string &);
+template void write_csr_file<OutEdgeWeighted>(const HostCsr<OutEdgeWeighted> &, const std::string &);
+
+}  // namespace subway
+
+// =================================================================== Graph
+using subway::HostCsr;
+
+template <class E>
+Graph<E>::Graph(std::string path, bool weighted)
+    : graphFilePath(std::move(path)), isWeighted(weighted), isLarge(false), num_nodes(0), num_edges(0),
+      nodePointer(nullptr), edgeList(nullptr), outDegree(nullptr), label1(nullptr), label2(nullptr),
+      value(nullptr), d_outDegree(nullptr), d_value(nullptr), d_label1(nullptr), d_label2(nullptr),
+      d_nodePointer(nullptr), d_edgeList(nullptr)
+{
+}
+
+template <class E>
+Graph<E>::Graph(std::shared_ptr<HostCsr<E>> c) : Graph<E>(std::string(), sizeof(E) == 8)
+{
+    csr = std::move(c);
+    graphFormat = "memory";
+    attach();
+}
+
+template <class E>
+Graph<E>::~Graph()
+{
+    delete[] label1;
+    delete[] label2;
+    delete[] value;
+    if (d_outDegree) (void)hipFree(d_outDegree);
+    if (d_value) (void)hipFree(d_value);
+    if (d_label1) (void)hipFree(d_label1);
+    if (d_label2) (void)hipFree(d_label2);
+    if (d_nodePointer) (void)hipFree(d_nodePointer);
+}
+
+template <class E>
+std::string Graph<E>::GetFileExtension(std::string fileName)
+{
+    return subway::file_extension(fileName);
+}
+
+template <class E>
+void Graph<E>::AssignW8(uint w8, uint index)
+{
+    if constexpr (sizeof(E) == 8) reinterpret_cast<uint *>(&edgeList[index])[1] = w8;
+    (void)w8;
+    (void)index;
+}
+
+template <class E>
+void Graph<E>::ReadGraph()
+{
+    std::cout << "Reading the input graph from the following file:\n>> " << graphFilePath << std::endl;
+    graphFormat = GetFileExtension(graphFilePath);
+    csr = subway::read_csr_file<E>(graphFilePath);
+    attach();
+    std::cout << "Done reading.\n";
+    std::cout << "Number of nodes = " << num_nodes << std::endl;
+    std::cout << "Number of edges = " << num_edges << std::endl;
+}
+
+template <class E>
+void Graph<E>::attach()
+{
+    if (!csr->pinned) throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
+    num_nodes = (uint)csr->n;
+    num_edges = csr->m;
+    isLarge = csr->large;
+    nodePointer = csr->ptr;
+    edgeList = csr->edges;
+    outDegree = csr->degree;
+    d_edgeList = csr->d_edges;
+    label1 = new bool[num_nodes];
+    label2 = new bool[num_nodes];
+    value = new uint[num_nodes];
+    SUBWAY_CHECK(hipMalloc(&d_outDegree, (size_t)num_nodes * sizeof(uint)));
+    SUBWAY_CHECK(hipMalloc(&d_value, (size_t)num_nodes * sizeof(uint)));
+    SUBWAY_CHECK(hipMalloc(&d_label1, (size_t)num_nodes * sizeof(bool)));
+    SUBWAY_CHECK(hipMalloc(&d_label2, (size_t)num_nodes * sizeof(bool)));
+    SUBWAY_CHECK(hipMalloc(&d_nodePointer, ((size_t)num_nodes + 1) * sizeof(ull)));
+    SUBWAY_CHECK(hipMemcpy(d_nodePointer, nodePointer, ((size_t)num_nodes + 1) * sizeof(ull),
+                           hipMemcpyHostToDevice));
+    SUBWAY_CHECK(hipMemcpy(d_outDegree, outDegree, (size_t)num_nodes * sizeof(uint), hipMemcpyHostToDevice));
+}
+
+// ================================================================= GraphPR
+template <class E>
+GraphPR<E>::GraphPR(std::string path, bool weighted)
+    : graphFilePath(std::move(path)), isWeighted(weighted), isLarge(false), num_nodes(0), num_edges(0),
+      nodePointer(nullptr), edgeList(nullptr), outDegree(nullptr), value(nullptr), delta(nullptr),
+      d_outDegree(nullptr), d_value(nullptr), d_delta(nullptr), d_nodePointer(nullptr), d_edgeList(nullptr)
+{
+}
+
+template <class E>
+GraphPR<E>::GraphPR(std::shared_ptr<HostCsr<E>> c) : GraphPR<E>(std::string(), sizeof(E) == 8)
+{
+    csr = std::move(c);
+    graphFormat = "memory";
+    attach();
+}
+
+template <class E>
+GraphPR<E>::~GraphPR()
+{
+    delete[] value;
+    delete[] delta;
+    if (d_outDegree) (void)hipFree(d_outDegree);
+    if (d_value) (void)hipFree(d_value);
+    if (d_delta) (void)hipFree(d_delta);
+    if (d_nodePointer) (void)hipFree(d_nodePointer);
+}
+
+template <class E>
+std::string GraphPR<E>::GetFileExtension(std::string fileName)
+{
+    return subway::file_extension(fileName);
+}
+
+template <class E>
+void GraphPR<E>::AssignW8(uint w8, uint index)
+{
+    if constexpr (sizeof(E) == 8) reinterpret_cast<uint *>(&edgeList[index])[1] = w8;
+    (void)w8;
+    (void)index;
+}
+
+template <class E>
+void GraphPR<E>::ReadGraph()
+{
+    std::cout << "Reading the input graph from the following file:\n>> " << graphFilePath << std::endl;
+    graphFormat = GetFileExtension(graphFilePath);
+    csr = subway::read_csr_file<E>(graphFilePath);
+    attach();
+    std::cout << "Done reading.\n";
+    std::cout << "Number of nodes = " << num_nodes << std::endl;
+    std::cout << "Number of edges = " << num_edges << std::endl;
+}
+
+template <class E>
+void GraphPR<E>::attach()
+{
+    if (!csr->pinned) throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
+    num_nodes = (uint)csr->n;
+    num_edges = csr->m;
+    isLarge = csr->large;
+    nodePointer = csr->ptr;
+    edgeList = csr->edges;
+    outDegree = csr->degree;
+    d_edgeList = csr->d_edges;
+    value = new float[num_nodes];
+    delta = new float[num_nodes];
+    SUBWAY_CHECK(hipMalloc(&d_outDegree, (size_t)num_nodes * sizeof(uint)));
+    SUBWAY_CHECK(hipMalloc(&d_value, (size_t)num_nodes * sizeof(float)));
+    SUBWAY_CHECK(hipMalloc(&d_delta, (size_t)num_nodes * sizeof(float)));
+    SUBWAY_CHECK(hipMalloc(&d_nodePointer, ((size_t)num_nodes + 1) * sizeof(ull)));
+    SUBWAY_CHECK(hipMemcpy(d_nodePointer, nodePointer, ((size_t)num_nodes + 1) * sizeof(ull),
+                           hipMemcpyHostToDevice));
+    SUBWAY_CHECK(hipMemcpy(d_outDegree, outDegree, (size_t)num_nodes * sizeof(uint), hipMemcpyHostToDevice));
+}
+
+// shared/graph.cu:397-401
+template class Graph<OutEdge>;
+template class Graph<OutEdgeWeighted>;
+template class GraphPR<OutEdge>;
+template class GraphPR<OutEdgeWeighted>;

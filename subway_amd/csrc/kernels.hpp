@@ -1,0 +1,131 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels that make up the
+// out-of-memory pipeline.  Every launcher enqueues on the given stream and
+// returns immediately; shapes are validated on the host before launch.
+#ifndef SUBWAY_KERNELS_HPP
+#define SUBWAY_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "scan_types.hpp"
+#include "subway/globals.hpp"
+
+namespace subway {
+
+using dev::ScanCtl;
+using dev::TileDesc;
+
+enum Algo { ALGO_BFS = 0, ALGO_CC = 1, ALGO_SSSP = 2, ALGO_SSWP = 3, ALGO_PR = 4 };
+
+// Select-scan tiling (generator and frontier kernels).
+constexpr int SCAN_BLOCK = 256;
+constexpr int GEN_IPT = 16;                            // vertices per thread
+constexpr int GEN_TILE = SCAN_BLOCK * GEN_IPT;         // 4096 vertices per tile
+constexpr int FR_IPT = 8;
+constexpr int FR_TILE = SCAN_BLOCK * FR_IPT;           // 2048 active-list items per tile
+// Advance kernel: work space is cut into chunks of ADV_CHUNK edges.
+constexpr int ADV_BLOCK = 256;
+constexpr int ADV_CHUNK = 2048;
+// Gather kernel: destination chunk per workgroup.
+constexpr int GATHER_BLOCK = 256;
+constexpr int GATHER_CHUNK = 4096;
+
+// One worklist entry per selected vertex: where its edges sit in the device
+// buffer, where its edges start in work space, and the value it pushes
+// (uint32 value, or float bits of the PageRank share).
+struct Entry {
+    uint32_t edge_local;
+    uint32_t prefix;
+    uint32_t val;
+    uint32_t pad;
+};
+
+// One partition row as produced by the device partitioner.
+struct PartRow {
+    ull fromNode;
+    ull nodeCount;
+    ull fromEdge;
+    ull edgeCount;
+    ull firstId;
+    ull lastId;
+};
+
+// Scratch for one select-scan kernel family.
+struct ScanScratch {
+    ScanCtl *ctl = nullptr;
+    TileDesc td{};
+    ull maxTiles = 0;
+};
+
+void alloc_scan_scratch(ScanScratch &s, ull maxTiles);
+void free_scan_scratch(ScanScratch &s);
+
+// ---- subgraph generator (flags -> scan -> compaction, one pass)
+void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
+                            const uint32_t *outDegree, uint32_t *activeNodes,
+                            ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
+void launch_generate_pr(ull n, const float *delta, float acc,
+                        const uint32_t *outDegree, uint32_t *activeNodes,
+                        ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
+
+// ---- partitioner: greedy contiguous ranges of the active list
+// Writes at most maxRows rows starting at active index `start`; out[0] =
+// rows written, out[1] = next start, out[2] = active count, out[3] = active
+// edge total.
+// The active count is read from dNumActive when it is non-null (so the
+// launch can follow the generator without a host round trip).
+void launch_partition(const ull *activeNodesPointer, const uint32_t *activeNodes,
+                      const ull *dNumActive, ull numActive, ull budgetEdges,
+                      ull start, uint32_t maxRows, PartRow *rows, ull *out,
+                      hipStream_t st);
+
+// ---- zero-copy gather of a partition's adjacency from mapped host memory
+void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *activeNodesPointer,
+                   ull kbeg, ull kend, const ull *nodePointer,
+                   const void *hostEdges, void *dst, ull edgeBase, ull edgeCount,
+                   hipStream_t st);
+
+// ---- per-round frontier (filter) kernel
+// Label algorithms: an item is selected when labA (or labB) is set; both are
+// cleared.  PR: selected when delta > acc; value absorbs delta.
+struct FrontierArgs {
+    int algo;
+    ull n;                      // items in the partition
+    const uint32_t *nodes;           // activeNodes + fromNode
+    const ull *ptr;             // activeNodesPointer + fromNode
+    ull fromEdge;               // subtracted from ptr to get buffer offsets
+    uint8_t *labA;
+    uint8_t *labB;                   // may be null
+    uint32_t *value;                 // u32 value (label algos) or float bits (PR)
+    float *delta;                    // PR only
+    float acc;                       // PR only
+    Entry *entries;
+    uint32_t *chunkMap;
+    int first;                       // 1: ignore the stop flag of the previous launch
+};
+void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
+
+// ---- per-round advance (edge relaxation) kernel
+struct AdvanceArgs {
+    int algo;
+    int edgeBytes;                   // 4 (OutEdge) or 8 (OutEdgeWeighted)
+    const Entry *entries;
+    const uint32_t *chunkMap;
+    const void *edges;               // device edge buffer
+    uint32_t *value;                 // u32 values; PR: unused
+    uint8_t *labelOut;               // label algos
+    float *delta;                    // PR
+    ull w0, w1;                 // work window [w0, w1); w1 = ~0 for all
+    ull shift;                  // subtracted from edge indices
+    int grid;                        // persistent workgroups
+};
+void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st);
+
+// Fill helpers (kernels rather than memsets so they can sit in graphs).
+void launch_fill_u8(uint8_t *p, uint8_t v, ull n, hipStream_t st);
+void launch_fill_u32(uint32_t *p, uint32_t v, ull n, hipStream_t st);
+
+}  // namespace subway
+
+#endif  // SUBWAY_KERNELS_HPP

@@ -1,0 +1,79 @@
+// subgraph_generator.cpp -- host side of SubgraphGenerator<E>
+// (reference: shared/subgraph_generator.cu:100-285).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "subway/gpu_error_check.hpp"
+#include "subway/subgraph_generator.hpp"
+
+using namespace subway;
+
+template <class E>
+SubgraphGenerator<E>::SubgraphGenerator(Graph<E> &graph) : scratch(new ScanScratch), h_totals(nullptr)
+{
+    alloc_scan_scratch(*scratch, ((ull)graph.num_nodes + GEN_TILE - 1) / GEN_TILE);
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 2 * sizeof(ull), 0));
+}
+
+template <class E>
+SubgraphGenerator<E>::SubgraphGenerator(GraphPR<E> &graph) : scratch(new ScanScratch), h_totals(nullptr)
+{
+    alloc_scan_scratch(*scratch, ((ull)graph.num_nodes + GEN_TILE - 1) / GEN_TILE);
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 2 * sizeof(ull), 0));
+}
+
+template <class E>
+SubgraphGenerator<E>::~SubgraphGenerator()
+{
+    free_scan_scratch(*scratch);
+    delete scratch;
+    if (h_totals) (void)hipHostFree(h_totals);
+}
+
+template <class E>
+const unsigned long long *SubgraphGenerator<E>::deviceTotals() const
+{
+    return (const unsigned long long *)&scratch->ctl->count;
+}
+
+template <class E>
+void SubgraphGenerator<E>::enqueue(Graph<E> &graph, Subgraph<E> &subgraph)
+{
+    launch_generate_labels(graph.num_nodes, (const uint8_t *)graph.d_label1, (const uint8_t *)graph.d_label2,
+                           graph.d_outDegree, subgraph.d_activeNodes, subgraph.d_activeNodesPointer,
+                           *scratch, stream);
+}
+
+template <class E>
+void SubgraphGenerator<E>::enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc)
+{
+    launch_generate_pr(graph.num_nodes, graph.d_delta, acc, graph.d_outDegree, subgraph.d_activeNodes,
+                       subgraph.d_activeNodesPointer, *scratch, stream);
+}
+
+template <class E>
+void SubgraphGenerator<E>::readback(Subgraph<E> &subgraph)
+{
+    SUBWAY_CHECK(hipMemcpyAsync(h_totals, &scratch->ctl->count, 2 * sizeof(ull), hipMemcpyDeviceToHost, stream));
+    SUBWAY_CHECK(hipStreamSynchronize(stream));
+    subgraph.numActiveNodes = (uint)h_totals[0];
+    subgraph.numActiveEdges = h_totals[1];
+}
+
+template <class E>
+void SubgraphGenerator<E>::generate(Graph<E> &graph, Subgraph<E> &subgraph)
+{
+    enqueue(graph, subgraph);
+    readback(subgraph);
+}
+
+template <class E>
+void SubgraphGenerator<E>::generate(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc)
+{
+    enqueue(graph, subgraph, acc);
+    readback(subgraph);
+}
+
+// shared/subgraph_generator.cu:287-288
+template class SubgraphGenerator<OutEdge>;
+template class SubgraphGenerator<OutEdgeWeighted>;

@@ -1,0 +1,218 @@
+"""GPU parity: every stage and every application against the CPU oracle,
+through the C ABI.  Integer results are bit-exact; PageRank (FP32) stays
+within 1e-4 max relative error of the FP64 oracle at the same threshold."""
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+PR_RTOL = 1e-4
+
+
+def hand():
+    with open(os.path.join(HERE, "golden", "hand_graphs.json")) as f:
+        return json.load(f)
+
+
+def graph_of(sw, n, edges, weighted):
+    src = np.array([e[0] for e in edges], np.uint32)
+    dst = np.array([e[1] for e in edges], np.uint32)
+    w = np.array([e[2] for e in edges], np.uint32) if weighted else None
+    return sw.Graph.from_edges(n, src, dst, w)
+
+
+def random_graph(sw, rng, n, m, weighted=True, hub=None):
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    if hub is not None:
+        v, d = hub
+        src = np.concatenate([src, np.full(d, v, np.uint32)])
+        dst = np.concatenate([dst, rng.integers(0, n, d).astype(np.uint32)])
+    w = rng.integers(1, 65, len(src)).astype(np.uint32) if weighted else None
+    return sw.Graph.from_edges(n, src, dst, w)
+
+
+def check_app(sw, g, algo, mode, src=0, **kw):
+    vals, st = sw.run(g, algo, mode=mode, source=src, **kw)
+    ptr, adj, w = g.csr()
+    if algo == "pr":
+        acc = kw.get("threshold", 0.01)
+        exp, _ = O.pr(ptr, adj, acc)
+        rel = np.abs(vals.astype(np.float64) - exp) / np.maximum(exp, 1e-30)
+        assert rel.max() < PR_RTOL, (algo, mode, float(rel.max()))
+    else:
+        exp = O.expected(algo, ptr, adj, w, src)
+        bad = np.nonzero(vals != exp)[0]
+        assert bad.size == 0, (algo, mode, kw, bad[:10], vals[bad[:10]], exp[bad[:10]])
+    return vals, st
+
+
+# ------------------------------------------------------------- generator
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 4095, 4096, 4097, 100_000, 1_048_583])
+@pytest.mark.parametrize("density", [0.0, 0.003, 0.5, 1.0])
+def test_generate_matches_oracle(sw, gpu, n, density):
+    rng = np.random.default_rng(n + int(density * 1000))
+    l1 = (rng.random(n) < density / 2).astype(np.uint8)
+    l2 = (rng.random(n) < density / 2).astype(np.uint8)
+    if density == 1.0:
+        l1[:] = 1
+    deg = rng.integers(0, 1000, n).astype(np.uint32)
+    an, ap = sw.generate(l1, l2, deg)
+    ean, eap = O.generate(l1 | l2, deg)
+    assert (an == ean).all() and (ap == eap).all()
+
+
+def test_generate_pr_matches_oracle(sw, gpu):
+    rng = np.random.default_rng(5)
+    n = 300_001
+    delta = rng.random(n).astype(np.float32) * 0.02
+    deg = rng.integers(0, 30, n).astype(np.uint32)
+    an, ap = sw.generate_pr(delta, 0.01, deg)
+    ean, eap = O.generate((delta > np.float32(0.01)).astype(np.uint8), deg)
+    assert (an == ean).all() and (ap == eap).all()
+
+
+def test_generate_64bit_offsets(sw, gpu):
+    """Active edge totals past 2^32 (the reference's u32 scan overflows)."""
+    n = 3_000_000
+    deg = np.full(n, 3000, np.uint32)
+    ones = np.ones(n, np.uint8)
+    an, ap = sw.generate(ones, np.zeros(n, np.uint8), deg)
+    assert len(an) == n and int(ap[-1]) == 3000 * n > 2**32
+    assert (ap[1:] - ap[:-1] == 3000).all()
+
+
+def test_generate_repeated_launches(sw, gpu):
+    """Epoch-tagged descriptors: many launches in a row stay exact."""
+    rng = np.random.default_rng(9)
+    n = 50_000
+    deg = rng.integers(0, 10, n).astype(np.uint32)
+    for i in range(20):
+        l1 = (rng.random(n) < 0.3).astype(np.uint8)
+        an, ap = sw.generate(l1, np.zeros(n, np.uint8), deg)
+        ean, eap = O.generate(l1, deg)
+        assert (an == ean).all() and (ap == eap).all()
+
+
+# ----------------------------------------------------------- partitioner
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("budget", [1, 64, 5000, 10**12])
+def test_partition_matches_oracle(sw, gpu, seed, budget):
+    rng = np.random.default_rng(seed)
+    a = int(rng.integers(1, 200_000))
+    deg = rng.integers(0, 300, a).astype(np.uint64)
+    deg[rng.integers(0, a, 3)] = 20_000
+    ap = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    an = np.sort(rng.choice(10 * a, a, replace=False)).astype(np.uint32)
+    rows = sw.partition(ap, an, budget)
+    exp = O.partition(ap, budget)
+    assert rows.shape[0] == exp.shape[0]
+    assert (rows[:, :4] == exp).all()
+    assert (rows[:, 4] == an[rows[:, 0].astype(np.int64)]).all()
+    assert (rows[:, 5] == an[(rows[:, 0] + rows[:, 1] - 1).astype(np.int64)]).all()
+
+
+# ---------------------------------------------------------------- gather
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("frac", [0.0005, 0.05, 0.6, 1.0])
+def test_gather_matches_csr_slices(sw, gpu, weighted, frac):
+    g = sw.Graph.rmat(14, 16, seed=3, weighted=weighted)
+    ptr, adj, w = g.csr()
+    rng = np.random.default_rng(int(frac * 1e4))
+    act = np.nonzero(rng.random(g.n) < frac)[0].astype(np.uint32)
+    if act.size == 0:
+        act = np.array([0], np.uint32)
+    ends, wts = sw.gather(g, act)
+    idx = np.concatenate([np.arange(ptr[v], ptr[v + 1]) for v in act]).astype(np.int64)
+    assert (ends == adj[idx]).all()
+    if weighted:
+        assert (wts == w[idx]).all()
+
+
+# ---------------------------------------------------------- applications
+APPS = [(a, m) for a in ("bfs", "cc", "sssp", "sswp") for m in ("sync", "async")]
+
+
+@pytest.mark.parametrize("name", ["readme", "detour", "cycles", "wide"])
+@pytest.mark.parametrize("algo,mode", APPS)
+def test_apps_golden(sw, gpu, name, algo, mode):
+    gd = hand()[name]
+    g = graph_of(sw, gd["n"], gd["edges"], weighted=True)
+    vals, _ = sw.run(g, algo, mode=mode, source=gd["source"])
+    assert vals.tolist() == gd[algo]
+
+
+@pytest.mark.parametrize("mode", ["sync", "async"])
+def test_pr_golden_readme(sw, gpu, mode):
+    gd = hand()["readme"]
+    g = graph_of(sw, gd["n"], gd["edges"], weighted=False)
+    vals, _ = sw.run(g, "pr", mode=mode, threshold=1e-9)
+    np.testing.assert_allclose(vals, gd["pr"], rtol=1e-6)
+
+
+@pytest.mark.parametrize("algo,mode", APPS + [("pr", "sync"), ("pr", "async")])
+@pytest.mark.parametrize("budget", [0, 64 << 10])
+@pytest.mark.parametrize("transfer", ["auto", "zerocopy"])
+def test_apps_random(sw, gpu, algo, mode, budget, transfer):
+    rng = np.random.default_rng(zlib.crc32(f"{algo}-{mode}-{budget}-{transfer}".encode()))
+    n = 20_000
+    weighted = algo in ("sssp", "sswp")
+    g = random_graph(sw, rng, n, 8 * n, weighted=weighted, hub=(17, 9000))
+    kw = dict(budget=budget, transfer=transfer)
+    if algo == "pr":
+        kw["threshold"] = 1e-6
+    _, st = check_app(sw, g, algo, mode, src=17 if algo != "cc" else 0, **kw)
+    if budget:
+        assert st.partitions > st.global_iterations  # the out-of-memory path ran
+
+
+@pytest.mark.parametrize("algo,mode", APPS)
+@pytest.mark.parametrize("buffers", [1, 2])
+def test_apps_rmat_tiny_budget(sw, gpu, algo, mode, buffers):
+    """R-MAT scale 14 through a 32 KiB budget: dozens of partitions and
+    hub vertices larger than a buffer (streamed in chunks)."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(14, 16, seed=11, weighted=weighted, symmetrize=(algo == "cc"))
+    assert g.max_degree > 2048
+    check_app(sw, g, algo, mode, src=g.max_degree_vertex, budget=32 << 10, buffers=buffers)
+
+
+def test_config1_bfs_sync_scale16(sw, gpu):
+    """R-MAT scale 16, edge factor 16, source 0, no budget cap: bit-exact
+    against the oracle and the OpenMP build."""
+    g = sw.Graph.rmat(16, 16, seed=42)
+    vals, st = check_app(sw, g, "bfs", "sync", src=0)
+    ptr, adj, _ = g.csr()
+    omp, _, _ = O.omp_label("bfs", ptr, adj, None, 0)
+    assert (vals == omp).all()
+    assert st.global_iterations >= 2
+
+
+def test_deterministic_repeats(sw, gpu):
+    g = sw.Graph.rmat(15, 16, seed=5, weighted=True)
+    first, _ = sw.run(g, "sssp", mode="async", budget=256 << 10)
+    for _ in range(3):
+        again, _ = sw.run(g, "sssp", mode="async", budget=256 << 10)
+        assert (again == first).all()
+
+
+def test_pr_rmat_tolerance(sw, gpu):
+    g = sw.Graph.rmat(14, 16, seed=8)
+    check_app(sw, g, "pr", "async", threshold=1e-6, budget=64 << 10)
+    check_app(sw, g, "pr", "sync", threshold=1e-6)
+
+
+def test_errors_are_reported(sw, gpu):
+    g = sw.Graph.rmat(10, 4, seed=1)
+    with pytest.raises(sw.SubwayError):
+        sw.run(g, "sssp")  # unweighted graph
+    with pytest.raises(sw.SubwayError):
+        sw.run(g, "bfs", source=g.n)  # source out of range
+    with pytest.raises(sw.SubwayError):
+        sw.run(g, "bfs", budget=100)  # below one advance chunk per buffer
