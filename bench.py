@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--transfer", default="auto")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--probe-copy-mb", type=int, default=0,
+                    help="launch the plain copy kernel once on this many MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -75,23 +77,35 @@ def main():
     log(f"R-MAT scale {args.scale}: {g.n} vertices, {g.m} edges in {time.perf_counter() - t0:.1f} s")
     src = g.max_degree_vertex
     edge_bytes = 8 if weighted else 4
-    kw = dict(mode=args.mode, source=src, budget=budget, buffers=args.buffers, transfer=args.transfer)
+    kw = dict(mode=args.mode, budget=budget, buffers=args.buffers, transfer=args.transfer, profile=True)
     if args.algo == "pr":
         kw["threshold"] = 1e-6
+    # device state and every pipeline buffer are built once; each step is a
+    # full run from the initial state (one query)
+    t0 = time.perf_counter()
+    sess = sw.Session(g, args.algo, **kw)
+    log(f"session setup {1e3 * (time.perf_counter() - t0):.1f} ms")
 
+    if args.probe_copy_mb:
+        ms = sw.probe_copy(args.probe_copy_mb << 20)
+        log(f"copy probe: {args.probe_copy_mb} MiB read + written in {ms:.3f} ms")
     peak = sw.h2d_peak(256 << 20, 10)
     log(f"pinned H2D peak {peak:.1f} GB/s")
     for i in range(args.warmup):
-        _, st = sw.run(g, args.algo, **kw)
+        _, st = sess.run(src)
         log(f"warmup {i}: {st.ms_total:.1f} ms, {st.global_iterations} global iterations")
 
     stats = []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        vals, st = sw.run(g, args.algo, profile=True, **kw)
+        vals, st = sess.run(src)
         stats.append(st)
         log(f"step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
-            f"H2D {st.h2d_bytes / 1e9:.2f} GB")
+            f"H2D {st.h2d_bytes / 1e9:.2f} GB, advance {st.ms_advance:.1f} ms, frontier {st.ms_frontier:.1f} ms, "
+            f"transfer {st.ms_transfer:.1f} ms, generate {st.ms_generate:.1f} ms")
+    for r in stats[-1].trace:
+        log(f"  iter {r['iteration']}: active {r['active_nodes']} v / {r['active_edges']} e, "
+            f"{r['partitions']} partitions, H2D {r['h2d_bytes'] / 1e6:.1f} MB, {r['ms']:.2f} ms")
     wall = time.perf_counter() - t_start
     ms_step = wall / args.steps * 1e3
     e2e_ms = float(np.mean([s.ms_total for s in stats]))
@@ -146,6 +160,10 @@ def main():
         },
         "e2e_ms_library": round(e2e_ms, 3),
         "global_iterations": stats[-1].global_iterations,
+        "ms_breakdown": {"advance": round(adv_ms / args.steps, 3),
+                         "frontier": round(sum(s.ms_frontier for s in stats) / args.steps, 3),
+                         "transfer": round(xfer_ms, 3),
+                         "generate": round(sum(s.ms_generate for s in stats) / args.steps, 3)},
         "inner_rounds": stats[-1].inner_rounds,
         "partitions": stats[-1].partitions,
         "h2d_gb_per_step": round(h2d / 1e9, 4),

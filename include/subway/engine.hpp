@@ -11,6 +11,7 @@
 #ifndef SUBWAY_ENGINE_HPP
 #define SUBWAY_ENGINE_HPP
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,12 @@ struct RunOptions {
     int advanceGrid = 0;    // persistent workgroups of the advance kernel; 0 = auto
     int maxRoundBatch = 32; // async: rounds issued between convergence checks
     int profile = 0;        // 1: time every kernel launch and transfer with HIP events
+    int persistent = 1;     // async: all rounds of a partition in one cooperative launch
+    int roundsGrid = 0;     // workgroups of the persistent kernel; 0 = 4 per CU
+    int maxRounds = 1 << 16;  // async rounds per partition visit (labels left set resume next iteration)
+    int batch = 0;          // 1: batched k-source queries (runBatch), BFS/SSSP/SSWP async
+    int batchUnroll = 0;    // batched advance: edges in flight per wave (1, 2, 4, 8); 0 = 4
+    float theta = 0.5f;     // learned policy without a model: whole-range density threshold
 };
 
 struct IterRecord {
@@ -62,6 +69,8 @@ struct RunStats {
     double msAdvance = 0;      // advance kernel time, profile only
     double msFrontier = 0;     // frontier kernel time, profile only
     double msGenerate = 0;     // generator + partition kernels, profile only
+    double msRounds = 0;       // persistent rounds kernel time, profile only
+    ull roundsLaunches = 0;
     ull advanceLaunches = 0;
     ull frontierLaunches = 0;
     ull transferOps = 0;
@@ -76,6 +85,27 @@ struct RunStats {
     ull updates = 0;           // successful relaxations / PR pushes
     std::vector<IterRecord> trace;
 };
+
+// A graph's device state and all pipeline buffers, built once and reused
+// for many runs (e.g. queries from different sources).
+class Session {
+public:
+    virtual ~Session() = default;
+    virtual RunStats run(uint source) = 0;   // the source is ignored by CC and PR
+    virtual void values(void *out) = 0;      // result of the last run (u32 or float per vertex)
+    virtual ull numNodes() const = 0;
+    // Batched sessions (RunOptions::batch): k <= 64 sources per run; rows()
+    // copies the n x 64 vertex-major values of the last run (lane i = sources[i]).
+    virtual RunStats runBatch(const uint *sources, int k) = 0;
+    virtual void rows(uint *out) = 0;
+    // Autotuner of the batched advance: times complete runs (exact results)
+    // for each candidate wave assignment and keeps the fastest.  Returns the
+    // chosen edges-in-flight per wave; `timesMs` (may be null) gets 4 times.
+    virtual int tuneBatch(const uint *sources, int k, double *timesMs) = 0;
+};
+
+template <class E>
+std::unique_ptr<Session> make_session(std::shared_ptr<HostCsr<E>> csr, const RunOptions &opt);
 
 template <class E>
 RunStats runLabel(Graph<E> &graph, const RunOptions &opt);

@@ -38,6 +38,9 @@ public:
     void enqueue(Graph<E> &graph, Subgraph<E> &subgraph);
     void enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc);
     const unsigned long long *deviceTotals() const;   // [count, total]
+    // Scan scratch of the generator kernel (other selection rules, e.g. the
+    // batched per-source masks, run through it so deviceTotals() holds them).
+    subway::ScanScratch &scanScratch() { return *scratch; }
 
 private:
     subway::ScanScratch *scratch;

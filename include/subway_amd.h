@@ -42,6 +42,7 @@ extern "C" {
 #define SUBWAY_DIST_INFINITY 4294967294u
 
 typedef struct subway_graph subway_graph;
+typedef struct subway_session subway_session;
 
 typedef struct subway_options {
     int32_t algo;           /* SUBWAY_BFS .. SUBWAY_PR */
@@ -56,7 +57,15 @@ typedef struct subway_options {
     int32_t verbose;        /* print per-partition progress lines */
     int32_t advance_grid;   /* persistent workgroups of the advance kernel; 0 = auto */
     int32_t profile;        /* 1: time every kernel launch and transfer with HIP events */
-    int32_t reserved[5];
+    int32_t rounds_mode;    /* async: 0 = all rounds of a partition in one cooperative launch,
+                               1 = one frontier + advance launch pair per round */
+    int32_t rounds_grid;    /* workgroups of the cooperative rounds kernel; 0 = 4 per CU */
+    int32_t max_rounds;     /* async rounds per partition visit; 0 = 65536 */
+    int32_t batch;          /* 1: batched session (BFS/SSSP/SSWP async, <= 64 sources per run) */
+    int32_t batch_unroll;   /* batched advance: edges in flight per wave, 1/2/4/8; 0 = 4 */
+    float theta;            /* learned policy without a model: id ranges whose active-edge density is at
+                               least theta move whole (default 0.5; > 1 never) */
+    int32_t reserved[7];
 } subway_options;
 
 typedef struct subway_stats {
@@ -65,6 +74,7 @@ typedef struct subway_stats {
     double ms_advance;      /* advance kernel time (profile = 1) */
     double ms_frontier;     /* frontier kernel time (profile = 1) */
     double ms_generate;     /* generator + partition kernels (profile = 1) */
+    double ms_rounds;       /* cooperative rounds kernel (profile = 1) */
     uint64_t h2d_bytes;     /* edge bytes moved host -> device */
     uint64_t h2d_dma_bytes;
     uint64_t h2d_gather_bytes;
@@ -74,6 +84,7 @@ typedef struct subway_stats {
     uint64_t advance_launches;
     uint64_t frontier_launches;
     uint64_t transfer_ops;
+    uint64_t rounds_launches;
     uint64_t work_edges;    /* edges relaxed over all rounds */
     uint64_t work_entries;  /* vertices that pushed over all rounds */
     uint32_t global_iterations;
@@ -121,6 +132,26 @@ void subway_graph_free(subway_graph *g);
 int32_t subway_run(const subway_graph *g, const subway_options *opt, void *values, subway_stats *stats,
                    subway_iter_record *trace, uint32_t trace_cap);
 
+/* -------- sessions: the graph's device state and every pipeline buffer
+ * built once, then any number of runs (e.g. one query per source).  A
+ * session keeps its graph alive; runs on one session are serialised by the
+ * caller. */
+int32_t subway_session_create(const subway_graph *g, const subway_options *opt, subway_session **out);
+int32_t subway_session_run(subway_session *s, uint32_t source, void *values, subway_stats *stats,
+                           subway_iter_record *trace, uint32_t trace_cap);
+void subway_session_free(subway_session *s);
+/* Batched session (options.batch = 1): k <= 64 sources advance together in
+ * one cooperative kernel per transferred partition.  rows: n x 64 uint32,
+ * vertex-major (rows[v * 64 + i] is the value of v for sources[i]; lanes
+ * i >= k hold the initial value).  No reference counterpart: the reference
+ * runs k separate processes. */
+int32_t subway_session_run_batch(subway_session *s, const uint32_t *sources, int32_t k, uint32_t *rows,
+                                 subway_stats *stats, subway_iter_record *trace, uint32_t trace_cap);
+/* Autotuner: runs the batch once per candidate wave assignment (1, 2, 4, 8
+ * edges in flight per wave), keeps the fastest; times_ms[4] may be NULL. */
+int32_t subway_session_tune_batch(subway_session *s, const uint32_t *sources, int32_t k, int32_t *best_unroll,
+                                  double *times_ms);
+
 /* -------- single stages on host arrays (each runs the GPU kernel) */
 /* SubgraphGenerator::generate: shared/subgraph_generator.cu:128-162.
  * active_nodes[n], active_nodes_pointer[n+1]. */
@@ -144,6 +175,10 @@ int32_t subway_gather(const subway_graph *g, uint64_t num_active, const uint32_t
  * the device, best of `reps`, in GB/s (10^9 B/s).  No reference counterpart;
  * the denominator of the achieved host-to-device rate. */
 int32_t subway_h2d_peak(uint64_t bytes, int32_t reps, double *gbps);
+/* One launch of a plain device-to-device copy kernel that reads and writes
+ * `bytes` (multiple of 16): a kernel of known traffic against which hardware
+ * byte counters are calibrated.  *ms receives its duration. */
+int32_t subway_probe_copy(uint64_t bytes, double *ms);
 
 #ifdef __cplusplus
 }

@@ -22,8 +22,8 @@ import numpy as np
 from ._lib import lib, LIB_PATH, SubwayError, check
 
 __all__ = [
-    "Graph", "RunStats", "run", "generate", "generate_pr", "partition", "gather",
-    "device_count", "DIST_INFINITY", "ALGOS", "LIB_PATH", "SubwayError", "APPS", "h2d_peak",
+    "Graph", "RunStats", "Session", "BatchSession", "run", "generate", "generate_pr", "partition", "gather",
+    "device_count", "DIST_INFINITY", "ALGOS", "LIB_PATH", "SubwayError", "APPS", "h2d_peak", "probe_copy",
 ]
 
 DIST_INFINITY = 4294967294  # shared/globals.hpp:27
@@ -46,18 +46,22 @@ class _Options(ctypes.Structure):
         ("num_buffers", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64), ("acc", ctypes.c_float),
         ("init_pr", ctypes.c_float), ("transfer", ctypes.c_int32), ("policy", ctypes.c_int32),
         ("verbose", ctypes.c_int32), ("advance_grid", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
+        ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
 class _Stats(ctypes.Structure):
     _fields_ = [
         ("ms_total", ctypes.c_double), ("ms_transfer", ctypes.c_double), ("ms_advance", ctypes.c_double),
-        ("ms_frontier", ctypes.c_double), ("ms_generate", ctypes.c_double), ("h2d_bytes", ctypes.c_uint64),
+        ("ms_frontier", ctypes.c_double), ("ms_generate", ctypes.c_double), ("ms_rounds", ctypes.c_double),
+        ("h2d_bytes", ctypes.c_uint64),
         ("h2d_dma_bytes", ctypes.c_uint64), ("h2d_gather_bytes", ctypes.c_uint64),
         ("inner_rounds", ctypes.c_uint64), ("partitions", ctypes.c_uint64), ("updates", ctypes.c_uint64),
         ("advance_launches", ctypes.c_uint64), ("frontier_launches", ctypes.c_uint64),
-        ("transfer_ops", ctypes.c_uint64), ("work_edges", ctypes.c_uint64), ("work_entries", ctypes.c_uint64),
+        ("transfer_ops", ctypes.c_uint64), ("rounds_launches", ctypes.c_uint64), ("work_edges", ctypes.c_uint64),
+        ("work_entries", ctypes.c_uint64),
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
     ]
 
@@ -168,6 +172,7 @@ class RunStats:
     ms_advance: float
     ms_frontier: float
     ms_generate: float
+    ms_rounds: float
     h2d_bytes: int
     h2d_dma_bytes: int
     h2d_gather_bytes: int
@@ -177,21 +182,18 @@ class RunStats:
     advance_launches: int
     frontier_launches: int
     transfer_ops: int
+    rounds_launches: int
     work_edges: int
     work_entries: int
     global_iterations: int
     trace: list = field(default_factory=list)
 
 
-def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
-        threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
-        verbose: bool = False, advance_grid: int = 0, profile: bool = False):
-    """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
+ROUNDS = {"persistent": 0, "per-round": 1}
 
-    Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
-    float32 (PR).  ``budget`` is the device edge budget in bytes (0 = 90% of
-    device memory, the reference's sizing).
-    """
+
+def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
+             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -206,17 +208,118 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     o.verbose = int(verbose)
     o.advance_grid = advance_grid
     o.profile = int(profile)
-    vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
-    st = _Stats()
-    cap = 4096
-    rec = (_IterRecord * cap)()
-    check(lib.subway_run(graph._h, ctypes.byref(o), vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st),
-                         rec, cap))
+    o.rounds_mode = ROUNDS[rounds]
+    o.rounds_grid = rounds_grid
+    o.max_rounds = max_rounds
+    o.theta = theta
+    return o
+
+
+def _stats(st, rec):
     trace = [dict(iteration=r.iteration, partitions=r.partitions, active_nodes=r.active_nodes,
                   active_edges=r.active_edges, h2d_bytes=r.h2d_bytes, ms=r.ms_iteration)
              for r in rec[: st.trace_len]]
     names = [f.name for f in RunStats.__dataclass_fields__.values() if f.name != "trace"]
-    return vals, RunStats(**{k: getattr(st, k) for k in names}, trace=trace)
+    return RunStats(**{k: getattr(st, k) for k in names}, trace=trace)
+
+
+_TRACE_CAP = 4096
+
+
+class Session:
+    """Device state and pipeline buffers of one (graph, algorithm, options)
+    built once; :meth:`run` answers one query per call."""
+
+    def __init__(self, graph: Graph, algo: str, mode: str = "sync", budget: int = 0, buffers: int = 2,
+                 threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
+                 verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
+                 rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5):
+        self.graph, self.algo = graph, algo
+        o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
+                     profile, rounds, rounds_grid, max_rounds, theta)
+        h = ctypes.c_void_p()
+        check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, source: int = 0):
+        vals = np.empty(self.graph.n, dtype=np.float32 if self.algo == "pr" else np.uint32)
+        st = _Stats()
+        rec = (_IterRecord * _TRACE_CAP)()
+        check(lib.subway_session_run(self._h, source, vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st),
+                                     rec, _TRACE_CAP))
+        return vals, _stats(st, rec)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.subway_session_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class BatchSession:
+    """k <= 64 sources of BFS/SSSP/SSWP advancing together (async), one
+    cooperative kernel per transferred partition; :meth:`run` returns a
+    (k, n) array (row i = values for sources[i])."""
+
+    def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",
+                 unroll: int = 0, rounds_grid: int = 0, profile: bool = False, policy: str = "plain",
+                 theta: float = 0.5):
+        self.graph, self.algo = graph, algo
+        o = _options(algo, "async", 0, budget, buffers, 0.01, 0.15, transfer, policy, False, 0, profile,
+                     "persistent", rounds_grid, 0, theta)
+        o.batch = 1
+        o.batch_unroll = unroll
+        h = ctypes.c_void_p()
+        check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, sources, rows_out: bool = True):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        k = len(src)
+        rows = np.empty(self.graph.n * 64, dtype=np.uint32) if rows_out else None
+        st = _Stats()
+        rec = (_IterRecord * _TRACE_CAP)()
+        check(lib.subway_session_run_batch(self._h, _ptr(src, _u32p), k, _ptr(rows, _u32p), ctypes.byref(st), rec,
+                                           _TRACE_CAP))
+        vals = rows.reshape(self.graph.n, 64)[:, :k].T.copy() if rows_out else None
+        return vals, _stats(st, rec)
+
+    def tune(self, sources):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        best = ctypes.c_int32()
+        times = (ctypes.c_double * 4)()
+        check(lib.subway_session_tune_batch(self._h, _ptr(src, _u32p), len(src), ctypes.byref(best), times))
+        return int(best.value), list(times)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.subway_session_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
+        threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
+        verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
+        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5):
+    """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
+
+    Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
+    float32 (PR).  ``budget`` is the device edge budget in bytes (0 = 90% of
+    device memory, the reference's sizing).
+    """
+    o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
+                 profile, rounds, rounds_grid, max_rounds, theta)
+    vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
+    st = _Stats()
+    rec = (_IterRecord * _TRACE_CAP)()
+    check(lib.subway_run(graph._h, ctypes.byref(o), vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st),
+                         rec, _TRACE_CAP))
+    return vals, _stats(st, rec)
 
 
 def generate(label1, label2, out_degree):
@@ -280,4 +383,11 @@ def h2d_peak(nbytes: int = 256 << 20, reps: int = 10) -> float:
     """Measured pinned host-to-device copy peak in GB/s (best of reps)."""
     out = ctypes.c_double()
     check(lib.subway_h2d_peak(nbytes, reps, ctypes.byref(out)))
+    return float(out.value)
+
+
+def probe_copy(nbytes: int = 256 << 20) -> float:
+    """One launch of a plain copy kernel of nbytes read + nbytes written (ms)."""
+    out = ctypes.c_double()
+    check(lib.subway_probe_copy(nbytes, ctypes.byref(out)))
     return float(out.value)

@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libsubway_amd.so")
+# SUBWAY_AMD_LIB points at another build of the same library (e.g. an
+# out-of-tree build directory); the default is the in-tree build.
+LIB_PATH = os.environ.get("SUBWAY_AMD_LIB") or os.path.join(HERE, "lib", "libsubway_amd.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with `make -C subway_amd` "
@@ -50,11 +52,17 @@ _SIGS = {
     "subway_graph_export": (_i32, [_vp, _u64p, _u32p, _u32p]),
     "subway_graph_free": (None, [_vp]),
     "subway_run": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32]),
+    "subway_session_create": (_i32, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "subway_session_run": (_i32, [_vp, _u32, _vp, _vp, _vp, _u32]),
+    "subway_session_free": (None, [_vp]),
+    "subway_session_run_batch": (_i32, [_vp, _u32p, _i32, _u32p, _vp, _vp, _u32]),
+    "subway_session_tune_batch": (_i32, [_vp, _u32p, _i32, _i32p, ctypes.POINTER(ctypes.c_double)]),
     "subway_generate": (_i32, [_u64, _u8p, _u8p, _u32p, _u32p, _u64p, _u64p]),
     "subway_generate_pr": (_i32, [_u64, _f32p, ctypes.c_float, _u32p, _u32p, _u64p, _u64p]),
     "subway_partition": (_i32, [_u64, _u64p, _u32p, _u64, _u64, _u64p, _u64p]),
     "subway_gather": (_i32, [_vp, _u64, _u32p, _vp, _u64p]),
     "subway_h2d_peak": (_i32, [_u64, _i32, ctypes.POINTER(ctypes.c_double)]),
+    "subway_probe_copy": (_i32, [_u64, ctypes.POINTER(ctypes.c_double)]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -77,6 +77,7 @@ void subway_default_options(subway_options *o)
     o->num_buffers = 2;
     o->acc = 0.01f;
     o->init_pr = 0.15f;
+    o->theta = 0.5f;
 }
 
 int32_t subway_device_count(void)
@@ -215,6 +216,12 @@ static RunOptions to_options(const subway_options *o)
     r.verbose = o->verbose;
     r.advanceGrid = o->advance_grid;
     r.profile = o->profile;
+    r.persistent = o->rounds_mode == 0;
+    r.batch = o->batch;
+    r.batchUnroll = o->batch_unroll;
+    r.theta = o->theta;
+    r.roundsGrid = o->rounds_grid;
+    if (o->max_rounds > 0) r.maxRounds = o->max_rounds;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -222,67 +229,114 @@ static RunOptions to_options(const subway_options *o)
 
 }  // extern "C"
 
-template <class E>
-static RunStats run_on(const std::shared_ptr<HostCsr<E>> &csr, const RunOptions &o, void *values)
+struct subway_session {
+    std::shared_ptr<void> keep;          // the graph's CSR stays alive with the session
+    std::unique_ptr<Session> impl;
+};
+
+static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_record *trace, uint32_t trace_cap)
 {
-    if (o.algo == PR) {
-        GraphPR<E> g(csr);
-        RunStats s = runPageRank(g, o);
-        if (values) std::memcpy(values, g.value, (size_t)g.num_nodes * sizeof(float));
-        return s;
+    uint32_t k = 0;
+    if (trace) {
+        for (const IterRecord &r : s.trace) {
+            if (k >= trace_cap) break;
+            trace[k].iteration = r.iteration;
+            trace[k].partitions = r.partitions;
+            trace[k].active_nodes = r.activeNodes;
+            trace[k].active_edges = r.activeEdges;
+            trace[k].h2d_bytes = r.h2dBytes;
+            trace[k].ms_iteration = r.msIteration;
+            k++;
+        }
     }
-    Graph<E> g(csr);
-    RunStats s = runLabel(g, o);
-    if (values) std::memcpy(values, g.value, (size_t)g.num_nodes * sizeof(uint));
-    return s;
+    if (!stats) return;
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ms_total = s.msTotal;
+    stats->ms_transfer = s.msTransfer;
+    stats->ms_advance = s.msAdvance;
+    stats->ms_frontier = s.msFrontier;
+    stats->ms_generate = s.msGenerate;
+    stats->ms_rounds = s.msRounds;
+    stats->rounds_launches = s.roundsLaunches;
+    stats->h2d_bytes = s.h2dBytes;
+    stats->h2d_dma_bytes = s.h2dDmaBytes;
+    stats->h2d_gather_bytes = s.h2dGatherBytes;
+    stats->inner_rounds = s.innerRounds;
+    stats->partitions = s.partitions;
+    stats->updates = s.updates;
+    stats->advance_launches = s.advanceLaunches;
+    stats->frontier_launches = s.frontierLaunches;
+    stats->transfer_ops = s.transferOps;
+    stats->work_edges = s.workEdges;
+    stats->work_entries = s.workEntries;
+    stats->global_iterations = s.globalIterations;
+    stats->trace_len = k;
 }
 
 extern "C" {
 
-int32_t subway_run(const subway_graph *g, const subway_options *opt, void *values, subway_stats *stats,
-                   subway_iter_record *trace, uint32_t trace_cap)
+int32_t subway_session_create(const subway_graph *g, const subway_options *opt, subway_session **out)
 {
     return guard([&] {
-        if (!g || !opt) throw Error(ERR_ARG, "null argument");
+        if (!g || !opt || !out) throw Error(ERR_ARG, "null argument");
         RunOptions o = to_options(opt);
         if ((o.algo == SSSP || o.algo == SSWP) && !g->weighted)
             throw Error(ERR_ARG, "SSSP/SSWP need a weighted graph");
-        RunStats s = g->weighted ? run_on(g->w, o, values) : run_on(g->u, o, values);
-        if (stats) {
-            std::memset(stats, 0, sizeof(*stats));
-            stats->ms_total = s.msTotal;
-            stats->ms_transfer = s.msTransfer;
-            stats->ms_advance = s.msAdvance;
-            stats->ms_frontier = s.msFrontier;
-            stats->ms_generate = s.msGenerate;
-            stats->advance_launches = s.advanceLaunches;
-            stats->frontier_launches = s.frontierLaunches;
-            stats->transfer_ops = s.transferOps;
-            stats->work_edges = s.workEdges;
-            stats->work_entries = s.workEntries;
-            stats->h2d_bytes = s.h2dBytes;
-            stats->h2d_dma_bytes = s.h2dDmaBytes;
-            stats->h2d_gather_bytes = s.h2dGatherBytes;
-            stats->inner_rounds = s.innerRounds;
-            stats->partitions = s.partitions;
-            stats->updates = s.updates;
-            stats->global_iterations = s.globalIterations;
+        std::unique_ptr<subway_session> s(new subway_session);
+        if (g->weighted) {
+            s->keep = g->w;
+            s->impl = make_session(g->w, o);
+        } else {
+            s->keep = g->u;
+            s->impl = make_session(g->u, o);
         }
-        uint32_t k = 0;
-        if (trace) {
-            for (const IterRecord &r : s.trace) {
-                if (k >= trace_cap) break;
-                trace[k].iteration = r.iteration;
-                trace[k].partitions = r.partitions;
-                trace[k].active_nodes = r.activeNodes;
-                trace[k].active_edges = r.activeEdges;
-                trace[k].h2d_bytes = r.h2dBytes;
-                trace[k].ms_iteration = r.msIteration;
-                k++;
-            }
-        }
-        if (stats) stats->trace_len = k;
+        *out = s.release();
     });
+}
+
+int32_t subway_session_run(subway_session *s, uint32_t source, void *values, subway_stats *stats,
+                           subway_iter_record *trace, uint32_t trace_cap)
+{
+    return guard([&] {
+        if (!s) throw Error(ERR_ARG, "null session");
+        RunStats r = s->impl->run(source);
+        if (values) s->impl->values(values);
+        fill_stats(r, stats, trace, trace_cap);
+    });
+}
+
+void subway_session_free(subway_session *s) { delete s; }
+
+int32_t subway_session_run_batch(subway_session *s, const uint32_t *sources, int32_t k, uint32_t *rows,
+                                 subway_stats *stats, subway_iter_record *trace, uint32_t trace_cap)
+{
+    return guard([&] {
+        if (!s || !sources) throw Error(ERR_ARG, "null argument");
+        RunStats r = s->impl->runBatch(sources, k);
+        if (rows) s->impl->rows(rows);
+        fill_stats(r, stats, trace, trace_cap);
+    });
+}
+
+int32_t subway_session_tune_batch(subway_session *s, const uint32_t *sources, int32_t k, int32_t *best_unroll,
+                                  double *times_ms)
+{
+    return guard([&] {
+        if (!s || !sources) throw Error(ERR_ARG, "null argument");
+        int b = s->impl->tuneBatch(sources, k, times_ms);
+        if (best_unroll) *best_unroll = b;
+    });
+}
+
+int32_t subway_run(const subway_graph *g, const subway_options *opt, void *values, subway_stats *stats,
+                   subway_iter_record *trace, uint32_t trace_cap)
+{
+    subway_session *s = nullptr;
+    int32_t rc = subway_session_create(g, opt, &s);
+    if (rc) return rc;
+    rc = subway_session_run(s, opt->source, values, stats, trace, trace_cap);
+    subway_session_free(s);
+    return rc;
 }
 
 int32_t subway_generate(uint64_t n, const uint8_t *label1, const uint8_t *label2, const uint32_t *out_degree,
@@ -443,6 +497,28 @@ int32_t subway_h2d_peak(uint64_t bytes, int32_t reps, double *gbps)
         (void)hipEventDestroy(b);
         (void)hipStreamDestroy(s);
         *gbps = (double)bytes / (best * 1e-3) / 1e9;
+    });
+}
+
+int32_t subway_probe_copy(uint64_t bytes, double *ms)
+{
+    return guard([&] {
+        if (!bytes || bytes % 16) throw Error(ERR_ARG, "size must be a positive multiple of 16");
+        DevBuf<uint8_t> a(bytes), b(bytes);
+        SUBWAY_CHECK(hipMemset(a.p, 1, bytes));
+        hipEvent_t e0, e1;
+        SUBWAY_CHECK(hipEventCreate(&e0));
+        SUBWAY_CHECK(hipEventCreate(&e1));
+        SUBWAY_CHECK(hipDeviceSynchronize());
+        SUBWAY_CHECK(hipEventRecord(e0, nullptr));
+        launch_copy_probe(a.p, b.p, bytes, nullptr);
+        SUBWAY_CHECK(hipEventRecord(e1, nullptr));
+        SUBWAY_CHECK(hipEventSynchronize(e1));
+        float t = 0;
+        SUBWAY_CHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms) *ms = t;
     });
 }
 

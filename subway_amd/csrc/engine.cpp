@@ -76,6 +76,7 @@ public:
         SUBWAY_CHECK(hipEventRecord(ev_[used_ + 1], s));
         used_ += 2;
     }
+    void reset() { used_ = 0; }
     // total milliseconds of all pairs; call after the streams are drained
     double total(ull &count)
     {
@@ -105,21 +106,36 @@ struct Unit {
     ull count;          // edges
     ull kbeg, kend;     // active-list range (gather)
     ull edgeBase;       // activeNodesPointer value at kbeg (gather)
+    // the items the rounds run over
+    ull n;              // number of items
+    const uint *nodes;  // active-list slice, or null for a whole id range
+    const ull *ptr;     // offsets of the items (n + 1 readable)
+    ull fromEdge;       // subtracted from ptr to get buffer offsets
+    uint idBase;        // first id of a whole range
+    bool whole;         // whole id range resident (learned policy)
 };
 
 template <class E, class G>
 class Engine {
 public:
-    Engine(G &g, const RunOptions &o) : g_(g), o_(o)
+    // Non-owning: runs on a graph object the caller keeps (drop-in apps).
+    Engine(G &g, const RunOptions &o) : g_(&g), o_(o) { setup(); }
+    // Owning: builds the graph's device state from a shared CSR (sessions).
+    Engine(std::shared_ptr<HostCsr<E>> csr, const RunOptions &o) : own_(new G(std::move(csr))), o_(o)
+    {
+        g_ = own_.get();
+        setup();
+    }
+    G &graph() { return *g_; }
+
+    void setup()
     {
         if (o_.numBuffers != 1 && o_.numBuffers != 2) throw Error(ERR_ARG, "numBuffers must be 1 or 2");
-        if (o_.algo != PR && o_.source >= g_.num_nodes && o_.algo != CC)
-            throw Error(ERR_ARG, "source vertex " + std::to_string(o_.source) + " out of range");
         if (o_.budgetBytes)
-            sub_.reset(new Subgraph<E>(g_.num_nodes, g_.num_edges, o_.budgetBytes, o_.numBuffers));
+            sub_.reset(new Subgraph<E>(g_->num_nodes, g_->num_edges, o_.budgetBytes, o_.numBuffers));
         else
-            sub_.reset(new Subgraph<E>(g_.num_nodes, g_.num_edges));
-        gen_.reset(new SubgraphGenerator<E>(g_));
+            sub_.reset(new Subgraph<E>(g_->num_nodes, g_->num_edges));
+        gen_.reset(new SubgraphGenerator<E>(*g_));
         part_.reset(new Partitioner<E>());
         SUBWAY_CHECK(hipStreamCreateWithFlags(&sc_, hipStreamNonBlocking));
         SUBWAY_CHECK(hipStreamCreateWithFlags(&sx_, hipStreamNonBlocking));
@@ -128,35 +144,106 @@ public:
         for (int b = 0; b < 2; b++) {
             SUBWAY_CHECK(hipEventCreateWithFlags(&evLoaded_[b], hipEventDisableTiming));
             SUBWAY_CHECK(hipEventCreateWithFlags(&evFree_[b], hipEventDisableTiming));
+            SUBWAY_CHECK(hipEventCreateWithFlags(&evStaged_[b], hipEventDisableTiming));
         }
         SUBWAY_CHECK(hipEventCreate(&evX0_));
         SUBWAY_CHECK(hipEventCreate(&evX1_));
         maxPart_ = sub_->max_partition_size;
-        alloc_scan_scratch(fr_, ((ull)g_.num_nodes + FR_TILE - 1) / FR_TILE + 1);
-        ull maxEntries = std::min<ull>((ull)g_.num_nodes, maxPart_) + 1;
+        alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
+        ull maxEntries = std::min<ull>((ull)g_->num_nodes, maxPart_) + 1;
         SUBWAY_CHECK(hipMalloc(&entries_, maxEntries * sizeof(Entry)));
-        ull maxWork = std::max<ull>(maxPart_, (ull)g_.csr->maxDegree);
-        SUBWAY_CHECK(hipMalloc(&chunkMap_, (maxWork / ADV_CHUNK + 2) * sizeof(uint32_t)));
+        ull maxWork = std::max<ull>(maxPart_, (ull)g_->csr->maxDegree);
+        const ull chunk = o_.batch ? (ull)BATCH_CHUNK_EDGES : (ull)ADV_CHUNK;
+        SUBWAY_CHECK(hipMalloc(&chunkMap_, (maxWork / chunk + 2) * sizeof(uint32_t)));
         SUBWAY_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(ScanCtl), 0));
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
         advGrid_ = o_.advanceGrid > 0 ? o_.advanceGrid : prop.multiProcessorCount * 4;
+        if (o_.policy == POLICY_LEARNED) setupRanges();
+        if (o_.batch) {
+            setupBatch(prop, maxEntries);
+            return;
+        }
+        if (o_.persistent) {
+            int mx = rounds_max_grid(o_.algo, (int)sizeof(E));
+            roundsGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
+            if (roundsGrid_ > 0) {
+                SUBWAY_CHECK(hipMalloc(&pc_, sizeof(dev::PersistCtl)));
+                SUBWAY_CHECK(hipMemset(pc_, 0, sizeof(dev::PersistCtl)));
+                SUBWAY_CHECK(hipDeviceSynchronize());
+            }
+        }
     }
+
+    // Batched k-source state: dist rows [V][64], two 64-bit mask arrays,
+    // per-entry masks, and a cooperative grid for the batched rounds.
+    void setupBatch(const hipDeviceProp_t &prop, ull maxEntries)
+    {
+        if constexpr (isPR) {
+            throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
+        } else {
+            if (o_.algo != BFS && o_.algo != SSSP && o_.algo != SSWP)
+                throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
+            if (o_.mode != ASYNC) throw Error(ERR_ARG, "batched runs are async");
+            const ull n = g_->num_nodes;
+            SUBWAY_CHECK(hipMalloc(&d_dist_, n * BATCH_LANES * sizeof(uint32_t)));
+            SUBWAY_CHECK(hipMalloc(&d_m1_, n * sizeof(ull)));
+            SUBWAY_CHECK(hipMalloc(&d_m2_, n * sizeof(ull)));
+            SUBWAY_CHECK(hipMalloc(&d_entryMask_, maxEntries * sizeof(ull)));
+            SUBWAY_CHECK(hipMalloc(&d_sources_, BATCH_LANES * sizeof(uint32_t)));
+            SUBWAY_CHECK(hipHostMalloc((void **)&h_sources_, BATCH_LANES * sizeof(uint32_t), 0));
+            setBatchUnroll(o_.batchUnroll > 0 ? o_.batchUnroll : 4);
+            SUBWAY_CHECK(hipMalloc(&pc_, sizeof(dev::PersistCtl)));
+            SUBWAY_CHECK(hipMemset(pc_, 0, sizeof(dev::PersistCtl)));
+            SUBWAY_CHECK(hipDeviceSynchronize());
+            (void)prop;
+        }
+    }
+
+public:
+    void setBatchUnroll(int u)
+    {
+        int dev = 0;
+        SUBWAY_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
+        const int mx = batch_max_grid(o_.algo, (int)sizeof(E), u);
+        if (mx <= 0) throw Error(ERR_ARG, "this device cannot run the cooperative batched rounds kernel");
+        batchUnroll_ = u;
+        batchGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
+    }
+    int batchUnroll() const { return batchUnroll_; }
+    RunStats runBatch(const uint *sources, int k);
+    // rows of the last batched run: n x 64 u32, vertex-major
+    const uint32_t *batchRows() const { return h_dist_; }
 
     ~Engine()
     {
         if (sc_) (void)hipStreamSynchronize(sc_);
         if (sx_) (void)hipStreamSynchronize(sx_);
+        if (d_bounds_) (void)hipFree(d_bounds_);
+        if (d_rstats_) (void)hipFree(d_rstats_);
+        if (h_rstats_) (void)hipHostFree(h_rstats_);
+        if (d_dist_) (void)hipFree(d_dist_);
+        if (d_m1_) (void)hipFree(d_m1_);
+        if (d_m2_) (void)hipFree(d_m2_);
+        if (d_entryMask_) (void)hipFree(d_entryMask_);
+        if (d_sources_) (void)hipFree(d_sources_);
+        if (h_sources_) (void)hipHostFree(h_sources_);
+        if (h_dist_) (void)hipHostFree(h_dist_);
         free_scan_scratch(fr_);
         if (entries_) (void)hipFree(entries_);
         if (chunkMap_) (void)hipFree(chunkMap_);
         if (h_ctl_) (void)hipHostFree(h_ctl_);
+        if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
             (void)hipEventDestroy(evLoaded_[b]);
             (void)hipEventDestroy(evFree_[b]);
+            (void)hipEventDestroy(evStaged_[b]);
         }
+        if (h_ids_) (void)hipHostFree(h_ids_);
         (void)hipEventDestroy(evX0_);
         (void)hipEventDestroy(evX1_);
         part_.reset();
@@ -166,11 +253,13 @@ public:
         if (sx_) (void)hipStreamDestroy(sx_);
     }
 
-    RunStats run();
+    RunStats run(uint source);
 
 private:
-    G &g_;
+    std::unique_ptr<G> own_;
+    G *g_ = nullptr;
     RunOptions o_;
+    uint src_ = 0;
     std::unique_ptr<Subgraph<E>> sub_;
     std::unique_ptr<SubgraphGenerator<E>> gen_;
     std::unique_ptr<Partitioner<E>> part_;
@@ -184,7 +273,13 @@ private:
     int advGrid_ = 0;
     RunStats st_;
     std::vector<Unit> units_;
-    EventPairs pAdv_, pFr_, pXfer_, pGen_;
+    EventPairs pAdv_, pFr_, pXfer_, pGen_, pRounds_;
+    dev::PersistCtl *pc_ = nullptr;
+    int roundsGrid_ = 0;    // 0: persistent rounds kernel unavailable
+    // batched state
+    int batchK_ = 0, batchUnroll_ = 4, batchGrid_ = 0;
+    uint32_t *d_dist_ = nullptr, *d_sources_ = nullptr, *h_sources_ = nullptr, *h_dist_ = nullptr;
+    ull *d_m1_ = nullptr, *d_m2_ = nullptr, *d_entryMask_ = nullptr;
 
     static constexpr bool isPR = std::is_same<G, GraphPR<E>>::value;
 
@@ -196,55 +291,213 @@ private:
     void frontier(int first, uint8_t *labA, uint8_t *labB, const Unit &un);
     void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf);
     void asyncRounds(const Unit &un, int buf);
+    void hostGather(const Unit &u, int b);
+    // items of a compacted unit: a slice of the active list
+    void setItems(Unit &u, ull fromNode, ull cnt, ull fromEdge)
+    {
+        u.n = cnt;
+        u.nodes = sub_->d_activeNodes + fromNode;
+        u.ptr = sub_->d_activeNodesPointer + fromNode;
+        u.fromEdge = fromEdge;
+        u.idBase = 0;
+        u.whole = false;
+    }
+    int roundsCap_ = 1 << 16;
+    // learned policy: static id ranges of at most one buffer of edges
+    std::vector<uint> bounds_;
+    uint32_t *d_bounds_ = nullptr;
+    ull *d_rstats_ = nullptr, *h_rstats_ = nullptr;
+    double theta_ = 2.0;    // density at or above which a range moves whole
+    void setupRanges();
+    void buildUnitsLearned();
+    void pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, ull edges);
+    void computeBatch(const Unit &u, int b);
+    RunStats loop(const std::chrono::steady_clock::time_point &t0);
+    void resetRun();
+    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0);
+    uint *pinnedIds(ull n)
+    {
+        if (n > idsCap_) {
+            if (h_ids_) SUBWAY_CHECK(hipHostFree(h_ids_));
+            idsCap_ = std::max<ull>(n, (ull)g_->num_nodes);
+            SUBWAY_CHECK(hipHostMalloc((void **)&h_ids_, idsCap_ * sizeof(uint), 0));
+        }
+        return h_ids_;
+    }
+    uint *h_ids_ = nullptr;
+    ull idsCap_ = 0;
+    bool hostIdsValid_ = false;
+    hipEvent_t evStaged_[2];
     ull deviceRounds();
 };
 
 template <class E, class G>
 void Engine<E, G>::initState()
 {
-    const uint n = g_.num_nodes;
+    const uint n = g_->num_nodes;
     if constexpr (isPR) {
-        for (uint i = 0; i < n; i++) {
-            g_.delta[i] = o_.initPR;
-            g_.value[i] = 0.0f;
-        }
-        SUBWAY_CHECK(hipMemcpy(g_.d_value, g_.value, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
-        SUBWAY_CHECK(hipMemcpy(g_.d_delta, g_.delta, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+        launch_init_pr_state(n, g_->d_value, g_->d_delta, o_.initPR, sc_);
     } else {
-        const bool async = o_.mode == ASYNC;
-        for (uint i = 0; i < n; i++) {
-            if (o_.algo == CC) g_.value[i] = i;
-            else if (o_.algo == SSWP) g_.value[i] = 0;
-            else g_.value[i] = DIST_INFINITY;
-            g_.label1[i] = async;
-            g_.label2[i] = (!async && o_.algo == CC);
-        }
-        if (o_.algo != CC) {
-            g_.value[o_.source] = (o_.algo == SSWP) ? DIST_INFINITY : 0;
-            if (!async) g_.label2[o_.source] = true;
-        }
-        SUBWAY_CHECK(hipMemcpy(g_.d_value, g_.value, (size_t)n * sizeof(uint), hipMemcpyHostToDevice));
-        SUBWAY_CHECK(hipMemcpy(g_.d_label1, g_.label1, (size_t)n, hipMemcpyHostToDevice));
-        SUBWAY_CHECK(hipMemcpy(g_.d_label2, g_.label2, (size_t)n, hipMemcpyHostToDevice));
+        launch_init_label_state(o_.algo, o_.mode == ASYNC, n, g_->d_value, (uint8_t *)g_->d_label1,
+                                (uint8_t *)g_->d_label2, src_, sc_);
     }
-    SUBWAY_CHECK(hipDeviceSynchronize());
 }
 
 template <class E, class G>
 void Engine<E, G>::generateAndPartition()
 {
+    hostIdsValid_ = false;
     if (o_.profile) pGen_.begin(sc_);
-    if constexpr (isPR) gen_->enqueue(g_, *sub_, o_.acc);
-    else gen_->enqueue(g_, *sub_);
+    if (batchK_)
+        launch_generate_masks(g_->num_nodes, d_m1_, d_m2_, g_->d_outDegree, sub_->d_activeNodes,
+                              sub_->d_activeNodesPointer, gen_->scanScratch(), sc_);
+    else if constexpr (isPR)
+        gen_->enqueue(*g_, *sub_, o_.acc);
+    else
+        gen_->enqueue(*g_, *sub_);
     if (o_.profile) pGen_.end(sc_);
+    if (d_bounds_) {
+        const int P = (int)bounds_.size() - 1;
+        launch_range_stats(sub_->d_activeNodes, sub_->d_activeNodesPointer, gen_->deviceTotals(), d_bounds_, P,
+                           d_rstats_, sc_);
+        SUBWAY_CHECK(hipMemcpyAsync(h_rstats_, d_rstats_, (size_t)P * 4 * sizeof(ull), hipMemcpyDeviceToHost, sc_));
+    }
     part_->partition(*sub_, gen_->deviceTotals(), maxPart_);
     sub_->numActiveNodes = (uint)part_->numActiveNodes;
     sub_->numActiveEdges = part_->numActiveEdges;
 }
 
 template <class E, class G>
+void Engine<E, G>::setupRanges()
+{
+    const ull n = g_->num_nodes;
+    const ull *np = g_->nodePointer;
+    bounds_.assign(1, 0);
+    ull v = 0;
+    while (v < n) {
+        // largest t with np[t] - np[v] <= maxPart_
+        ull lo = v, hi = n;
+        while (lo < hi) {
+            ull mid = lo + (hi - lo + 1) / 2;
+            if (np[mid] - np[v] <= maxPart_) lo = mid;
+            else hi = mid - 1;
+        }
+        v = (lo == v) ? v + 1 : lo;  // a vertex larger than a buffer is a range of its own
+        bounds_.push_back((uint)v);
+    }
+    const size_t P = bounds_.size() - 1;
+    SUBWAY_CHECK(hipMalloc(&d_bounds_, bounds_.size() * sizeof(uint32_t)));
+    SUBWAY_CHECK(hipMemcpy(d_bounds_, bounds_.data(), bounds_.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SUBWAY_CHECK(hipMalloc(&d_rstats_, P * 4 * sizeof(ull)));
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_rstats_, P * 4 * sizeof(ull), 0));
+}
+
+template <class E, class G>
+void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, ull edges)
+{
+    const ull base = g_->nodePointer[v];
+    for (ull c = 0; c * maxPart_ < edges; c++) {
+        Unit h{};
+        h.part = part;
+        h.hub = true;
+        h.chunk = c;
+        h.dma = true;
+        h.hostEdge = base + c * maxPart_;
+        h.count = std::min(maxPart_, edges - c * maxPart_);
+        setItems(h, fromNode, 1, fromEdge);
+        units_.push_back(h);
+    }
+}
+
+// Learned schedule: ranges dense enough (active edges / range edges >=
+// theta_) move whole by DMA and their rounds cover every vertex of the
+// range; the other ranges' active slices are packed into compacted units of
+// at most one buffer (DMA when they form a contiguous fully active run,
+// else gathered).
+template <class E, class G>
+void Engine<E, G>::buildUnitsLearned()
+{
+    units_.clear();
+    const ull *np = g_->nodePointer;
+    const size_t P = bounds_.size() - 1;
+    uint part = 0;
+    bool open = false;
+    ull gLo = 0, gHi = 0, gEdges = 0, gBase = 0;
+    bool gFull = true;
+    uint gFirstRange = 0;
+    auto flush = [&]() {
+        if (!open) return;
+        Unit u{};
+        u.part = part++;
+        u.count = gEdges;
+        u.kbeg = gLo;
+        u.kend = gHi;
+        u.edgeBase = gBase;
+        u.dma = gFull || gEdges == 0;
+        u.hostEdge = np[bounds_[gFirstRange]];
+        if (o_.transfer == TRANSFER_ZEROCOPY && gEdges) u.dma = false;
+        setItems(u, gLo, gHi - gLo, gBase);
+        units_.push_back(u);
+        open = false;
+    };
+    for (size_t i = 0; i < P; i++) {
+        const ull lo = h_rstats_[4 * i], hi = h_rstats_[4 * i + 1], ea = h_rstats_[4 * i + 2];
+        const ull base = h_rstats_[4 * i + 3];
+        if (hi == lo) {
+            flush();
+            continue;
+        }
+        const uint b0 = bounds_[i], b1 = bounds_[i + 1];
+        const ull re = np[b1] - np[b0];
+        const bool hub = (b1 - b0 == 1) && re > maxPart_;
+        if (hub) {
+            flush();
+            pushHubUnits(part++, b0, lo, base, re);
+            continue;
+        }
+        if (re > 0 && (double)ea >= theta_ * (double)re) {
+            flush();
+            Unit u{};
+            u.part = part++;
+            u.dma = true;
+            u.hostEdge = np[b0];
+            u.count = re;
+            u.whole = true;
+            u.n = b1 - b0;
+            u.nodes = nullptr;
+            u.ptr = g_->d_nodePointer + b0;
+            u.fromEdge = np[b0];
+            u.idBase = b0;
+            units_.push_back(u);
+            continue;
+        }
+        const bool full = (hi - lo) == (ull)(b1 - b0);
+        if (open && gEdges + ea > maxPart_) flush();
+        if (!open) {
+            open = true;
+            gLo = lo;
+            gHi = hi;
+            gEdges = ea;
+            gBase = base;
+            gFull = full;
+            gFirstRange = (uint)i;
+        } else {
+            // consecutive ranges: their active slices are adjacent
+            gFull = gFull && full;
+            gHi = hi;
+            gEdges += ea;
+        }
+    }
+    flush();
+}
+
+template <class E, class G>
 void Engine<E, G>::buildUnits()
 {
+    if (d_bounds_) {
+        buildUnitsLearned();
+        return;
+    }
     units_.clear();
     const Partitioner<E> &p = *part_;
     for (uint i = 0; i < p.numPartitions; i++) {
@@ -255,12 +508,13 @@ void Engine<E, G>::buildUnits()
             // nothing to move; labels of its vertices are still processed
             u.dma = true;
             u.count = 0;
+            setItems(u, p.fromNode[i], cnt, p.fromEdge[i]);
             units_.push_back(u);
             continue;
         }
         if (cnt == 1 && edges > maxPart_) {
             const uint v = p.firstVertex[i];
-            const ull base = g_.nodePointer[v];
+            const ull base = g_->nodePointer[v];
             for (ull c = 0; c * maxPart_ < edges; c++) {
                 Unit h{};
                 h.part = i;
@@ -269,6 +523,7 @@ void Engine<E, G>::buildUnits()
                 h.dma = true;
                 h.hostEdge = base + c * maxPart_;
                 h.count = std::min(maxPart_, edges - c * maxPart_);
+                setItems(h, p.fromNode[i], cnt, p.fromEdge[i]);
                 units_.push_back(h);
             }
             continue;
@@ -276,7 +531,7 @@ void Engine<E, G>::buildUnits()
         const bool contiguous = (ull)p.lastVertex[i] - (ull)p.firstVertex[i] + 1 == cnt;
         if (contiguous && o_.transfer != TRANSFER_ZEROCOPY) {
             u.dma = true;
-            u.hostEdge = g_.nodePointer[p.firstVertex[i]];
+            u.hostEdge = g_->nodePointer[p.firstVertex[i]];
         } else {
             u.dma = false;
         }
@@ -284,6 +539,7 @@ void Engine<E, G>::buildUnits()
         u.kbeg = p.fromNode[i];
         u.kend = p.fromNode[i] + cnt;
         u.edgeBase = p.fromEdge[i];
+        setItems(u, p.fromNode[i], cnt, p.fromEdge[i]);
         units_.push_back(u);
     }
 }
@@ -298,12 +554,18 @@ void Engine<E, G>::issue(size_t idx)
         if (o_.profile) pXfer_.begin(sx_);
         E *dst = sub_->d_buffers[b];
         if (u.dma) {
-            SUBWAY_CHECK(hipMemcpyAsync(dst, g_.edgeList + u.hostEdge, (size_t)u.count * sizeof(E),
+            SUBWAY_CHECK(hipMemcpyAsync(dst, g_->edgeList + u.hostEdge, (size_t)u.count * sizeof(E),
                                         hipMemcpyHostToDevice, sx_));
+            st_.h2dDmaBytes += u.count * sizeof(E);
+        } else if (o_.transfer == TRANSFER_HOST) {
+            hostGather(u, b);
+            SUBWAY_CHECK(hipMemcpyAsync(dst, sub_->stagingBuffer(b), (size_t)u.count * sizeof(E),
+                                        hipMemcpyHostToDevice, sx_));
+            SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
             st_.h2dDmaBytes += u.count * sizeof(E);
         } else {
             launch_gather((int)sizeof(E), sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend,
-                          g_.d_nodePointer, g_.d_edgeList, dst, u.edgeBase, u.count, sx_);
+                          g_->d_nodePointer, g_->d_edgeList, dst, u.edgeBase, u.count, sx_);
             st_.h2dGatherBytes += u.count * sizeof(E);
         }
         st_.h2dBytes += u.count * sizeof(E);
@@ -312,24 +574,69 @@ void Engine<E, G>::issue(size_t idx)
     SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));
 }
 
+// Host-thread gather (the reference's dynamic<E>, subgraph_generator.cu:63-97,
+// without its tail bug): the partition's active ids are mirrored once per
+// global iteration; chunks of the active range are claimed by host threads,
+// whose destination offsets come from a parallel prefix of chunk edge sums;
+// the result lands in the pinned staging buffer of device buffer b.
+template <class E, class G>
+void Engine<E, G>::hostGather(const Unit &u, int b)
+{
+    if (!hostIdsValid_) {
+        const ull A = sub_->numActiveNodes;
+        SUBWAY_CHECK(hipMemcpyAsync(pinnedIds(A), sub_->d_activeNodes, A * sizeof(uint), hipMemcpyDeviceToHost, sc_));
+        SUBWAY_CHECK(hipStreamSynchronize(sc_));
+        hostIdsValid_ = true;
+    }
+    // the staging buffer is reused only after the copy out of it finished
+    SUBWAY_CHECK(hipEventSynchronize(evStaged_[b]));
+    E *stage = sub_->stagingBuffer(b);
+    const uint *ids = h_ids_ + u.kbeg;
+    const ull cnt = u.kend - u.kbeg;
+    const ull *np = g_->nodePointer;
+    const E *src = g_->edgeList;
+    const int T = std::max(1, std::min(host_threads(), 64));
+    const ull nch = std::min<ull>(cnt, (ull)T * 8);
+    std::vector<ull> chunkEdges(nch + 1, 0);
+    parallel_for(nch, [&](ull c0, ull c1, int) {
+        for (ull c = c0; c < c1; c++) {
+            ull v0 = cnt * c / nch, v1 = cnt * (c + 1) / nch, e = 0;
+            for (ull i = v0; i < v1; i++) e += np[ids[i] + 1] - np[ids[i]];
+            chunkEdges[c + 1] = e;
+        }
+    }, T);
+    for (ull c = 0; c < nch; c++) chunkEdges[c + 1] += chunkEdges[c];
+    if (chunkEdges[nch] != u.count) throw Error(ERR_INTERNAL, "host gather: edge count mismatch");
+    parallel_for(nch, [&](ull c0, ull c1, int) {
+        for (ull c = c0; c < c1; c++) {
+            ull v0 = cnt * c / nch, v1 = cnt * (c + 1) / nch, at = chunkEdges[c];
+            for (ull i = v0; i < v1; i++) {
+                const ull a = np[ids[i]], d = np[ids[i] + 1] - a;
+                std::memcpy(stage + at, src + a, d * sizeof(E));
+                at += d;
+            }
+        }
+    }, T);
+}
+
 template <class E, class G>
 void Engine<E, G>::frontier(int first, uint8_t *labA, uint8_t *labB, const Unit &un)
 {
-    const Partitioner<E> &p = *part_;
     FrontierArgs a{};
     a.algo = o_.algo;
-    a.n = p.partitionNodeSize[un.part];
-    a.nodes = sub_->d_activeNodes + p.fromNode[un.part];
-    a.ptr = sub_->d_activeNodesPointer + p.fromNode[un.part];
-    a.fromEdge = p.fromEdge[un.part];
+    a.n = un.n;
+    a.nodes = un.nodes;
+    a.ptr = un.ptr;
+    a.fromEdge = un.fromEdge;
+    a.idBase = un.idBase;
     a.labA = labA;
     a.labB = labB;
     if constexpr (isPR) {
-        a.value = (uint32_t *)g_.d_value;
-        a.delta = g_.d_delta;
+        a.value = (uint32_t *)g_->d_value;
+        a.delta = g_->d_delta;
         a.acc = o_.acc;
     } else {
-        a.value = g_.d_value;
+        a.value = g_->d_value;
     }
     a.entries = entries_;
     a.chunkMap = chunkMap_;
@@ -349,9 +656,9 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf)
     a.chunkMap = chunkMap_;
     a.edges = sub_->d_buffers[buf];
     if constexpr (isPR) {
-        a.delta = g_.d_delta;
+        a.delta = g_->d_delta;
     } else {
-        a.value = g_.d_value;
+        a.value = g_->d_value;
         a.labelOut = labOut;
     }
     a.w0 = w0;
@@ -368,8 +675,40 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
 {
     uint8_t *l1 = nullptr, *l2 = nullptr;
     if constexpr (!isPR) {
-        l1 = (uint8_t *)g_.d_label1;
-        l2 = (uint8_t *)g_.d_label2;
+        l1 = (uint8_t *)g_->d_label1;
+        l2 = (uint8_t *)g_->d_label2;
+    }
+    if (roundsGrid_ > 0) {
+        // every round of the partition in one cooperative launch; a partition
+        // that has not converged after maxRounds keeps its labels and is
+        // picked up by the next generate, so no host check is needed
+        RoundsLaunch l{};
+        l.n = un.n;
+        l.nodes = un.nodes;
+        l.ptr = un.ptr;
+        l.fromEdge = un.fromEdge;
+        l.idBase = un.idBase;
+        l.l1 = l1;
+        l.l2 = l2;
+        if constexpr (isPR) {
+            l.value = (uint32_t *)g_->d_value;
+            l.delta = g_->d_delta;
+            l.acc = o_.acc;
+        } else {
+            l.value = g_->d_value;
+        }
+        l.entries = entries_;
+        l.chunkMap = chunkMap_;
+        l.edges = sub_->d_buffers[buf];
+        l.sc = fr_.ctl;
+        l.pc = pc_;
+        l.td = fr_.td;
+        l.maxRounds = (uint32_t)std::max(1, roundsCap_);
+        l.roundBase = 0;
+        if (o_.profile) pRounds_.begin(sc_);
+        launch_rounds(o_.algo, (int)sizeof(E), l, roundsGrid_, sc_);
+        if (o_.profile) pRounds_.end(sc_);
+        return;
     }
     ull r = 0;
     int batch = 2;
@@ -396,11 +735,13 @@ void Engine<E, G>::compute(size_t idx)
     SUBWAY_CHECK(hipStreamWaitEvent(sc_, evLoaded_[b], 0));
     uint8_t *l1 = nullptr, *l2 = nullptr;
     if constexpr (!isPR) {
-        l1 = (uint8_t *)g_.d_label1;
-        l2 = (uint8_t *)g_.d_label2;
+        l1 = (uint8_t *)g_->d_label1;
+        l2 = (uint8_t *)g_->d_label2;
     }
     const bool async = o_.mode == ASYNC;
-    if (u.hub) {
+    if (batchK_) {
+        computeBatch(u, b);
+    } else if (u.hub) {
         // one round over the oversized vertex, streamed chunk by chunk: the
         // frontier kernel runs once (clearing its label, fixing its value),
         // then each chunk is relaxed through a window of the work space
@@ -420,10 +761,45 @@ void Engine<E, G>::compute(size_t idx)
     }
     SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
     if (o_.verbose && (!u.hub || u.chunk == 0)) {
-        std::cout << "Global Iteration " << st_.globalIterations << ", Partition " << u.part << " of "
-                  << part_->numPartitions << ": " << part_->partitionNodeSize[u.part] << " vertices, "
-                  << part_->partitionEdgeSize[u.part] << " edges\n";
+        std::cout << "Global Iteration " << st_.globalIterations << ", Partition " << u.part << ": " << u.n
+                  << (u.whole ? " vertices (whole range), " : " vertices, ") << u.count << " edges\n";
     }
+}
+
+// Batched partition: all rounds for every source in one cooperative launch;
+// an oversized vertex gets one masked frontier and windowed advances.
+template <class E, class G>
+void Engine<E, G>::computeBatch(const Unit &u, int b)
+{
+    if (u.hub) {
+        if (u.chunk == 0)
+            launch_frontier_masks(u.n, u.nodes, u.ptr, u.fromEdge, d_m1_, d_m2_, entries_, d_entryMask_, chunkMap_,
+                                  fr_, sc_, u.idBase);
+        const ull w0 = u.chunk * maxPart_;
+        launch_advance_batch_window(o_.algo, (int)sizeof(E), entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
+                                    d_dist_, d_m2_, w0, w0 + u.count, w0, fr_.ctl, advGrid_, sc_);
+        return;
+    }
+    BatchLaunch l{};
+    l.n = u.n;
+    l.nodes = u.nodes;
+    l.ptr = u.ptr;
+    l.fromEdge = u.fromEdge;
+    l.idBase = u.idBase;
+    l.m1 = d_m1_;
+    l.m2 = d_m2_;
+    l.dist = d_dist_;
+    l.entries = entries_;
+    l.entryMask = d_entryMask_;
+    l.chunkMap = chunkMap_;
+    l.edges = sub_->d_buffers[b];
+    l.sc = fr_.ctl;
+    l.pc = pc_;
+    l.td = fr_.td;
+    l.maxRounds = (uint32_t)std::max(1, roundsCap_);
+    if (o_.profile) pRounds_.begin(sc_);
+    launch_rounds_batch(o_.algo, (int)sizeof(E), batchUnroll_, l, batchGrid_, sc_);
+    if (o_.profile) pRounds_.end(sc_);
 }
 
 template <class E, class G>
@@ -435,26 +811,25 @@ ull Engine<E, G>::deviceRounds()
 }
 
 template <class E, class G>
-RunStats Engine<E, G>::run()
+void Engine<E, G>::resetRun()
 {
-    initState();
-    const uint n = g_.num_nodes;
-    ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
-    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum;
-    for (int b = 0; b < 2; b++) SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
-    SUBWAY_CHECK(hipDeviceSynchronize());
+    st_ = RunStats();
+    roundsCap_ = o_.maxRounds;
+    theta_ = o_.theta;
+    pAdv_.reset();
+    pFr_.reset();
+    pXfer_.reset();
+    pGen_.reset();
+    pRounds_.reset();
+}
 
-    const auto t0 = clk::now();
-    generateAndPartition();
-    if constexpr (!isPR) {
-        if (o_.mode == ASYNC && o_.algo != CC) {
-            // every vertex was active for the first generate; from now on only
-            // the source is labelled (sssp-async.cu:59-64)
-            launch_fill_u8((uint8_t *)g_.d_label1, 0, n, sc_);
-            static const uint8_t one = 1;
-            SUBWAY_CHECK(hipMemcpyAsync((uint8_t *)g_.d_label1 + o_.source, &one, 1, hipMemcpyHostToDevice, sc_));
-        }
-    }
+// The global-iteration loop shared by single-source and batched runs; the
+// first generate has been enqueued by the caller.  Returns with the device
+// work of the last iteration drained from the copy stream.
+template <class E, class G>
+RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
+{
+    (void)t0;
     while (sub_->numActiveNodes > 0) {
         const auto ti = clk::now();
         st_.globalIterations++;
@@ -472,18 +847,98 @@ RunStats Engine<E, G>::run()
             while (issued < nu && issued <= u + ahead) issue(issued++);
             compute(u);
         }
-        st_.partitions += part_->numPartitions;
+        st_.partitions += d_bounds_ ? (units_.empty() ? 0 : units_.back().part + 1) : part_->numPartitions;
         rec.h2dBytes = st_.h2dBytes - h2d0;
         generateAndPartition();
         rec.msIteration = ms_since(ti);
         st_.trace.push_back(rec);
     }
+    return st_;
+}
+
+template <class E, class G>
+RunStats Engine<E, G>::run(uint source)
+{
+    if (batchK_ || o_.batch) throw Error(ERR_ARG, "this engine runs batched queries (runBatch)");
+    if (o_.algo != PR && o_.algo != CC && source >= g_->num_nodes)
+        throw Error(ERR_ARG, "source vertex " + std::to_string(source) + " out of range");
+    src_ = source;
+    resetRun();
+    initState();
+    const uint n = g_->num_nodes;
+    ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
+    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum;
+    for (int b = 0; b < 2; b++) {
+        SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
+        SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
+    }
+    SUBWAY_CHECK(hipDeviceSynchronize());
+
+    const auto t0 = clk::now();
+    generateAndPartition();
+    if constexpr (!isPR) {
+        if (o_.mode == ASYNC && o_.algo != CC) {
+            // every vertex was active for the first generate; from now on only
+            // the source is labelled (sssp-async.cu:59-64)
+            launch_fill_u8((uint8_t *)g_->d_label1, 0, n, sc_);
+            static const uint8_t one = 1;
+            SUBWAY_CHECK(hipMemcpyAsync((uint8_t *)g_->d_label1 + src_, &one, 1, hipMemcpyHostToDevice, sc_));
+        }
+    }
+    loop(t0);
     if constexpr (isPR)
-        SUBWAY_CHECK(hipMemcpyAsync(g_.value, g_.d_value, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, sc_));
+        SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, sc_));
     else
-        SUBWAY_CHECK(hipMemcpyAsync(g_.value, g_.d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
+        SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
+    finishStats(rounds0, updates0, work0, entries0);
+    return st_;
+}
+
+// k sources (k <= 64) advance together: lane i of every distance row and bit
+// i of every mask belong to sources[i].  Like the async apps, the first
+// generate sees every vertex active (the whole graph moves once), then only
+// the sources' bits are set.
+template <class E, class G>
+RunStats Engine<E, G>::runBatch(const uint *sources, int k)
+{
+    if (!o_.batch) throw Error(ERR_ARG, "the engine was not built for batched queries");
+    if (k < 1 || k > BATCH_LANES) throw Error(ERR_ARG, "a batch holds 1..64 sources");
+    const uint n = g_->num_nodes;
+    for (int i = 0; i < k; i++)
+        if (sources[i] >= n) throw Error(ERR_ARG, "batch source " + std::to_string(sources[i]) + " out of range");
+    batchK_ = k;
+    resetRun();
+    std::memcpy(h_sources_, sources, (size_t)k * sizeof(uint32_t));
+    SUBWAY_CHECK(hipMemcpyAsync(d_sources_, h_sources_, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, sc_));
+    launch_init_batch_state(o_.algo, n, d_dist_, d_m1_, d_m2_, 1, sc_);
+    ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
+    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum;
+    for (int b = 0; b < 2; b++) {
+        SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
+        SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
+    }
+    if (!h_dist_) SUBWAY_CHECK(hipHostMalloc((void **)&h_dist_, (size_t)n * BATCH_LANES * sizeof(uint32_t), 0));
+    SUBWAY_CHECK(hipDeviceSynchronize());
+
+    const auto t0 = clk::now();
+    generateAndPartition();
+    launch_fill_u32((uint32_t *)d_m1_, 0, 2ull * n, sc_);
+    launch_set_batch_sources(o_.algo, d_sources_, k, d_dist_, d_m1_, sc_);
+    loop(t0);
+    SUBWAY_CHECK(hipMemcpyAsync(h_dist_, d_dist_, (size_t)n * BATCH_LANES * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                sc_));
+    SUBWAY_CHECK(hipStreamSynchronize(sc_));
+    st_.msTotal = ms_since(t0);
+    batchK_ = 0;
+    finishStats(rounds0, updates0, work0, entries0);
+    return st_;
+}
+
+template <class E, class G>
+void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0)
+{
     SUBWAY_CHECK(hipStreamSynchronize(sx_));
     st_.innerRounds = deviceRounds() - rounds0;
     st_.updates = h_ctl_->updates - updates0;
@@ -495,11 +950,61 @@ RunStats Engine<E, G>::run()
         st_.msFrontier = pFr_.total(st_.frontierLaunches);
         st_.msTransfer = pXfer_.total(st_.transferOps);
         st_.msGenerate = pGen_.total(c);
+        st_.msRounds = pRounds_.total(st_.roundsLaunches);
     }
-    return st_;
 }
 
+template <class E, class G>
+class SessionImpl : public Session {
+public:
+    SessionImpl(std::shared_ptr<HostCsr<E>> csr, const RunOptions &o) : eng_(std::move(csr), o), algo_(o.algo) {}
+    RunStats run(uint source) override { return eng_.run(source); }
+    void values(void *out) override
+    {
+        G &g = eng_.graph();
+        std::memcpy(out, g.value, (size_t)g.num_nodes * (algo_ == PR ? sizeof(float) : sizeof(uint)));
+    }
+    ull numNodes() const override { return const_cast<SessionImpl *>(this)->eng_.graph().num_nodes; }
+    RunStats runBatch(const uint *sources, int k) override { return eng_.runBatch(sources, k); }
+    void rows(uint *out) override
+    {
+        const uint32_t *r = eng_.batchRows();
+        if (!r) throw Error(ERR_ARG, "no batched run yet");
+        std::memcpy(out, r, (size_t)numNodes() * BATCH_LANES * sizeof(uint32_t));
+    }
+    int tuneBatch(const uint *sources, int k, double *timesMs) override
+    {
+        static const int cand[4] = {1, 2, 4, 8};
+        int best = eng_.batchUnroll();
+        double bestMs = 1e300;
+        for (int i = 0; i < 4; i++) {
+            eng_.setBatchUnroll(cand[i]);
+            RunStats st = eng_.runBatch(sources, k);
+            if (timesMs) timesMs[i] = st.msTotal;
+            if (st.msTotal < bestMs) {
+                bestMs = st.msTotal;
+                best = cand[i];
+            }
+        }
+        eng_.setBatchUnroll(best);
+        return best;
+    }
+
+private:
+    Engine<E, G> eng_;
+    int algo_;
+};
+
 }  // namespace
+
+template <class E>
+std::unique_ptr<Session> make_session(std::shared_ptr<HostCsr<E>> csr, const RunOptions &o)
+{
+    if ((o.algo == SSSP || o.algo == SSWP) && sizeof(E) != 8)
+        throw Error(ERR_ARG, "SSSP/SSWP need a weighted graph (.bwcsr/.wel)");
+    if (o.algo == PR) return std::unique_ptr<Session>(new SessionImpl<E, GraphPR<E>>(std::move(csr), o));
+    return std::unique_ptr<Session>(new SessionImpl<E, Graph<E>>(std::move(csr), o));
+}
 
 template <class E>
 RunStats runLabel(Graph<E> &graph, const RunOptions &opt)
@@ -508,7 +1013,7 @@ RunStats runLabel(Graph<E> &graph, const RunOptions &opt)
     if ((opt.algo == SSSP || opt.algo == SSWP) && sizeof(E) != 8)
         throw Error(ERR_ARG, "SSSP/SSWP need a weighted graph (.bwcsr/.wel)");
     Engine<E, Graph<E>> eng(graph, opt);
-    return eng.run();
+    return eng.run(opt.source);
 }
 
 template <class E>
@@ -517,9 +1022,12 @@ RunStats runPageRank(GraphPR<E> &graph, const RunOptions &opt)
     RunOptions o = opt;
     o.algo = PR;
     Engine<E, GraphPR<E>> eng(graph, o);
-    return eng.run();
+    return eng.run(0);
 }
 
+template std::unique_ptr<Session> make_session<OutEdge>(std::shared_ptr<HostCsr<OutEdge>>, const RunOptions &);
+template std::unique_ptr<Session> make_session<OutEdgeWeighted>(std::shared_ptr<HostCsr<OutEdgeWeighted>>,
+                                                                const RunOptions &);
 template RunStats runLabel<OutEdge>(Graph<OutEdge> &, const RunOptions &);
 template RunStats runLabel<OutEdgeWeighted>(Graph<OutEdgeWeighted> &, const RunOptions &);
 template RunStats runPageRank<OutEdge>(GraphPR<OutEdge> &, const RunOptions &);

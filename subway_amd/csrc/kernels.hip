@@ -30,31 +30,19 @@ namespace subway {
 
 using namespace dev;
 
-// ====================================================== select-scan kernel
+// ====================================================== select-scan tile
+// One tile of flag -> workgroup scan -> decoupled lookback -> compaction.
+// `lastTile` names the tile that publishes the totals through op.finish.
 template <class Op, int BLOCK, int IPT>
-__global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCtl *ctl,
-                                                            TileDesc td, int first)
+__device__ __forceinline__ void scan_tile(const Op &op, u64 n, u32 tile, u32 epoch, u32 lastTile,
+                                          const TileDesc &td, ScanCtl *ctl)
 {
     constexpr int NW = BLOCK / WAVE;
-    __shared__ u32 s_tile, s_epoch;
-    __shared__ int s_exit;
     __shared__ u32 s_wcnt[NW];
     __shared__ u64 s_wsum[NW];
     __shared__ u32 s_xcnt;
     __shared__ u64 s_xsum;
-
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        int ex = (!first && ld_relaxed(&ctl->stop)) ? 1 : 0;
-        s_exit = ex;
-        if (!ex) {
-            s_epoch = ld_relaxed(&ctl->epoch);
-            s_tile = fetch_add_acq_rel(&ctl->tile_counter, 1u);
-        }
-    }
-    __syncthreads();
-    if (s_exit) return;
-    const u32 tile = s_tile, epoch = s_epoch;
 
     const u64 i0 = (u64)tile * (u64)(BLOCK * IPT) + (u64)tid * IPT;
     typename Op::Item it[IPT];
@@ -107,8 +95,30 @@ __global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCt
             pre += w[j];
         }
     }
-    if (tile == gridDim.x - 1 && tid == 0) op.finish((u64)s_xcnt + totc, s_xsum + tots, ctl);
+    if (tile == lastTile && tid == 0) op.finish((u64)s_xcnt + totc, s_xsum + tots, ctl);
     __syncthreads();
+}
+
+// Standalone launch: one workgroup per tile, tiles claimed in start order.
+template <class Op, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCtl *ctl,
+                                                            TileDesc td, int first)
+{
+    __shared__ u32 s_tile, s_epoch;
+    __shared__ int s_exit;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int ex = (!first && ld_relaxed(&ctl->stop)) ? 1 : 0;
+        s_exit = ex;
+        if (!ex) {
+            s_epoch = ld_relaxed(&ctl->epoch);
+            s_tile = fetch_add_acq_rel(&ctl->tile_counter, 1u);
+        }
+    }
+    __syncthreads();
+    if (s_exit) return;
+    const u32 tile = s_tile, epoch = s_epoch;
+    scan_tile<Op, BLOCK, IPT>(op, n, tile, epoch, gridDim.x - 1, td, ctl);
     if (tid == 0) {
         __threadfence();
         u32 d = fetch_add_acq_rel(&ctl->done_counter, 1u);
@@ -237,14 +247,15 @@ struct FrItem {
     u32 val;
 };
 
+template <int CHUNK = ADV_CHUNK>
 __device__ __forceinline__ void write_entry(Entry *entries, u32 *chunkMap, u64 k, u64 pre,
                                             u32 loc, u32 val, u64 deg)
 {
     uint4 e = make_uint4(loc, (u32)pre, val, 0u);
     *reinterpret_cast<uint4 *>(entries + k) = e;
     // chunk j starts inside [pre, pre + deg): record the entry that holds it
-    u64 j = (pre + ADV_CHUNK - 1) / ADV_CHUNK;
-    u64 jend = (pre + deg - 1) / ADV_CHUNK;
+    u64 j = (pre + CHUNK - 1) / CHUNK;
+    u64 jend = (pre + deg - 1) / CHUNK;
     for (; j <= jend; j++) chunkMap[j] = (u32)k;
 }
 
@@ -259,6 +270,7 @@ struct FrontierLabelOp {
     const u32 *value;
     Entry *entries;
     u32 *chunkMap;
+    u32 idBase;    // items are ids idBase + i when nodes is null (a whole id range)
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -266,7 +278,7 @@ struct FrontierLabelOp {
         for (int j = 0; j < FR_IPT; j++) {
             u64 i = i0 + j;
             if (i < n) {
-                u32 id = nodes[i];
+                u32 id = nodes ? nodes[i] : idBase + (u32)i;
                 uint8_t f = labA[id];
                 if (labB) f |= labB[id];
                 if (f) {
@@ -310,6 +322,7 @@ struct FrontierPrOp {
     float acc;
     Entry *entries;
     u32 *chunkMap;
+    u32 idBase;
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -317,7 +330,7 @@ struct FrontierPrOp {
         for (int j = 0; j < FR_IPT; j++) {
             u64 i = i0 + j;
             if (i < n) {
-                u32 id = nodes[i];
+                u32 id = nodes ? nodes[i] : idBase + (u32)i;
                 float d = delta[id];
                 if (d > acc) {
                     value[id] += d;
@@ -337,6 +350,101 @@ struct FrontierPrOp {
     __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
     {
         write_entry(entries, chunkMap, k, pre, it.loc, it.val, deg);
+    }
+    __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
+    {
+        ctl->count = cnt;
+        ctl->total = sum;
+        ctl->stop = (cnt == 0) ? 1u : 0u;
+        if (cnt) ctl->rounds += 1;
+        ctl->entry_sum += cnt;
+        ctl->work_sum += sum;
+    }
+};
+
+// ------------------------------------------------- batched (k sources)
+// Per-vertex 64-bit activeness masks, one bit per source (lane).
+constexpr int BATCH_CHUNK = 256;
+
+// activeness = (mask1 | mask2) != 0
+struct GenMaskOp {
+    typedef Empty Item;
+    const u64 *m1, *m2;
+    const u32 *outDegree;
+    u32 *activeNodes;
+    u64 *activeNodesPointer;
+
+    __device__ __forceinline__ void load(u64 i0, u64 n, Item *, u64 *w, u32 &mask) const
+    {
+#pragma unroll
+        for (int j = 0; j < GEN_IPT; j++) {
+            u64 i = i0 + j;
+            if (i < n && (m1[i] | m2[i])) {
+                mask |= 1u << j;
+                w[j] = outDegree[i];
+            }
+        }
+    }
+    __device__ __forceinline__ void emit(u64 k, u64 pre, u64 i, const Item &, u64) const
+    {
+        activeNodes[k] = (u32)i;
+        activeNodesPointer[k] = pre;
+    }
+    __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
+    {
+        activeNodesPointer[cnt] = sum;
+        ctl->count = cnt;
+        ctl->total = sum;
+    }
+};
+
+struct FrMaskItem {
+    u32 loc;
+    u32 id;
+    u64 m;
+};
+
+// A vertex with any source bit set pushes for those sources; its masks are
+// cleared (the batched form of the label test/clear).
+struct FrontierMaskOp {
+    typedef FrMaskItem Item;
+    const u32 *nodes;
+    const u64 *ptr;
+    u64 fromEdge;
+    u64 *labA, *labB;
+    Entry *entries;
+    u64 *entryMask;
+    u32 *chunkMap;
+    u32 idBase;
+
+    __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
+    {
+#pragma unroll
+        for (int j = 0; j < FR_IPT; j++) {
+            u64 i = i0 + j;
+            if (i < n) {
+                u32 id = nodes ? nodes[i] : idBase + (u32)i;
+                u64 m = labA[id];
+                if (labB) m |= labB[id];
+                if (m) {
+                    labA[id] = 0;
+                    if (labB) labB[id] = 0;
+                    u64 p0 = ptr[i], p1 = ptr[i + 1];
+                    if (p1 > p0) {
+                        mask |= 1u << j;
+                        w[j] = p1 - p0;
+                        it[j].loc = (u32)(p0 - fromEdge);
+                        it[j].id = id;
+                        it[j].m = m;
+                    }
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
+    {
+        write_entry<BATCH_CHUNK>(entries, chunkMap, k, pre, it.loc, it.id, deg);
+        entryMask[k] = it.m;
     }
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
@@ -435,17 +543,56 @@ struct Relax<ALGO_PR> {
     }
 };
 
+// Relaxes the edges of one work chunk [cbeg, cbeg + ADV_CHUNK) clipped to
+// [w0, wend): the chunk's entries are staged in LDS and every position finds
+// its entry by binary search.  Returns the successful relaxations of the
+// calling thread.
+template <int ALGO, class E>
+__device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
+                                             u32 *value, uint8_t *lab, float *delta, u64 count, u64 W, u64 w0,
+                                             u64 wend, u64 shift)
+{
+    __shared__ u32 s_pre[ADV_CHUNK + 1];
+    __shared__ u32 s_loc[ADV_CHUNK + 1];
+    __shared__ u32 s_val[ADV_CHUNK + 1];
+    const int tid = threadIdx.x;
+    const u64 cbeg = j * ADV_CHUNK;
+    const u64 pbeg = cbeg > w0 ? cbeg : w0;
+    const u64 pend = (cbeg + ADV_CHUNK) < wend ? (cbeg + ADV_CHUNK) : wend;
+    const u64 k0 = chunkMap[j];
+    const u64 k1 = (cbeg + ADV_CHUNK < W) ? chunkMap[j + 1] : count - 1;
+    const int nk = (int)(k1 - k0 + 1);
+    for (int t = tid; t < nk; t += ADV_BLOCK) {
+        uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
+        s_loc[t] = e.x;
+        s_pre[t] = e.y;
+        s_val[t] = e.z;
+    }
+    __syncthreads();
+    u32 updates = 0;
+    for (u64 p = pbeg + tid; p < pend; p += ADV_BLOCK) {
+        const u32 pp = (u32)p;
+        int lo = 0, hi = nk - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= pp) lo = mid;
+            else hi = mid - 1;
+        }
+        const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
+        const E e = edges[eidx];
+        if (Relax<ALGO>::apply(s_val[lo], e, value, lab, delta)) updates++;
+    }
+    __syncthreads();
+    return updates;
+}
+
 template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries, const u32 *chunkMap,
                                                             const E *edges, u32 *value, uint8_t *lab,
                                                             float *delta, u64 w0, u64 w1, u64 shift,
                                                             ScanCtl *ctl)
 {
-    __shared__ u32 s_pre[ADV_CHUNK + 1];
-    __shared__ u32 s_loc[ADV_CHUNK + 1];
-    __shared__ u32 s_val[ADV_CHUNK + 1];
     __shared__ u32 s_chunk;
-
     const int tid = threadIdx.x;
     const u64 count = ctl->count;
     const u64 W = ctl->total;
@@ -459,34 +606,10 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
         if (tid == 0) s_chunk = atomicAdd(&ctl->adv_counter, 1u);
         __syncthreads();
         const u64 c = s_chunk;
+        __syncthreads();
         if (c >= nchunks) break;
-        const u64 j = j0 + c;
-        const u64 cbeg = j * ADV_CHUNK;
-        const u64 pbeg = cbeg > w0 ? cbeg : w0;
-        const u64 pend = (cbeg + ADV_CHUNK) < wend ? (cbeg + ADV_CHUNK) : wend;
-        const u64 k0 = chunkMap[j];
-        const u64 k1 = (cbeg + ADV_CHUNK < W) ? chunkMap[j + 1] : count - 1;
-        const int nk = (int)(k1 - k0 + 1);
-        for (int t = tid; t < nk; t += ADV_BLOCK) {
-            uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
-            s_loc[t] = e.x;
-            s_pre[t] = e.y;
-            s_val[t] = e.z;
-        }
-        __syncthreads();
-        for (u64 p = pbeg + tid; p < pend; p += ADV_BLOCK) {
-            const u32 pp = (u32)p;
-            int lo = 0, hi = nk - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= pp) lo = mid;
-                else hi = mid - 1;
-            }
-            const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-            const E e = edges[eidx];
-            if (Relax<ALGO>::apply(s_val[lo], e, value, lab, delta)) updates++;
-        }
-        __syncthreads();
+        updates += advance_chunk<ALGO, E>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0, wend,
+                                          shift);
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -497,6 +620,394 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
             st_relaxed(&ctl->adv_done, 0u);
             st_relaxed(&ctl->adv_counter, 0u);
         }
+    }
+}
+
+// ================================================== persistent rounds
+// Two-level grid barrier for a cooperative launch (every workgroup resident).
+// Thread 0 of each workgroup releases the workgroup's writes at agent scope,
+// arrives at its group counter, the last of a group arrives at the top
+// counter, and the last overall bumps the generation the others spin on.
+__device__ __forceinline__ void grid_sync(PersistCtl *pc, u32 &gen)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const u32 g = gen;
+        const u32 ngroups = gridDim.x < 8u ? gridDim.x : 8u;
+        const u32 grp = blockIdx.x % ngroups;
+        const u32 grpSize = (gridDim.x - grp + ngroups - 1) / ngroups;
+        __threadfence();
+        bool releaser = false;
+        u32 a = __hip_atomic_fetch_add(&pc->bar_sub[grp * 16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (a == grpSize) {
+            __hip_atomic_store(&pc->bar_sub[grp * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            u32 t = __hip_atomic_fetch_add(&pc->bar_top[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            if (t == ngroups) {
+                __hip_atomic_store(&pc->bar_top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&pc->bar_gen[0], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                releaser = true;
+            }
+        }
+        if (!releaser) {
+            while (__hip_atomic_load(&pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+                __builtin_amdgcn_s_sleep(1);
+        }
+        gen = g + 1;
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+struct RoundsArgs {
+    ull n;                 // items of the partition
+    const u32 *nodes;      // activeNodes + fromNode
+    const ull *ptr;        // activeNodesPointer + fromNode
+    ull fromEdge;
+    uint8_t *l1, *l2;      // label arrays (label algorithms)
+    u32 *value;            // u32 values or PR float values
+    float *delta;          // PR
+    float acc;             // PR
+    Entry *entries;
+    u32 *chunkMap;
+    const void *edges;     // device partition buffer
+    ScanCtl *sc;
+    PersistCtl *pc;
+    TileDesc td;
+    u32 maxRounds;
+    u32 roundBase;         // round index (within the partition) of this launch's first round
+    u32 idBase;            // nodes == null: items are the id range idBase ..
+};
+
+// All async inner rounds of one resident partition in one cooperative
+// launch: phase A selects the labelled vertices (scan tiles claimed by
+// counter), grid barrier, phase B relaxes their edges (work chunks claimed by
+// counter), grid barrier; the loop ends, uniformly for every workgroup, on
+// the first round that selects nothing or after maxRounds rounds.
+template <int ALGO, class E>
+__global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
+{
+    __shared__ u32 s_claim, s_gen, s_epoch;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_epoch = ld_relaxed(&a.sc->epoch);
+    }
+    __syncthreads();
+    u32 gen = s_gen;
+    const u32 epoch0 = s_epoch;
+    const u32 numTiles = (u32)((a.n + FR_TILE - 1) / FR_TILE);
+    u32 r = 0, conv = 0, updates = 0;
+    for (; r < a.maxRounds; r++) {
+        const u32 rg = a.roundBase + r;
+        const bool even = (rg & 1u) == 0;
+        uint8_t *labIn = even ? a.l1 : a.l2;
+        uint8_t *labOut = even ? a.l2 : a.l1;
+        // phase A: frontier
+        for (;;) {
+            if (tid == 0) s_claim = atomicAdd(&a.pc->tile_ctr[r & 1], 1u);
+            __syncthreads();
+            const u32 t = s_claim;
+            __syncthreads();
+            if (t >= numTiles) break;
+            if constexpr (ALGO == ALGO_PR) {
+                FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
+                                a.idBase};
+                scan_tile<FrontierPrOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+            } else {
+                FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
+                                   a.entries, a.chunkMap, a.idBase};
+                scan_tile<FrontierLabelOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+            }
+        }
+        grid_sync(a.pc, gen);
+        const u64 count = ld_relaxed(&a.sc->count);
+        const u64 W = ld_relaxed(&a.sc->total);
+        if (count == 0) {
+            conv = 1;
+            break;
+        }
+        if (blockIdx.x == 0 && tid == 0) {
+            st_relaxed(&a.pc->tile_ctr[(r + 1) & 1], 0u);
+            st_relaxed(&a.pc->chunk_ctr[(r + 1) & 1], 0u);
+        }
+        // phase B: advance
+        const u64 nchunks = (W + ADV_CHUNK - 1) / ADV_CHUNK;
+        for (;;) {
+            if (tid == 0) s_claim = atomicAdd(&a.pc->chunk_ctr[r & 1], 1u);
+            __syncthreads();
+            const u64 c = s_claim;
+            __syncthreads();
+            if (c >= nchunks) break;
+            updates += advance_chunk<ALGO, E>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
+                                              a.delta, count, W, 0, W, 0);
+        }
+        grid_sync(a.pc, gen);
+    }
+    u32 wsum = wave_reduce(updates);
+    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
+    if (blockIdx.x == 0 && tid == 0) {
+        st_relaxed(&a.sc->epoch, epoch0 + r + conv);
+        a.pc->rounds_done = r;
+        a.pc->converged = conv;
+        st_relaxed(&a.pc->tile_ctr[0], 0u);
+        st_relaxed(&a.pc->tile_ctr[1], 0u);
+        st_relaxed(&a.pc->chunk_ctr[0], 0u);
+        st_relaxed(&a.pc->chunk_ctr[1], 0u);
+    }
+}
+
+// ======================================================= batched rounds
+// Per-lane relaxation of one source's value (lane = source).
+template <int ALGO>
+struct BatchRelax;
+template <>
+struct BatchRelax<ALGO_SSSP> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du + w; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
+};
+template <>
+struct BatchRelax<ALGO_BFS> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32) { return du + 1; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
+};
+template <>
+struct BatchRelax<ALGO_SSWP> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du < w ? du : w; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c > old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMax(p, c); }
+};
+
+template <class E>
+__device__ __forceinline__ u32 edge_weight(const E &e)
+{
+    if constexpr (sizeof(E) == 8) return reinterpret_cast<const u32 *>(&e)[1];
+    else return 1u;
+}
+
+// One chunk of BATCH_CHUNK positions: each wave walks a contiguous quarter of
+// it, U positions at a time (U edge loads and U destination rows in flight);
+// lane s relaxes source s when bit s of the pushing vertex's mask is set.
+// dist is vertex-major, 64 u32 per vertex, so a wave's row access is one
+// coalesced 256-byte transaction.
+template <int ALGO, class E, int U>
+__device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, const u64 *entryMask,
+                                                   const u32 *chunkMap, const E *edges, u32 *dist, u64 *maskOut,
+                                                   u64 count, u64 W, u64 w0, u64 wend, u64 shift)
+{
+    __shared__ u32 b_pre[BATCH_CHUNK + 1];
+    __shared__ u32 b_loc[BATCH_CHUNK + 1];
+    __shared__ u32 b_id[BATCH_CHUNK + 1];
+    __shared__ u64 b_msk[BATCH_CHUNK + 1];
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
+    const u64 cfirst = j * BATCH_CHUNK;
+    const u64 cbeg = cfirst > w0 ? cfirst : w0;
+    const u64 cend = (cfirst + BATCH_CHUNK) < wend ? (cfirst + BATCH_CHUNK) : wend;
+    const u64 k0 = chunkMap[j];
+    const u64 k1 = (cfirst + BATCH_CHUNK < W) ? chunkMap[j + 1] : count - 1;
+    const int nk = (int)(k1 - k0 + 1);
+    for (int t = tid; t < nk; t += ADV_BLOCK) {
+        uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
+        b_loc[t] = e.x;
+        b_pre[t] = e.y;
+        b_id[t] = e.z;
+        b_msk[t] = entryMask[k0 + t];
+    }
+    __syncthreads();
+    constexpr int NW = ADV_BLOCK / WAVE;
+    const u64 span = cend > cbeg ? (cend - cbeg + NW - 1) / NW : 0;
+    const u64 q0 = cbeg + (u64)wid * span;
+    const u64 q1 = (q0 + span) < cend ? (q0 + span) : cend;
+    u32 updates = 0;
+    if (q0 < q1) {
+        // entry holding q0 (uniform across the wave)
+        int lo = 0, hi = nk - 1;
+        const u32 pq = (u32)q0;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (b_pre[mid] <= pq) lo = mid;
+            else hi = mid - 1;
+        }
+        int k = lo, kcur = -1;
+        u32 du = 0;
+        for (u64 p = q0; p < q1; p += U) {
+            u32 vv[U], ww[U], dd[U], dv[U];
+            u64 mm[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const u64 pp = p + u;
+                ok[u] = pp < q1;
+                mm[u] = 0;
+                if (ok[u]) {
+                    while (k + 1 < nk && b_pre[k + 1] <= (u32)pp) k++;
+                    if (k != kcur) {
+                        kcur = k;
+                        du = dist[(u64)b_id[k] * WAVE + lane];
+                    }
+                    const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
+                    const E e = edges[eidx];
+                    vv[u] = e.end;
+                    ww[u] = edge_weight(e);
+                    mm[u] = b_msk[k];
+                    dd[u] = du;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                dv[u] = 0;
+                if (ok[u] && ((mm[u] >> lane) & 1ull)) dv[u] = dist[(u64)vv[u] * WAVE + lane];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                bool imp = false;
+                if (ok[u] && ((mm[u] >> lane) & 1ull)) {
+                    const u32 c = BatchRelax<ALGO>::cand(dd[u], ww[u]);
+                    if (BatchRelax<ALGO>::better(c, dv[u])) {
+                        const u32 old = BatchRelax<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], c);
+                        imp = BatchRelax<ALGO>::better(c, old);
+                    }
+                }
+                const unsigned long long bal = __ballot(imp);
+                if (bal) {
+                    if (lane == 0) atomicOr((unsigned long long *)&maskOut[vv[u]], bal);
+                    updates += (lane == 0) ? (u32)__popcll(bal) : 0u;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return updates;
+}
+
+struct BatchArgs {
+    ull n;
+    const u32 *nodes;
+    const ull *ptr;
+    ull fromEdge;
+    u64 *m1, *m2;          // activeness masks
+    u32 *dist;             // [V][64]
+    Entry *entries;
+    u64 *entryMask;
+    u32 *chunkMap;
+    const void *edges;
+    ScanCtl *sc;
+    PersistCtl *pc;
+    TileDesc td;
+    u32 maxRounds;
+    u32 idBase;
+};
+
+// All async rounds of one resident partition for up to 64 sources at once,
+// in one cooperative launch (same phase / barrier structure as rounds_kernel).
+template <int ALGO, class E, int U>
+__global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
+{
+    __shared__ u32 s_claim, s_gen, s_epoch;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_epoch = ld_relaxed(&a.sc->epoch);
+    }
+    __syncthreads();
+    u32 gen = s_gen;
+    const u32 epoch0 = s_epoch;
+    const u32 numTiles = (u32)((a.n + FR_TILE - 1) / FR_TILE);
+    u32 r = 0, conv = 0, updates = 0;
+    for (; r < a.maxRounds; r++) {
+        const bool even = (r & 1u) == 0;
+        u64 *labIn = even ? a.m1 : a.m2;
+        u64 *labOut = even ? a.m2 : a.m1;
+        for (;;) {
+            if (tid == 0) s_claim = atomicAdd(&a.pc->tile_ctr[r & 1], 1u);
+            __syncthreads();
+            const u32 t = s_claim;
+            __syncthreads();
+            if (t >= numTiles) break;
+            FrontierMaskOp op{a.nodes, a.ptr, a.fromEdge, labIn, r == 0 ? a.m2 : nullptr, a.entries, a.entryMask,
+                              a.chunkMap, a.idBase};
+            scan_tile<FrontierMaskOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+        }
+        grid_sync(a.pc, gen);
+        const u64 count = ld_relaxed(&a.sc->count);
+        const u64 W = ld_relaxed(&a.sc->total);
+        if (count == 0) {
+            conv = 1;
+            break;
+        }
+        if (blockIdx.x == 0 && tid == 0) {
+            st_relaxed(&a.pc->tile_ctr[(r + 1) & 1], 0u);
+            st_relaxed(&a.pc->chunk_ctr[(r + 1) & 1], 0u);
+        }
+        const u64 nchunks = (W + BATCH_CHUNK - 1) / BATCH_CHUNK;
+        for (;;) {
+            if (tid == 0) s_claim = atomicAdd(&a.pc->chunk_ctr[r & 1], 1u);
+            __syncthreads();
+            const u64 c = s_claim;
+            __syncthreads();
+            if (c >= nchunks) break;
+            updates += advance_chunk_batch<ALGO, E, U>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
+                                                       a.dist, labOut, count, W, 0, W, 0);
+        }
+        grid_sync(a.pc, gen);
+    }
+    u32 wsum = wave_reduce(updates);
+    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
+    if (blockIdx.x == 0 && tid == 0) {
+        st_relaxed(&a.sc->epoch, epoch0 + r + conv);
+        a.pc->rounds_done = r;
+        a.pc->converged = conv;
+        st_relaxed(&a.pc->tile_ctr[0], 0u);
+        st_relaxed(&a.pc->tile_ctr[1], 0u);
+        st_relaxed(&a.pc->chunk_ctr[0], 0u);
+        st_relaxed(&a.pc->chunk_ctr[1], 0u);
+    }
+}
+
+// Standalone batched advance over the work window [w0, w1) (a vertex larger
+// than a buffer, streamed chunk by chunk).
+template <int ALGO, class E>
+__global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *entries, const u64 *entryMask,
+                                                                  const u32 *chunkMap, const E *edges, u32 *dist,
+                                                                  u64 *maskOut, u64 w0, u64 w1, u64 shift,
+                                                                  ScanCtl *ctl)
+{
+    const u64 count = ctl->count;
+    const u64 W = ctl->total;
+    const u64 wend = W < w1 ? W : w1;
+    if (count == 0 || wend <= w0) return;
+    const u64 j0 = w0 / BATCH_CHUNK;
+    const u64 j1 = (wend + BATCH_CHUNK - 1) / BATCH_CHUNK;
+    u32 updates = 0;
+    for (u64 j = j0 + blockIdx.x; j < j1; j += gridDim.x)
+        updates += advance_chunk_batch<ALGO, E, 4>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
+                                                   w0, wend, shift);
+    u32 wsum = wave_reduce(updates);
+    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
+}
+
+// Batched initial state: every distance INF (SSWP: 0), masks cleared
+// (mask1 all ones when `allActive`, for the first whole-graph generate).
+__global__ void init_batch_state_kernel(int algo, u64 n, u32 *dist, u64 *m1, u64 *m2, int allActive)
+{
+    const u32 init = (algo == ALGO_SSWP) ? 0u : 4294967294u;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n * WAVE; i += (u64)gridDim.x * blockDim.x)
+        dist[i] = init;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+        m1[i] = allActive ? ~0ull : 0ull;
+        m2[i] = 0ull;
+    }
+}
+
+// Source s_i gets value 0 (SSWP: INF) in lane i and bit i in mask1.
+__global__ void set_batch_sources_kernel(int algo, const u32 *sources, int k, u32 *dist, u64 *m1)
+{
+    const int i = threadIdx.x;
+    if (i < k) {
+        const u32 s = sources[i];
+        dist[(u64)s * WAVE + i] = (algo == ALGO_SSWP) ? 4294967294u : 0u;
+        atomicOr((unsigned long long *)&m1[s], 1ull << i);
     }
 }
 
@@ -562,6 +1073,34 @@ __global__ __launch_bounds__(64) void partition_kernel(const u64 *ptr, const u32
         out[2] = A;
         out[3] = ptr[A];
     }
+}
+
+// ========================================================= range stats
+// For each static vertex-id range [bounds[i], bounds[i+1]): the slice of
+// the active list inside it (lo, hi) and its active edges.
+__device__ __forceinline__ u64 lower_bound_u32(const u32 *a, u64 n, u32 key)
+{
+    u64 lo = 0, hi = n;
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void range_stats_kernel(const u32 *activeNodes, const u64 *anp, const u64 *dA, const u32 *bounds,
+                                   int P, u64 *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const u64 A = *dA;
+    const u64 lo = lower_bound_u32(activeNodes, A, bounds[i]);
+    const u64 hi = lower_bound_u32(activeNodes, A, bounds[i + 1]);
+    out[4 * i + 0] = lo;
+    out[4 * i + 1] = hi;
+    out[4 * i + 2] = anp[hi] - anp[lo];
+    out[4 * i + 3] = anp[lo];
 }
 
 // ============================================================== gather
@@ -641,6 +1180,43 @@ __global__ void fill_u32_kernel(u32 *p, u32 v, u64 n)
 {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// Initial vertex state of the label algorithms, written on the device (the
+// reference fills host arrays and copies them, e.g. subway/bfs-sync.cu:28-42).
+__global__ void init_label_state_kernel(int algo, int async, u64 n, u32 *value, uint8_t *l1, uint8_t *l2,
+                                        u32 source)
+{
+    const u32 INF = 4294967294u;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+        u32 v = (algo == ALGO_CC) ? (u32)i : (algo == ALGO_SSWP ? 0u : INF);
+        uint8_t a = async ? 1 : 0;
+        uint8_t b = (!async && algo == ALGO_CC) ? 1 : 0;
+        if (algo != ALGO_CC && i == source) {
+            v = (algo == ALGO_SSWP) ? INF : 0u;
+            if (!async) b = 1;
+        }
+        value[i] = v;
+        l1[i] = a;
+        l2[i] = b;
+    }
+}
+
+// PageRank: value 0, delta = initPR (subway/pr-sync.cu:33-40)
+__global__ void init_pr_state_kernel(u64 n, float *value, float *delta, float init)
+{
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+        value[i] = 0.0f;
+        delta[i] = init;
+    }
+}
+
+// Plain streaming copy (16 B per lane and iteration); used to calibrate
+// hardware byte counters against a kernel of known traffic.
+__global__ void copy_probe_kernel(const uint4 *src, uint4 *dst, u64 n16)
+{
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (u64)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // ============================================================ launchers
@@ -748,11 +1324,12 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
     u64 tiles = (a.n + FR_TILE - 1) / FR_TILE;
     check_tiles(s, tiles);
     if (a.algo == ALGO_PR) {
-        FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap};
+        FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
+                        a.idBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                            dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first);
     } else {
-        FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap};
+        FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap, a.idBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp, SCAN_BLOCK, FR_IPT>),
                            dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td,
                            a.first);
@@ -794,6 +1371,243 @@ void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
             break;
         default: throw Error(ERR_ARG, "unknown algorithm");
     }
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
+                             uint32_t source, hipStream_t st)
+{
+    hipLaunchKernelGGL(init_label_state_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, algo, async,
+                       (u64)n, value, l1, l2, source);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_init_pr_state(ull n, float *value, float *delta, float init, hipStream_t st)
+{
+    hipLaunchKernelGGL(init_pr_state_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value,
+                       delta, init);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st)
+{
+    if (bytes % 16) throw Error(ERR_ARG, "probe copy size must be a multiple of 16");
+    ull n16 = bytes / 16;
+    if (!n16) return;
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, st, (const uint4 *)src,
+                       (uint4 *)dst, n16);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+template <int ALGO, class E>
+static const void *rounds_fn()
+{
+    return reinterpret_cast<const void *>(&rounds_kernel<ALGO, E>);
+}
+
+static const void *rounds_fn_for(int algo, int edgeBytes)
+{
+    switch (algo) {
+        case ALGO_BFS: return edgeBytes == 4 ? rounds_fn<ALGO_BFS, OutEdge>() : rounds_fn<ALGO_BFS, OutEdgeWeighted>();
+        case ALGO_CC: return edgeBytes == 4 ? rounds_fn<ALGO_CC, OutEdge>() : rounds_fn<ALGO_CC, OutEdgeWeighted>();
+        case ALGO_SSSP:
+            if (edgeBytes != 8) throw Error(ERR_ARG, "SSSP needs weighted edges");
+            return rounds_fn<ALGO_SSSP, OutEdgeWeighted>();
+        case ALGO_SSWP:
+            if (edgeBytes != 8) throw Error(ERR_ARG, "SSWP needs weighted edges");
+            return rounds_fn<ALGO_SSWP, OutEdgeWeighted>();
+        case ALGO_PR: return edgeBytes == 4 ? rounds_fn<ALGO_PR, OutEdge>() : rounds_fn<ALGO_PR, OutEdgeWeighted>();
+        default: throw Error(ERR_ARG, "unknown algorithm");
+    }
+}
+
+int rounds_max_grid(int algo, int edgeBytes)
+{
+    const void *f = rounds_fn_for(algo, edgeBytes);
+    int perCU = 0;
+    SUBWAY_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, f, ADV_BLOCK, 0));
+    int dev = 0;
+    SUBWAY_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
+    if (!prop.cooperativeLaunch) return 0;
+    return perCU * prop.multiProcessorCount;
+}
+
+void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st)
+{
+    if (l.n == 0) throw Error(ERR_ARG, "rounds over an empty partition");
+    if (grid <= 0) throw Error(ERR_ARG, "rounds grid must be positive");
+    RoundsArgs a;
+    a.n = l.n;
+    a.nodes = l.nodes;
+    a.ptr = l.ptr;
+    a.fromEdge = l.fromEdge;
+    a.l1 = l.l1;
+    a.l2 = l.l2;
+    a.value = l.value;
+    a.delta = l.delta;
+    a.acc = l.acc;
+    a.entries = l.entries;
+    a.chunkMap = l.chunkMap;
+    a.edges = l.edges;
+    a.sc = l.sc;
+    a.pc = l.pc;
+    a.td = l.td;
+    a.maxRounds = l.maxRounds;
+    a.roundBase = l.roundBase;
+    a.idBase = l.idBase;
+    void *params[] = {&a};
+    SUBWAY_CHECK(hipLaunchCooperativeKernel(rounds_fn_for(algo, edgeBytes), dim3((unsigned)grid), dim3(ADV_BLOCK),
+                                            params, 0, st));
+}
+
+void launch_generate_masks(ull n, const ull *m1, const ull *m2, const uint32_t *outDegree, uint32_t *activeNodes,
+                           ull *activeNodesPointer, ScanScratch &s, hipStream_t st)
+{
+    if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
+    u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
+    check_tiles(s, tiles);
+    GenMaskOp op{m1, m2, outDegree, activeNodes, activeNodesPointer};
+    hipLaunchKernelGGL((select_scan_kernel<GenMaskOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_init_batch_state(int algo, ull n, uint32_t *dist, ull *m1, ull *m2, int allActive, hipStream_t st)
+{
+    hipLaunchKernelGGL(init_batch_state_kernel, dim3(grid_for(n * WAVE, 256, 16384)), dim3(256), 0, st, algo,
+                       (u64)n, dist, m1, m2, allActive);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_set_batch_sources(int algo, const uint32_t *sources, int k, uint32_t *dist, ull *m1, hipStream_t st)
+{
+    if (k < 1 || k > 64) throw Error(ERR_ARG, "a batch holds 1..64 sources");
+    hipLaunchKernelGGL(set_batch_sources_kernel, dim3(1), dim3(64), 0, st, algo, sources, k, dist, m1);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
+                           Entry *entries, ull *entryMask, uint32_t *chunkMap, ScanScratch &s, hipStream_t st,
+                           uint32_t idBase)
+{
+    if (n == 0) throw Error(ERR_ARG, "frontier over an empty partition");
+    u64 tiles = (n + FR_TILE - 1) / FR_TILE;
+    check_tiles(s, tiles);
+    FrontierMaskOp op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase};
+    hipLaunchKernelGGL((select_scan_kernel<FrontierMaskOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+template <int ALGO, class E>
+static void adv_batch(const Entry *entries, const ull *entryMask, const uint32_t *chunkMap, const void *edges,
+                      uint32_t *dist, ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)grid), dim3(ADV_BLOCK), 0, st, entries,
+                       entryMask, chunkMap, (const E *)edges, dist, maskOut, (u64)w0, (u64)w1, (u64)shift, ctl);
+}
+
+void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
+                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
+                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+{
+    if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
+    if (algo == ALGO_SSSP && edgeBytes == 8)
+        adv_batch<ALGO_SSSP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                              grid, st);
+    else if (algo == ALGO_SSWP && edgeBytes == 8)
+        adv_batch<ALGO_SSWP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                              grid, st);
+    else if (algo == ALGO_BFS && edgeBytes == 8)
+        adv_batch<ALGO_BFS, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                             grid, st);
+    else if (algo == ALGO_BFS && edgeBytes == 4)
+        adv_batch<ALGO_BFS, OutEdge>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl, grid, st);
+    else
+        throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP (SSSP/SSWP on weighted edges)");
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+template <int ALGO, class E, int U>
+static const void *batch_fn()
+{
+    return reinterpret_cast<const void *>(&rounds_batch_kernel<ALGO, E, U>);
+}
+
+template <int ALGO, class E>
+static const void *batch_fn_u(int unroll)
+{
+    switch (unroll) {
+        case 1: return batch_fn<ALGO, E, 1>();
+        case 2: return batch_fn<ALGO, E, 2>();
+        case 4: return batch_fn<ALGO, E, 4>();
+        case 8: return batch_fn<ALGO, E, 8>();
+        default: throw Error(ERR_ARG, "batched unroll must be 1, 2, 4 or 8");
+    }
+}
+
+static const void *batch_fn_for(int algo, int edgeBytes, int unroll)
+{
+    switch (algo) {
+        case ALGO_BFS: return edgeBytes == 4 ? batch_fn_u<ALGO_BFS, OutEdge>(unroll)
+                                             : batch_fn_u<ALGO_BFS, OutEdgeWeighted>(unroll);
+        case ALGO_SSSP:
+            if (edgeBytes != 8) throw Error(ERR_ARG, "SSSP needs weighted edges");
+            return batch_fn_u<ALGO_SSSP, OutEdgeWeighted>(unroll);
+        case ALGO_SSWP:
+            if (edgeBytes != 8) throw Error(ERR_ARG, "SSWP needs weighted edges");
+            return batch_fn_u<ALGO_SSWP, OutEdgeWeighted>(unroll);
+        default: throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
+    }
+}
+
+int batch_max_grid(int algo, int edgeBytes, int unroll)
+{
+    const void *f = batch_fn_for(algo, edgeBytes, unroll);
+    int perCU = 0;
+    SUBWAY_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, f, ADV_BLOCK, 0));
+    int dev = 0;
+    SUBWAY_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
+    if (!prop.cooperativeLaunch) return 0;
+    return perCU * prop.multiProcessorCount;
+}
+
+void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st)
+{
+    if (l.n == 0) throw Error(ERR_ARG, "rounds over an empty partition");
+    if (grid <= 0) throw Error(ERR_ARG, "rounds grid must be positive");
+    BatchArgs a;
+    a.n = l.n;
+    a.nodes = l.nodes;
+    a.ptr = l.ptr;
+    a.fromEdge = l.fromEdge;
+    a.m1 = l.m1;
+    a.m2 = l.m2;
+    a.dist = l.dist;
+    a.entries = l.entries;
+    a.entryMask = l.entryMask;
+    a.chunkMap = l.chunkMap;
+    a.edges = l.edges;
+    a.sc = l.sc;
+    a.pc = l.pc;
+    a.td = l.td;
+    a.maxRounds = l.maxRounds;
+    a.idBase = l.idBase;
+    void *params[] = {&a};
+    SUBWAY_CHECK(hipLaunchCooperativeKernel(batch_fn_for(algo, edgeBytes, unroll), dim3((unsigned)grid),
+                                            dim3(ADV_BLOCK), params, 0, st));
+}
+
+void launch_range_stats(const uint32_t *activeNodes, const ull *anp, const ull *dA, const uint32_t *bounds, int P,
+                        ull *out, hipStream_t st)
+{
+    if (P <= 0) return;
+    hipLaunchKernelGGL(range_stats_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, activeNodes,
+                       (const u64 *)anp, (const u64 *)dA, bounds, P, (u64 *)out);
     SUBWAY_CHECK(hipGetLastError());
 }
 

@@ -80,6 +80,11 @@ void launch_partition(const ull *activeNodesPointer, const uint32_t *activeNodes
                       ull start, uint32_t maxRows, PartRow *rows, ull *out,
                       hipStream_t st);
 
+// ---- per static id range: active-list slice [lo, hi), its active edges and
+// the active-edge offset at lo (out: 4 x u64 per range)
+void launch_range_stats(const uint32_t *activeNodes, const ull *anp, const ull *dA, const uint32_t *bounds, int P,
+                        ull *out, hipStream_t st);
+
 // ---- zero-copy gather of a partition's adjacency from mapped host memory
 void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *activeNodesPointer,
                    ull kbeg, ull kend, const ull *nodePointer,
@@ -103,6 +108,7 @@ struct FrontierArgs {
     Entry *entries;
     uint32_t *chunkMap;
     int first;                       // 1: ignore the stop flag of the previous launch
+    uint32_t idBase;                 // nodes == null: the items are ids idBase .. idBase + n - 1
 };
 void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
 
@@ -121,6 +127,71 @@ struct AdvanceArgs {
     int grid;                        // persistent workgroups
 };
 void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st);
+
+// ---- persistent async rounds (one cooperative launch per partition)
+struct RoundsLaunch {
+    ull n;
+    const uint32_t *nodes;
+    const ull *ptr;
+    ull fromEdge;
+    uint8_t *l1, *l2;
+    uint32_t *value;
+    float *delta;
+    float acc;
+    Entry *entries;
+    uint32_t *chunkMap;
+    const void *edges;
+    ScanCtl *sc;
+    dev::PersistCtl *pc;
+    TileDesc td;
+    uint32_t maxRounds;
+    uint32_t roundBase;
+    uint32_t idBase;       // nodes == null: a whole id range starting here
+};
+// Largest co-resident grid for the rounds kernel (0: no cooperative launch).
+int rounds_max_grid(int algo, int edgeBytes);
+void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st);
+
+// ---- batched k-source runs (k <= 64): dist[V][64], 64-bit masks
+constexpr int BATCH_LANES = 64;
+constexpr int BATCH_CHUNK_EDGES = 256;   // work-space chunk of the batched advance
+void launch_generate_masks(ull n, const ull *m1, const ull *m2, const uint32_t *outDegree, uint32_t *activeNodes,
+                           ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
+void launch_init_batch_state(int algo, ull n, uint32_t *dist, ull *m1, ull *m2, int allActive, hipStream_t st);
+void launch_set_batch_sources(int algo, const uint32_t *sources, int k, uint32_t *dist, ull *m1, hipStream_t st);
+struct BatchLaunch {
+    ull n;
+    const uint32_t *nodes;
+    const ull *ptr;
+    ull fromEdge;
+    ull *m1, *m2;
+    uint32_t *dist;
+    Entry *entries;
+    ull *entryMask;
+    uint32_t *chunkMap;
+    const void *edges;
+    ScanCtl *sc;
+    dev::PersistCtl *pc;
+    TileDesc td;
+    uint32_t maxRounds;
+    uint32_t idBase;
+};
+int batch_max_grid(int algo, int edgeBytes, int unroll);
+void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
+                           Entry *entries, ull *entryMask, uint32_t *chunkMap, ScanScratch &s, hipStream_t st,
+                           uint32_t idBase = 0);
+void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
+                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
+                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
+void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
+
+// Initial vertex state (values and both label arrays / PR value and delta).
+void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
+                             uint32_t source, hipStream_t st);
+void launch_init_pr_state(ull n, float *value, float *delta, float init, hipStream_t st);
+
+// Streaming copy of `bytes` (multiple of 16) for counter calibration.
+void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st);
 
 // Fill helpers (kernels rather than memsets so they can sit in graphs).
 void launch_fill_u8(uint8_t *p, uint8_t v, ull n, hipStream_t st);

@@ -29,6 +29,20 @@ struct ScanCtl {
     u64 work_sum;   // selected weight over all launches (statistics)
 };
 
+// Control block of the persistent rounds kernel: a two-level grid barrier
+// (8 group counters on separate cache lines, then one top counter), and the
+// tile / chunk claim counters of the two phases, double-buffered by round
+// parity so they can be reset one round ahead.
+struct PersistCtl {
+    u32 bar_sub[8 * 16];
+    u32 bar_top[16];
+    u32 bar_gen[16];
+    u32 tile_ctr[2];
+    u32 chunk_ctr[2];
+    u32 rounds_done;   // rounds that selected work in the last launch
+    u32 converged;     // 1: the last launch ended on a round that selected nothing
+};
+
 struct TileDesc {
     u64 *flag;     // packed descriptors
     u64 *agg;      // weight aggregate of the tile

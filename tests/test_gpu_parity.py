@@ -158,7 +158,7 @@ def test_pr_golden_readme(sw, gpu, mode):
 
 @pytest.mark.parametrize("algo,mode", APPS + [("pr", "sync"), ("pr", "async")])
 @pytest.mark.parametrize("budget", [0, 64 << 10])
-@pytest.mark.parametrize("transfer", ["auto", "zerocopy"])
+@pytest.mark.parametrize("transfer", ["auto", "zerocopy", "host"])
 def test_apps_random(sw, gpu, algo, mode, budget, transfer):
     rng = np.random.default_rng(zlib.crc32(f"{algo}-{mode}-{budget}-{transfer}".encode()))
     n = 20_000
@@ -170,6 +170,48 @@ def test_apps_random(sw, gpu, algo, mode, budget, transfer):
     _, st = check_app(sw, g, algo, mode, src=17 if algo != "cc" else 0, **kw)
     if budget:
         assert st.partitions > st.global_iterations  # the out-of-memory path ran
+
+
+ASYNC = [(a, "async") for a in ("bfs", "cc", "sssp", "sswp", "pr")]
+
+
+@pytest.mark.parametrize("algo,mode", ASYNC)
+@pytest.mark.parametrize("rounds", ["persistent", "per-round"])
+@pytest.mark.parametrize("budget", [0, 48 << 10])
+def test_async_rounds_modes(sw, gpu, algo, mode, rounds, budget):
+    """The cooperative persistent rounds kernel and the launch-per-round
+    path reach the same fixpoint as the oracle."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(13, 16, seed=21, weighted=weighted, symmetrize=(algo == "cc"))
+    kw = dict(budget=budget, rounds=rounds)
+    if algo == "pr":
+        kw["threshold"] = 1e-6
+    check_app(sw, g, algo, mode, src=g.max_degree_vertex, **kw)
+
+
+@pytest.mark.parametrize("max_rounds", [1, 2, 3])
+def test_async_round_cap_still_exact(sw, gpu, max_rounds):
+    """Capping the inner rounds per partition only changes the schedule:
+    labels left set are picked up by the next generate."""
+    g = sw.Graph.rmat(13, 16, seed=4, weighted=True)
+    _, st = check_app(sw, g, "sssp", "async", src=0, budget=64 << 10, max_rounds=max_rounds)
+    _, full = sw.run(g, "sssp", mode="async", source=0, budget=64 << 10)
+    assert st.global_iterations >= full.global_iterations
+
+
+@pytest.mark.parametrize("algo", ["bfs", "sssp", "sswp"])
+@pytest.mark.parametrize("mode", ["sync", "async"])
+def test_session_many_sources(sw, gpu, algo, mode):
+    """One session (device state built once) answers queries from several
+    sources, each bit-exact."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(13, 16, seed=33, weighted=weighted)
+    sess = sw.Session(g, algo, mode=mode, budget=64 << 10)
+    ptr, adj, w = g.csr()
+    rng = np.random.default_rng(1)
+    for src in [g.max_degree_vertex] + rng.integers(0, g.n, 4).tolist():
+        vals, _ = sess.run(int(src))
+        assert np.array_equal(vals, O.expected(algo, ptr, adj, w, int(src)))
 
 
 @pytest.mark.parametrize("algo,mode", APPS)
@@ -216,3 +258,64 @@ def test_errors_are_reported(sw, gpu):
         sw.run(g, "bfs", source=g.n)  # source out of range
     with pytest.raises(sw.SubwayError):
         sw.run(g, "bfs", budget=100)  # below one advance chunk per buffer
+
+
+# ------------------------------------------------------ batched sources
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "sswp"])
+@pytest.mark.parametrize("budget", [0, 64 << 10, 32 << 10])
+@pytest.mark.parametrize("k", [1, 5, 64])
+def test_batch_matches_single_source(sw, gpu, algo, budget, k):
+    """k sources advanced together equal k single-source oracle runs; the
+    32 KiB budget makes the hub vertex stream in chunks."""
+    g = sw.Graph.rmat(13, 16, seed=17, weighted=True)
+    ptr, adj, w = g.csr()
+    deg = np.diff(ptr)
+    rng = np.random.default_rng(k + budget)
+    cand = np.nonzero(deg > 0)[0]
+    sources = rng.choice(cand, k, replace=True).astype(np.uint32)
+    sources[0] = g.max_degree_vertex
+    bs = sw.BatchSession(g, algo, budget=budget)
+    vals, st = bs.run(sources)
+    for i, s in enumerate(sources):
+        exp = O.expected(algo, ptr, adj, w, int(s))
+        assert np.array_equal(vals[i], exp), (algo, budget, k, i, int(np.sum(vals[i] != exp)))
+
+
+def test_batch_tuner_and_repeats(sw, gpu):
+    g = sw.Graph.rmat(12, 16, seed=2, weighted=True)
+    ptr, adj, w = g.csr()
+    sources = np.arange(0, 64 * 13, 13, dtype=np.uint32) % g.n
+    bs = sw.BatchSession(g, "sssp", budget=128 << 10)
+    best, times = bs.tune(sources)
+    assert best in (1, 2, 4, 8) and all(t > 0 for t in times)
+    for _ in range(2):
+        vals, _ = bs.run(sources)
+        for i in (0, 17, 63):
+            assert np.array_equal(vals[i], O.sssp(ptr, adj, w, int(sources[i])))
+
+
+# ------------------------------------------------- learned-policy schedule
+@pytest.mark.parametrize("algo,mode", APPS + [("pr", "sync"), ("pr", "async")])
+@pytest.mark.parametrize("theta", [0.0, 0.3, 1.5])
+def test_range_schedule_exact(sw, gpu, algo, mode, theta):
+    """The learned policy's schedule (static id ranges; dense ranges move
+    whole and their rounds cover every vertex; the rest is compacted) only
+    changes the schedule: results equal the oracle for any threshold."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(13, 16, seed=29, weighted=weighted, symmetrize=(algo == "cc"))
+    kw = dict(budget=48 << 10, policy="learned", theta=theta)
+    if algo == "pr":
+        kw["threshold"] = 1e-6
+    _, st = check_app(sw, g, algo, mode, src=g.max_degree_vertex, **kw)
+    assert st.partitions >= st.global_iterations
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.4])
+def test_range_schedule_batched(sw, gpu, theta):
+    g = sw.Graph.rmat(12, 16, seed=31, weighted=True)
+    ptr, adj, w = g.csr()
+    sources = np.arange(0, 640, 10, dtype=np.uint32)
+    bs = sw.BatchSession(g, "sssp", budget=48 << 10, policy="learned", theta=theta)
+    vals, _ = bs.run(sources)
+    for i in (0, 9, 63):
+        assert np.array_equal(vals[i], O.sssp(ptr, adj, w, int(sources[i])))
