@@ -46,10 +46,11 @@ struct RunOptions {
     int profile = 0;        // 1: time every kernel launch and transfer with HIP events
     int persistent = 1;     // async: all rounds of a partition in one cooperative launch
     int roundsGrid = 0;     // workgroups of the persistent kernel; 0 = 4 per CU
-    int maxRounds = 1 << 16;  // async rounds per partition visit (labels left set resume next iteration)
+    int maxRounds = 4096;     // async rounds per partition visit (labels left set resume next iteration)
     int batch = 0;          // 1: batched k-source queries (runBatch), BFS/SSSP/SSWP async
     int batchUnroll = 0;    // batched advance: edges in flight per wave (1, 2, 4, 8); 0 = 4
     float theta = 0.5f;     // learned policy without a model: whole-range density threshold
+    uint32_t spinLimit = 0; // polls before a device wait gives up (the run then fails); 0 = default
 };
 
 struct IterRecord {
@@ -76,6 +77,7 @@ struct RunStats {
     ull transferOps = 0;
     ull workEdges = 0;         // edges relaxed: sum of frontier work over all rounds
     ull workEntries = 0;       // vertices that pushed: sum over all rounds
+    ull workItems = 0;         // partition items scanned: sum over all rounds
     ull h2dBytes = 0;          // edge bytes moved host -> device
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads

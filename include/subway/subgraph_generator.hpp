@@ -34,10 +34,12 @@ public:
     void generate(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc);
 
     // Enqueue only (no host synchronisation): totals stay on the device in
-    // the control block returned by deviceTotals() (count, edge total).
+    // the control block returned by deviceTotals(): u64 words [count, edge
+    // total, error]; error (low 32 bits) is nonzero when the kernel's bounded
+    // lookback wait gave up, and then count must be read as zero.
     void enqueue(Graph<E> &graph, Subgraph<E> &subgraph);
     void enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc);
-    const unsigned long long *deviceTotals() const;   // [count, total]
+    const unsigned long long *deviceTotals() const;   // [count, total, error]
     // Scan scratch of the generator kernel (other selection rules, e.g. the
     // batched per-source masks, run through it so deviceTotals() holds them).
     subway::ScanScratch &scanScratch() { return *scratch; }

@@ -60,12 +60,14 @@ typedef struct subway_options {
     int32_t rounds_mode;    /* async: 0 = all rounds of a partition in one cooperative launch,
                                1 = one frontier + advance launch pair per round */
     int32_t rounds_grid;    /* workgroups of the cooperative rounds kernel; 0 = 4 per CU */
-    int32_t max_rounds;     /* async rounds per partition visit; 0 = 65536 */
+    int32_t max_rounds;     /* async rounds per partition visit; 0 = 4096 */
     int32_t batch;          /* 1: batched session (BFS/SSSP/SSWP async, <= 64 sources per run) */
     int32_t batch_unroll;   /* batched advance: edges in flight per wave, 1/2/4/8; 0 = 4 */
     float theta;            /* learned policy without a model: id ranges whose active-edge density is at
                                least theta move whole (default 0.5; > 1 never) */
-    int32_t reserved[7];
+    uint32_t spin_limit;    /* polls before a device-side wait (scan lookback, grid barrier) gives up and
+                               the run fails with SUBWAY_ERR_INTERNAL; 0 = 4194304 */
+    int32_t reserved[6];
 } subway_options;
 
 typedef struct subway_stats {
@@ -89,6 +91,7 @@ typedef struct subway_stats {
     uint64_t work_entries;  /* vertices that pushed over all rounds */
     uint32_t global_iterations;
     uint32_t trace_len;     /* records written to the trace array */
+    uint64_t work_items;    /* partition items scanned by frontier passes over all rounds */
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -169,6 +172,24 @@ int32_t subway_partition(uint64_t num_active, const uint64_t *active_nodes_point
  * out receives sum(degree) edges of sizeof(edge) bytes. */
 int32_t subway_gather(const subway_graph *g, uint64_t num_active, const uint32_t *active_nodes, void *out,
                       uint64_t *num_edges);
+
+/* -------- the learned scheduler's network (12 -> 64 -> 64 -> 3, FP32)
+ * params: SUBWAY_MLP_PARAMS floats = feature mean[12], feature scale[12],
+ * W1[12x64], b1[64], W2[64x64], b2[64], W3[64x3], b3[3] (row-major in x out).
+ * No reference counterpart. */
+#define SUBWAY_MLP_PARAMS 5211
+#define SUBWAY_MLP_FEATURES 12
+#define SUBWAY_MLP_HEADS 3
+/* out[n x 3] = network(x[n x 12]) on the GPU */
+int32_t subway_mlp_forward(const float *params, const float *x, uint64_t n, float *out);
+/* loss = sum(mask * (out - y)^2) / n and its gradient (grad[SUBWAY_MLP_PARAMS],
+ * zero on the normalisation entries), computed on the GPU */
+int32_t subway_mlp_grad(const float *params, const float *x, const float *y, const float *mask, uint64_t n,
+                        float *grad, float *loss);
+/* full-batch Adam for `steps` steps on the GPU; params updated in place;
+ * loss_per_step (may be NULL) receives the loss before each step */
+int32_t subway_mlp_train(float *params, const float *x, const float *y, const float *mask, uint64_t n,
+                         int32_t steps, float lr, float *loss_per_step);
 
 /* -------- measurement */
 /* Pinned-copy peak: hipMemcpyAsync of `bytes` from hipHostMalloc memory to

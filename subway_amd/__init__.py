@@ -48,7 +48,7 @@ class _Options(ctypes.Structure):
         ("verbose", ctypes.c_int32), ("advance_grid", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 7),
+        ("spin_limit", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -63,6 +63,7 @@ class _Stats(ctypes.Structure):
         ("transfer_ops", ctypes.c_uint64), ("rounds_launches", ctypes.c_uint64), ("work_edges", ctypes.c_uint64),
         ("work_entries", ctypes.c_uint64),
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
+        ("work_items", ctypes.c_uint64),
     ]
 
 
@@ -186,6 +187,7 @@ class RunStats:
     work_edges: int
     work_entries: int
     global_iterations: int
+    work_items: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -193,7 +195,7 @@ ROUNDS = {"persistent": 0, "per-round": 1}
 
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5):
+             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -212,6 +214,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.rounds_grid = rounds_grid
     o.max_rounds = max_rounds
     o.theta = theta
+    o.spin_limit = spin_limit
     return o
 
 
@@ -233,10 +236,10 @@ class Session:
     def __init__(self, graph: Graph, algo: str, mode: str = "sync", budget: int = 0, buffers: int = 2,
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
-                 rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5):
+                 rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0):
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                     profile, rounds, rounds_grid, max_rounds, theta)
+                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -305,7 +308,7 @@ class BatchSession:
 def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
-        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5):
+        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
@@ -313,7 +316,7 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     device memory, the reference's sizing).
     """
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                 profile, rounds, rounds_grid, max_rounds, theta)
+                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

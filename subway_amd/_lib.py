@@ -63,6 +63,9 @@ _SIGS = {
     "subway_gather": (_i32, [_vp, _u64, _u32p, _vp, _u64p]),
     "subway_h2d_peak": (_i32, [_u64, _i32, ctypes.POINTER(ctypes.c_double)]),
     "subway_probe_copy": (_i32, [_u64, ctypes.POINTER(ctypes.c_double)]),
+    "subway_mlp_forward": (_i32, [_f32p, _f32p, _u64, _f32p]),
+    "subway_mlp_grad": (_i32, [_f32p, _f32p, _f32p, _f32p, _u64, _f32p, _f32p]),
+    "subway_mlp_train": (_i32, [_f32p, _f32p, _f32p, _f32p, _u64, _i32, ctypes.c_float, _f32p]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

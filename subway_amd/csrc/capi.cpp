@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "subway/engine.hpp"
 #include "subway/gpu_error_check.hpp"
 #include "subway/graph.hpp"
@@ -222,6 +223,7 @@ static RunOptions to_options(const subway_options *o)
     r.theta = o->theta;
     r.roundsGrid = o->rounds_grid;
     if (o->max_rounds > 0) r.maxRounds = o->max_rounds;
+    r.spinLimit = o->spin_limit;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -271,6 +273,7 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->work_entries = s.workEntries;
     stats->global_iterations = s.globalIterations;
     stats->trace_len = k;
+    stats->work_items = s.workItems;
 }
 
 extern "C" {
@@ -357,6 +360,9 @@ int32_t subway_generate(uint64_t n, const uint8_t *label1, const uint8_t *label2
         launch_generate_labels(n, l1.p, l2.p, deg.p, an.p, ap.p, s, nullptr);
         ScanCtl ctl;
         SUBWAY_CHECK(hipMemcpy(&ctl, s.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        if (ctl.error)
+            throw Error(ERR_INTERNAL, std::string("generator: device wait gave up (") + wait_error_name(ctl.error) +
+                                          ")");
         SUBWAY_CHECK(hipMemcpy(active_nodes, an.p, ctl.count * 4, hipMemcpyDeviceToHost));
         SUBWAY_CHECK(hipMemcpy(active_nodes_pointer, ap.p, (ctl.count + 1) * 8, hipMemcpyDeviceToHost));
         if (num_active) *num_active = ctl.count;
@@ -380,6 +386,9 @@ int32_t subway_generate_pr(uint64_t n, const float *delta, float acc, const uint
         launch_generate_pr(n, dl.p, acc, deg.p, an.p, ap.p, s, nullptr);
         ScanCtl ctl;
         SUBWAY_CHECK(hipMemcpy(&ctl, s.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        if (ctl.error)
+            throw Error(ERR_INTERNAL, std::string("generator: device wait gave up (") + wait_error_name(ctl.error) +
+                                          ")");
         SUBWAY_CHECK(hipMemcpy(active_nodes, an.p, ctl.count * 4, hipMemcpyDeviceToHost));
         SUBWAY_CHECK(hipMemcpy(active_nodes_pointer, ap.p, (ctl.count + 1) * 8, hipMemcpyDeviceToHost));
         if (num_active) *num_active = ctl.count;
@@ -519,6 +528,64 @@ int32_t subway_probe_copy(uint64_t bytes, double *ms)
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (ms) *ms = t;
+    });
+}
+
+int32_t subway_mlp_forward(const float *params, const float *x, uint64_t n, float *out)
+{
+    return guard([&] {
+        if (!params || (n && (!x || !out))) throw Error(ERR_ARG, "null argument");
+        if (!n) return;
+        DevBuf<float> p(MLP_PARAMS), dx(n * MLP_IN), dout(n * MLP_OUT);
+        SUBWAY_CHECK(hipMemcpy(p.p, params, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(dx.p, x, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+        launch_mlp_forward(p.p, dx.p, n, dout.p, nullptr);
+        SUBWAY_CHECK(hipMemcpy(out, dout.p, n * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int32_t subway_mlp_grad(const float *params, const float *x, const float *y, const float *mask, uint64_t n,
+                        float *grad, float *loss)
+{
+    return guard([&] {
+        if (!params || !grad || !loss || (n && (!x || !y || !mask))) throw Error(ERR_ARG, "null argument");
+        DevBuf<float> p(MLP_PARAMS), dx(n * MLP_IN + 1), dy(n * MLP_OUT + 1), dm(n * MLP_OUT + 1);
+        DevBuf<float> dg(MLP_PARAMS), dl(1);
+        SUBWAY_CHECK(hipMemcpy(p.p, params, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+        if (n) {
+            SUBWAY_CHECK(hipMemcpy(dx.p, x, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+            SUBWAY_CHECK(hipMemcpy(dy.p, y, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+            SUBWAY_CHECK(hipMemcpy(dm.p, mask, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+        }
+        launch_mlp_grad(p.p, dx.p, dy.p, dm.p, n, dg.p, dl.p, nullptr);
+        SUBWAY_CHECK(hipMemcpy(grad, dg.p, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+        SUBWAY_CHECK(hipMemcpy(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int32_t subway_mlp_train(float *params, const float *x, const float *y, const float *mask, uint64_t n,
+                         int32_t steps, float lr, float *loss_per_step)
+{
+    return guard([&] {
+        if (!params || !n || !x || !y || !mask || steps < 0) throw Error(ERR_ARG, "bad argument");
+        DevBuf<float> p(MLP_PARAMS), dx(n * MLP_IN), dy(n * MLP_OUT), dm(n * MLP_OUT);
+        DevBuf<float> dg(MLP_PARAMS), dl(1), m1(MLP_PARAMS), m2(MLP_PARAMS);
+        SUBWAY_CHECK(hipMemcpy(p.p, params, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(dx.p, x, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(dy.p, y, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemcpy(dm.p, mask, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+        SUBWAY_CHECK(hipMemset(m1.p, 0, MLP_PARAMS * sizeof(float)));
+        SUBWAY_CHECK(hipMemset(m2.p, 0, MLP_PARAMS * sizeof(float)));
+        std::vector<float> losses(steps > 0 ? steps : 1);
+        DevBuf<float> dls(steps > 0 ? steps : 1);
+        for (int t = 1; t <= steps; t++) {
+            launch_mlp_grad(p.p, dx.p, dy.p, dm.p, n, dg.p, dl.p, nullptr);
+            SUBWAY_CHECK(hipMemcpyAsync(dls.p + (t - 1), dl.p, sizeof(float), hipMemcpyDeviceToDevice, nullptr));
+            launch_mlp_adam(p.p, dg.p, m1.p, m2.p, t, lr, nullptr);
+        }
+        SUBWAY_CHECK(hipMemcpy(params, p.p, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss_per_step && steps > 0)
+            SUBWAY_CHECK(hipMemcpy(loss_per_step, dls.p, steps * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 

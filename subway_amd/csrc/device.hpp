@@ -47,6 +47,10 @@ __device__ __forceinline__ u32 fetch_add_acq_rel(u32 *p, u32 v)
 {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ u32 fetch_add_relaxed(u32 *p, u32 v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ------------------------------------------------------------- wave scans
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
@@ -88,65 +92,93 @@ __device__ __forceinline__ u32 wave_reduce(u32 x)
 }
 
 // --------------------------------------------- epoch-tagged tile descriptor
-// word = epoch(30) | state(2) | count(32).  A descriptor whose epoch differs
-// from the running launch's epoch reads as "not yet published".
+// A tile publishes two self-contained 64-bit words, each epoch(22) |
+// state(2) | value(40): one carries the selected count, the other the weight
+// sum.  A reader accepts a tile only when both words carry the running
+// launch's epoch and the same state, so no fence is needed between them
+// (every access is a relaxed agent-scope atomic); a word from an older
+// launch reads as "not yet published".
 constexpr u32 ST_NONE = 0, ST_AGG = 1, ST_INC = 2;
-constexpr u32 EPOCH_MASK = 0x3FFFFFFFu;
+constexpr u32 EPOCH_BITS = 22;
+constexpr u32 EPOCH_MASK = (1u << EPOCH_BITS) - 1u;
+constexpr u64 VALUE_MASK = (1ull << 40) - 1ull;
 
-__device__ __forceinline__ u64 pack_desc(u32 epoch, u32 state, u32 count)
+__device__ __forceinline__ u64 pack_desc(u32 epoch, u32 state, u64 value)
 {
-    return ((u64)(epoch & EPOCH_MASK) << 34) | ((u64)state << 32) | (u64)count;
+    return ((u64)(epoch & EPOCH_MASK) << 42) | ((u64)state << 40) | (value & VALUE_MASK);
 }
-__device__ __forceinline__ u32 desc_epoch(u64 w) { return (u32)(w >> 34); }
-__device__ __forceinline__ u32 desc_state(u64 w) { return (u32)(w >> 32) & 3u; }
-__device__ __forceinline__ u32 desc_count(u64 w) { return (u32)w; }
+__device__ __forceinline__ u32 desc_epoch(u64 w) { return (u32)(w >> 42); }
+__device__ __forceinline__ u32 desc_state(u64 w) { return (u32)(w >> 40) & 3u; }
+__device__ __forceinline__ u64 desc_value(u64 w) { return w & VALUE_MASK; }
+
+// Error word and poll limit of a bounded wait.
+struct Watch {
+    u32 *err;
+    u32 limit;
+};
 
 // Decoupled lookback, executed by one full wave of the workgroup.  Publishes
 // the tile's aggregate, walks back 64 predecessors at a time, and publishes
 // the inclusive prefix.  Returns the exclusive (count, weight) prefix on every
-// lane of the calling wave.
-__device__ __forceinline__ void lookback(const TileDesc &td, u32 tile, u32 epoch,
-                                         u32 tile_cnt, u64 tile_sum,
-                                         u32 &excl_cnt, u64 &excl_sum)
+// lane of the calling wave.  td.flag holds the count words, td.inc the sum
+// words.  Returns false (prefix unusable, WAIT_LOOKBACK recorded) when a
+// predecessor stays unpublished for more than wt.limit polls or another wait
+// of the launch has already given up.
+__device__ __forceinline__ bool lookback(const TileDesc &td, u32 tile, u32 epoch, u32 tile_cnt, u64 tile_sum,
+                                         u32 &excl_cnt, u64 &excl_sum, const Watch &wt)
 {
     const int lane = lane_id();
+    excl_cnt = 0;
+    excl_sum = 0;
     if (tile == 0) {
         if (lane == 0) {
-            st_relaxed(&td.inc[0], tile_sum);
-            st_release(&td.flag[0], pack_desc(epoch, ST_INC, tile_cnt));
+            st_relaxed(&td.flag[0], pack_desc(epoch, ST_INC, tile_cnt));
+            st_relaxed(&td.inc[0], pack_desc(epoch, ST_INC, tile_sum));
         }
-        excl_cnt = 0;
-        excl_sum = 0;
-        return;
+        return true;
     }
     if (lane == 0) {
-        st_relaxed(&td.agg[tile], tile_sum);
-        st_release(&td.flag[tile], pack_desc(epoch, ST_AGG, tile_cnt));
+        st_relaxed(&td.flag[tile], pack_desc(epoch, ST_AGG, tile_cnt));
+        st_relaxed(&td.inc[tile], pack_desc(epoch, ST_AGG, tile_sum));
     }
-    u32 run_cnt = 0;
+    u64 run_cnt = 0;
     u64 run_sum = 0;
     int64_t base = (int64_t)tile - 1;
     const u32 ep = epoch & EPOCH_MASK;
     for (;;) {
         int64_t idx = base - lane;
-        u64 w = 0;
+        u64 wc = 0, ws = 0;
         u32 st = ST_INC;  // lanes before tile 0 read as an empty inclusive prefix
+        bool gaveUp = false;
         if (idx >= 0) {
+            u32 polls = 0;
             for (;;) {
-                w = ld_acquire(&td.flag[idx]);
-                if (desc_epoch(w) == ep && desc_state(w) != ST_NONE) break;
+                wc = ld_relaxed(&td.flag[idx]);
+                ws = ld_relaxed(&td.inc[idx]);
+                const u32 sc = desc_state(wc);
+                if (desc_epoch(wc) == ep && desc_epoch(ws) == ep && (sc == ST_AGG || sc == ST_INC) &&
+                    sc == desc_state(ws))
+                    break;
+                ++polls;
+                if (polls > wt.limit || ((polls & 63u) == 0 && ld_relaxed(wt.err) != 0)) {
+                    gaveUp = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
-            st = desc_state(w);
+            st = desc_state(wc);
+        }
+        if (__ballot(gaveUp)) {
+            if (lane == 0) st_relaxed(wt.err, WAIT_LOOKBACK);
+            return false;
         }
         unsigned long long incm = __ballot(st == ST_INC);
         // incm is never zero once idx < 0 appears, so the walk terminates
         int first = incm ? (__ffsll((long long)incm) - 1) : WAVE;
-        u32 c = 0;
-        u64 s = 0;
+        u64 c = 0, s = 0;
         if (lane <= first && idx >= 0) {
-            c = desc_count(w);
-            s = (lane == first) ? ld_relaxed(&td.inc[idx]) : ld_relaxed(&td.agg[idx]);
+            c = desc_value(wc);
+            s = desc_value(ws);
         }
         run_cnt += wave_reduce(c);
         run_sum += wave_reduce(s);
@@ -154,11 +186,12 @@ __device__ __forceinline__ void lookback(const TileDesc &td, u32 tile, u32 epoch
         base -= WAVE;
     }
     if (lane == 0) {
-        st_relaxed(&td.inc[tile], run_sum + tile_sum);
-        st_release(&td.flag[tile], pack_desc(epoch, ST_INC, run_cnt + tile_cnt));
+        st_relaxed(&td.flag[tile], pack_desc(epoch, ST_INC, run_cnt + tile_cnt));
+        st_relaxed(&td.inc[tile], pack_desc(epoch, ST_INC, run_sum + tile_sum));
     }
-    excl_cnt = run_cnt;
+    excl_cnt = (u32)run_cnt;
     excl_sum = run_sum;
+    return true;
 }
 
 }  // namespace dev

@@ -156,6 +156,10 @@ public:
         const ull chunk = o_.batch ? (ull)BATCH_CHUNK_EDGES : (ull)ADV_CHUNK;
         SUBWAY_CHECK(hipMalloc(&chunkMap_, (maxWork / chunk + 2) * sizeof(uint32_t)));
         SUBWAY_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(ScanCtl), 0));
+        SUBWAY_CHECK(hipHostMalloc((void **)&h_err_, 4 * sizeof(uint32_t), 0));
+        std::memset(h_err_, 0, 4 * sizeof(uint32_t));
+        fr_.spinLimit = o_.spinLimit ? o_.spinLimit : dev::SPIN_LIMIT_DEFAULT;
+        gen_->scanScratch().spinLimit = fr_.spinLimit;
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
@@ -187,6 +191,13 @@ public:
             if (o_.algo != BFS && o_.algo != SSSP && o_.algo != SSWP)
                 throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
             if (o_.mode != ASYNC) throw Error(ERR_ARG, "batched runs are async");
+            // A vertex whose edges exceed one buffer would need the chunked
+            // (windowed) batched advance, which is switched off until its
+            // time-out at small budgets is explained.
+            if ((ull)g_->csr->maxDegree > maxPart_)
+                throw Error(ERR_BUDGET, "batched runs need every vertex's edges to fit one buffer: max out-degree " +
+                                            std::to_string(g_->csr->maxDegree) + " > " + std::to_string(maxPart_) +
+                                            " edges per buffer; raise the budget");
             const ull n = g_->num_nodes;
             SUBWAY_CHECK(hipMalloc(&d_dist_, n * BATCH_LANES * sizeof(uint32_t)));
             SUBWAY_CHECK(hipMalloc(&d_m1_, n * sizeof(ull)));
@@ -237,6 +248,7 @@ public:
         if (entries_) (void)hipFree(entries_);
         if (chunkMap_) (void)hipFree(chunkMap_);
         if (h_ctl_) (void)hipHostFree(h_ctl_);
+        if (h_err_) (void)hipHostFree(h_err_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
             (void)hipEventDestroy(evLoaded_[b]);
@@ -269,6 +281,8 @@ private:
     Entry *entries_ = nullptr;
     uint32_t *chunkMap_ = nullptr;
     ScanCtl *h_ctl_ = nullptr;
+    uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
+    bool broken_ = false;         // a device wait gave up: the device state is no longer trusted
     ull maxPart_ = 0;
     int advGrid_ = 0;
     RunStats st_;
@@ -314,7 +328,8 @@ private:
     void computeBatch(const Unit &u, int b);
     RunStats loop(const std::chrono::steady_clock::time_point &t0);
     void resetRun();
-    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0);
+    void checkWaits();
+    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0);
     uint *pinnedIds(ull n)
     {
         if (n > idsCap_) {
@@ -362,9 +377,33 @@ void Engine<E, G>::generateAndPartition()
                            d_rstats_, sc_);
         SUBWAY_CHECK(hipMemcpyAsync(h_rstats_, d_rstats_, (size_t)P * 4 * sizeof(ull), hipMemcpyDeviceToHost, sc_));
     }
+    // error words of every bounded wait so far; the partitioner synchronises
+    // the stream before they are read
+    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[0], &fr_.ctl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[1], &gen_->scanScratch().ctl->error, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, sc_));
+    if (pc_) SUBWAY_CHECK(hipMemcpyAsync(&h_err_[2], &pc_->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
     part_->partition(*sub_, gen_->deviceTotals(), maxPart_);
+    checkWaits();
     sub_->numActiveNodes = (uint)part_->numActiveNodes;
     sub_->numActiveEdges = part_->numActiveEdges;
+}
+
+// A bounded device wait that gave up leaves partial results: the run fails
+// and the engine refuses further runs.
+template <class E, class G>
+void Engine<E, G>::checkWaits()
+{
+    static const char *where[3] = {"frontier scan", "generator", "rounds kernel"};
+    for (int i = 0; i < 3; i++) {
+        if (h_err_[i]) {
+            broken_ = true;
+            throw Error(ERR_INTERNAL, std::string("device wait gave up in the ") + where[i] + " (" +
+                                          wait_error_name(h_err_[i]) + ", " + std::to_string(fr_.spinLimit) +
+                                          " polls) at global iteration " +
+                                          std::to_string(st_.globalIterations) + "; the session must be rebuilt");
+        }
+    }
 }
 
 template <class E, class G>
@@ -700,9 +739,8 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.entries = entries_;
         l.chunkMap = chunkMap_;
         l.edges = sub_->d_buffers[buf];
-        l.sc = fr_.ctl;
+        l.scan = &fr_;
         l.pc = pc_;
-        l.td = fr_.td;
         l.maxRounds = (uint32_t)std::max(1, roundsCap_);
         l.roundBase = 0;
         if (o_.profile) pRounds_.begin(sc_);
@@ -721,7 +759,9 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
             advance(odd ? l2 : l1, 0, ~0ull, 0, buf);
         }
         SUBWAY_CHECK(hipMemcpyAsync(&h_ctl_->stop, &fr_.ctl->stop, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+        SUBWAY_CHECK(hipMemcpyAsync(&h_err_[0], &fr_.ctl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
         SUBWAY_CHECK(hipStreamSynchronize(sc_));
+        if (h_err_[0]) checkWaits();
         if (h_ctl_->stop) break;
         batch = std::min(batch * 2, std::max(1, o_.maxRoundBatch));
     }
@@ -766,20 +806,12 @@ void Engine<E, G>::compute(size_t idx)
     }
 }
 
-// Batched partition: all rounds for every source in one cooperative launch;
-// an oversized vertex gets one masked frontier and windowed advances.
+// Batched partition: all rounds for every source in one cooperative launch.
+// Oversized vertices are rejected when the batched engine is set up.
 template <class E, class G>
 void Engine<E, G>::computeBatch(const Unit &u, int b)
 {
-    if (u.hub) {
-        if (u.chunk == 0)
-            launch_frontier_masks(u.n, u.nodes, u.ptr, u.fromEdge, d_m1_, d_m2_, entries_, d_entryMask_, chunkMap_,
-                                  fr_, sc_, u.idBase);
-        const ull w0 = u.chunk * maxPart_;
-        launch_advance_batch_window(o_.algo, (int)sizeof(E), entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
-                                    d_dist_, d_m2_, w0, w0 + u.count, w0, fr_.ctl, advGrid_, sc_);
-        return;
-    }
+    if (u.hub) throw Error(ERR_INTERNAL, "batched run reached an oversized vertex");
     BatchLaunch l{};
     l.n = u.n;
     l.nodes = u.nodes;
@@ -793,9 +825,8 @@ void Engine<E, G>::computeBatch(const Unit &u, int b)
     l.entryMask = d_entryMask_;
     l.chunkMap = chunkMap_;
     l.edges = sub_->d_buffers[b];
-    l.sc = fr_.ctl;
+    l.scan = &fr_;
     l.pc = pc_;
-    l.td = fr_.td;
     l.maxRounds = (uint32_t)std::max(1, roundsCap_);
     if (o_.profile) pRounds_.begin(sc_);
     launch_rounds_batch(o_.algo, (int)sizeof(E), batchUnroll_, l, batchGrid_, sc_);
@@ -813,6 +844,7 @@ ull Engine<E, G>::deviceRounds()
 template <class E, class G>
 void Engine<E, G>::resetRun()
 {
+    if (broken_) throw Error(ERR_INTERNAL, "a device wait of an earlier run gave up; the session must be rebuilt");
     st_ = RunStats();
     roundsCap_ = o_.maxRounds;
     theta_ = o_.theta;
@@ -867,7 +899,7 @@ RunStats Engine<E, G>::run(uint source)
     initState();
     const uint n = g_->num_nodes;
     ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
-    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum;
+    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -892,7 +924,7 @@ RunStats Engine<E, G>::run(uint source)
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
-    finishStats(rounds0, updates0, work0, entries0);
+    finishStats(rounds0, updates0, work0, entries0, items0);
     return st_;
 }
 
@@ -914,7 +946,7 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     SUBWAY_CHECK(hipMemcpyAsync(d_sources_, h_sources_, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, sc_));
     launch_init_batch_state(o_.algo, n, d_dist_, d_m1_, d_m2_, 1, sc_);
     ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
-    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum;
+    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -932,18 +964,19 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     batchK_ = 0;
-    finishStats(rounds0, updates0, work0, entries0);
+    finishStats(rounds0, updates0, work0, entries0, items0);
     return st_;
 }
 
 template <class E, class G>
-void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0)
+void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0)
 {
     SUBWAY_CHECK(hipStreamSynchronize(sx_));
     st_.innerRounds = deviceRounds() - rounds0;
     st_.updates = h_ctl_->updates - updates0;
     st_.workEdges = h_ctl_->work_sum - work0;
     st_.workEntries = h_ctl_->entry_sum - entries0;
+    st_.workItems = h_ctl_->item_sum - items0;
     if (o_.profile) {
         ull c = 0;
         st_.msAdvance = pAdv_.total(st_.advanceLaunches);

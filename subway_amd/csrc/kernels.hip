@@ -30,18 +30,29 @@ namespace subway {
 
 using namespace dev;
 
+// Selected-item count of a scan whose totals block is [count, total, error]
+// (the layout of ScanCtl); a scan whose lookback gave up selected nothing
+// usable, so its consumers see zero items.
+__device__ __forceinline__ u64 selected_count(const u64 *totals)
+{
+    const u64 c = ld_relaxed(&totals[0]);
+    return ld_relaxed(&totals[2]) ? 0ull : c;
+}
+
 // ====================================================== select-scan tile
 // One tile of flag -> workgroup scan -> decoupled lookback -> compaction.
 // `lastTile` names the tile that publishes the totals through op.finish.
+// Returns false, having emitted nothing, when the lookback gave up.
 template <class Op, int BLOCK, int IPT>
-__device__ __forceinline__ void scan_tile(const Op &op, u64 n, u32 tile, u32 epoch, u32 lastTile,
-                                          const TileDesc &td, ScanCtl *ctl)
+__device__ __forceinline__ bool scan_tile(const Op &op, u64 n, u32 tile, u32 epoch, u32 lastTile,
+                                          const TileDesc &td, ScanCtl *ctl, const Watch &wt)
 {
     constexpr int NW = BLOCK / WAVE;
     __shared__ u32 s_wcnt[NW];
     __shared__ u64 s_wsum[NW];
     __shared__ u32 s_xcnt;
     __shared__ u64 s_xsum;
+    __shared__ u32 s_ok;
     const int tid = threadIdx.x;
 
     const u64 i0 = (u64)tile * (u64)(BLOCK * IPT) + (u64)tid * IPT;
@@ -78,13 +89,18 @@ __device__ __forceinline__ void scan_tile(const Op &op, u64 n, u32 tile, u32 epo
     if (wid == 0) {
         u32 xc;
         u64 xs;
-        lookback(td, tile, epoch, totc, tots, xc, xs);
+        const bool ok = lookback(td, tile, epoch, totc, tots, xc, xs, wt);
         if (lane == 0) {
             s_xcnt = xc;
             s_xsum = xs;
+            s_ok = ok ? 1u : 0u;
         }
     }
     __syncthreads();
+    if (!s_ok) {
+        __syncthreads();
+        return false;
+    }
     u64 k = (u64)s_xcnt + wbc + (icnt - cnt);
     u64 pre = s_xsum + wbs + (isum - sum);
 #pragma unroll
@@ -95,14 +111,18 @@ __device__ __forceinline__ void scan_tile(const Op &op, u64 n, u32 tile, u32 epo
             pre += w[j];
         }
     }
-    if (tile == lastTile && tid == 0) op.finish((u64)s_xcnt + totc, s_xsum + tots, ctl);
+    if (tile == lastTile && tid == 0) {
+        op.finish((u64)s_xcnt + totc, s_xsum + tots, ctl);
+        ctl->item_sum += n;
+    }
     __syncthreads();
+    return true;
 }
 
 // Standalone launch: one workgroup per tile, tiles claimed in start order.
 template <class Op, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCtl *ctl,
-                                                            TileDesc td, int first)
+                                                            TileDesc td, int first, u32 spinLimit)
 {
     __shared__ u32 s_tile, s_epoch;
     __shared__ int s_exit;
@@ -112,16 +132,17 @@ __global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCt
         s_exit = ex;
         if (!ex) {
             s_epoch = ld_relaxed(&ctl->epoch);
-            s_tile = fetch_add_acq_rel(&ctl->tile_counter, 1u);
+            s_tile = fetch_add_relaxed(&ctl->tile_counter, 1u);
         }
     }
     __syncthreads();
     if (s_exit) return;
     const u32 tile = s_tile, epoch = s_epoch;
-    scan_tile<Op, BLOCK, IPT>(op, n, tile, epoch, gridDim.x - 1, td, ctl);
+    (void)scan_tile<Op, BLOCK, IPT>(op, n, tile, epoch, gridDim.x - 1, td, ctl, Watch{&ctl->error, spinLimit});
+    // the counters only need the count of finished workgroups; the outputs
+    // are consumed by later launches
     if (tid == 0) {
-        __threadfence();
-        u32 d = fetch_add_acq_rel(&ctl->done_counter, 1u);
+        u32 d = fetch_add_relaxed(&ctl->done_counter, 1u);
         if (d == gridDim.x - 1) {
             st_relaxed(&ctl->done_counter, 0u);
             st_relaxed(&ctl->tile_counter, 0u);
@@ -594,6 +615,8 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 {
     __shared__ u32 s_chunk;
     const int tid = threadIdx.x;
+    // a frontier scan whose wait gave up left no usable entries
+    if (ld_relaxed(&ctl->error)) return;
     const u64 count = ctl->count;
     const u64 W = ctl->total;
     const u64 wend = W < w1 ? W : w1;
@@ -614,8 +637,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
     if (tid == 0) {
-        __threadfence();
-        u32 d = fetch_add_acq_rel(&ctl->adv_done, 1u);
+        u32 d = fetch_add_relaxed(&ctl->adv_done, 1u);
         if (d == gridDim.x - 1) {
             st_relaxed(&ctl->adv_done, 0u);
             st_relaxed(&ctl->adv_counter, 0u);
@@ -628,8 +650,12 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 // Thread 0 of each workgroup releases the workgroup's writes at agent scope,
 // arrives at its group counter, the last of a group arrives at the top
 // counter, and the last overall bumps the generation the others spin on.
-__device__ __forceinline__ void grid_sync(PersistCtl *pc, u32 &gen)
+// The wait is bounded (`limit` polls, WAIT_BARRIER recorded on expiry).
+// Returns false on every workgroup once any wait of the launch has given up
+// (checked after the barrier, so all workgroups that pass one barrier agree).
+__device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit)
 {
+    __shared__ u32 s_fail;
     __syncthreads();
     if (threadIdx.x == 0) {
         const u32 g = gen;
@@ -648,14 +674,25 @@ __device__ __forceinline__ void grid_sync(PersistCtl *pc, u32 &gen)
                 releaser = true;
             }
         }
+        bool gaveUp = false;
         if (!releaser) {
-            while (__hip_atomic_load(&pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+            u32 polls = 0;
+            while (__hip_atomic_load(&pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                ++polls;
+                if (polls > limit || ((polls & 63u) == 0 && ld_relaxed(&pc->error) != 0)) {
+                    gaveUp = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
+            }
         }
         gen = g + 1;
         __threadfence();
+        if (gaveUp && ld_relaxed(&pc->error) == 0) st_relaxed(&pc->error, WAIT_BARRIER);
+        s_fail = (gaveUp || ld_relaxed(&pc->error) != 0) ? 1u : 0u;
     }
     __syncthreads();
+    return s_fail == 0;
 }
 
 struct RoundsArgs {
@@ -676,6 +713,7 @@ struct RoundsArgs {
     u32 maxRounds;
     u32 roundBase;         // round index (within the partition) of this launch's first round
     u32 idBase;            // nodes == null: items are the id range idBase ..
+    u32 spinLimit;         // poll limit of every wait of the launch
 };
 
 // All async inner rounds of one resident partition in one cooperative
@@ -696,6 +734,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
     u32 gen = s_gen;
     const u32 epoch0 = s_epoch;
     const u32 numTiles = (u32)((a.n + FR_TILE - 1) / FR_TILE);
+    const Watch wt{&a.pc->error, a.spinLimit};
     u32 r = 0, conv = 0, updates = 0;
     for (; r < a.maxRounds; r++) {
         const u32 rg = a.roundBase + r;
@@ -712,14 +751,20 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             if constexpr (ALGO == ALGO_PR) {
                 FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
                                 a.idBase};
-                scan_tile<FrontierPrOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+                if (!scan_tile<FrontierPrOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
+                                                                 wt))
+                    break;
             } else {
                 FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
                                    a.entries, a.chunkMap, a.idBase};
-                scan_tile<FrontierLabelOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+                if (!scan_tile<FrontierLabelOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td,
+                                                                    a.sc, wt))
+                    break;
             }
         }
-        grid_sync(a.pc, gen);
+        // a wait that gave up leaves the launch's results unusable; every
+        // workgroup leaves here and the host reports the error word
+        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
@@ -741,7 +786,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             updates += advance_chunk<ALGO, E>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
                                               a.delta, count, W, 0, W, 0);
         }
-        grid_sync(a.pc, gen);
+        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
@@ -897,6 +942,7 @@ struct BatchArgs {
     TileDesc td;
     u32 maxRounds;
     u32 idBase;
+    u32 spinLimit;
 };
 
 // All async rounds of one resident partition for up to 64 sources at once,
@@ -914,6 +960,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
     u32 gen = s_gen;
     const u32 epoch0 = s_epoch;
     const u32 numTiles = (u32)((a.n + FR_TILE - 1) / FR_TILE);
+    const Watch wt{&a.pc->error, a.spinLimit};
     u32 r = 0, conv = 0, updates = 0;
     for (; r < a.maxRounds; r++) {
         const bool even = (r & 1u) == 0;
@@ -927,9 +974,11 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             if (t >= numTiles) break;
             FrontierMaskOp op{a.nodes, a.ptr, a.fromEdge, labIn, r == 0 ? a.m2 : nullptr, a.entries, a.entryMask,
                               a.chunkMap, a.idBase};
-            scan_tile<FrontierMaskOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc);
+            if (!scan_tile<FrontierMaskOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
+                                                               wt))
+                break;
         }
-        grid_sync(a.pc, gen);
+        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
@@ -950,7 +999,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             updates += advance_chunk_batch<ALGO, E, U>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
                                                        a.dist, labOut, count, W, 0, W, 0);
         }
-        grid_sync(a.pc, gen);
+        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
@@ -963,28 +1012,6 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
         st_relaxed(&a.pc->chunk_ctr[0], 0u);
         st_relaxed(&a.pc->chunk_ctr[1], 0u);
     }
-}
-
-// Standalone batched advance over the work window [w0, w1) (a vertex larger
-// than a buffer, streamed chunk by chunk).
-template <int ALGO, class E>
-__global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *entries, const u64 *entryMask,
-                                                                  const u32 *chunkMap, const E *edges, u32 *dist,
-                                                                  u64 *maskOut, u64 w0, u64 w1, u64 shift,
-                                                                  ScanCtl *ctl)
-{
-    const u64 count = ctl->count;
-    const u64 W = ctl->total;
-    const u64 wend = W < w1 ? W : w1;
-    if (count == 0 || wend <= w0) return;
-    const u64 j0 = w0 / BATCH_CHUNK;
-    const u64 j1 = (wend + BATCH_CHUNK - 1) / BATCH_CHUNK;
-    u32 updates = 0;
-    for (u64 j = j0 + blockIdx.x; j < j1; j += gridDim.x)
-        updates += advance_chunk_batch<ALGO, E, 4>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
-                                                   w0, wend, shift);
-    u32 wsum = wave_reduce(updates);
-    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
 }
 
 // Batched initial state: every distance INF (SSWP: 0), masks cleared
@@ -1044,7 +1071,7 @@ __global__ __launch_bounds__(64) void partition_kernel(const u64 *ptr, const u32
                                                        u64 start, u32 maxRows, PartRow *rows,
                                                        u64 *out)
 {
-    const u64 A = dA ? *dA : hostA;
+    const u64 A = dA ? selected_count(dA) : hostA;
     u64 from = start;
     u32 r = 0;
     while (from < A && r < maxRows) {
@@ -1094,7 +1121,7 @@ __global__ void range_stats_kernel(const u32 *activeNodes, const u64 *anp, const
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    const u64 A = *dA;
+    const u64 A = selected_count(dA);
     const u64 lo = lower_bound_u32(activeNodes, A, bounds[i]);
     const u64 hi = lower_bound_u32(activeNodes, A, bounds[i + 1]);
     out[4 * i + 0] = lo;
@@ -1234,26 +1261,68 @@ void alloc_scan_scratch(ScanScratch &s, ull maxTiles)
     SUBWAY_CHECK(hipMalloc(&s.ctl, sizeof(ScanCtl)));
     SUBWAY_CHECK(hipMemset(s.ctl, 0, sizeof(ScanCtl)));
     SUBWAY_CHECK(hipMalloc(&s.td.flag, maxTiles * sizeof(u64)));
-    SUBWAY_CHECK(hipMalloc(&s.td.agg, maxTiles * sizeof(u64)));
     SUBWAY_CHECK(hipMalloc(&s.td.inc, maxTiles * sizeof(u64)));
     // epoch tags start at 0 in the control block; descriptors are cleared to
-    // an impossible tag so stale memory never reads as published
+    // state 3, which no reader accepts, so stale memory never reads as
+    // published
     SUBWAY_CHECK(hipMemset(s.td.flag, 0xFF, maxTiles * sizeof(u64)));
+    SUBWAY_CHECK(hipMemset(s.td.inc, 0xFF, maxTiles * sizeof(u64)));
     SUBWAY_CHECK(hipDeviceSynchronize());
     s.maxTiles = maxTiles;
+    s.epochsIssued = 0;
+}
+
+uint32_t scan_wait_error(const ScanScratch &s, hipStream_t st)
+{
+    uint32_t e = 0;
+    SUBWAY_CHECK(hipMemcpyAsync(&e, &s.ctl->error, sizeof(e), hipMemcpyDeviceToHost, st));
+    SUBWAY_CHECK(hipStreamSynchronize(st));
+    return e;
+}
+
+const char *wait_error_name(uint32_t code)
+{
+    switch (code) {
+        case 0: return "none";
+        case WAIT_LOOKBACK: return "scan lookback";
+        case WAIT_BARRIER: return "grid barrier";
+        default: return "unknown wait";
+    }
 }
 
 void free_scan_scratch(ScanScratch &s)
 {
     if (s.ctl) (void)hipFree(s.ctl);
     if (s.td.flag) (void)hipFree(s.td.flag);
-    if (s.td.agg) (void)hipFree(s.td.agg);
     if (s.td.inc) (void)hipFree(s.td.inc);
     s = ScanScratch{};
 }
 
-static void check_tiles(const ScanScratch &s, u64 tiles)
+__global__ void fill_u64_kernel(u64 *p, u64 v, u64 n)
 {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Charge `epochs` launch epochs to the scratch; clears the descriptors (with a
+// fill kernel, in stream order) before the 22-bit tag could wrap onto a word
+// that is still in memory.
+static void charge_epochs(ScanScratch &s, u64 epochs, hipStream_t st)
+{
+    constexpr u64 LIMIT = 1ull << 21;
+    if (epochs >= LIMIT) throw Error(ERR_ARG, "too many scan epochs in one launch");
+    if (s.epochsIssued + epochs >= LIMIT) {
+        const unsigned g = (unsigned)std::min<u64>((s.maxTiles + 255) / 256, 4096);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(g), dim3(256), 0, st, s.td.flag, ~0ull, (u64)s.maxTiles);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(g), dim3(256), 0, st, s.td.inc, ~0ull, (u64)s.maxTiles);
+        SUBWAY_CHECK(hipGetLastError());
+        s.epochsIssued = 0;
+    }
+    s.epochsIssued += epochs;
+}
+
+static void check_tiles(ScanScratch &s, u64 tiles, hipStream_t st)
+{
+    charge_epochs(s, 1, st);
     if (tiles > s.maxTiles)
         throw Error(ERR_INTERNAL, "scan launch needs " + std::to_string(tiles) +
                                       " tiles but scratch holds " + std::to_string(s.maxTiles));
@@ -1266,10 +1335,10 @@ void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
 {
     if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
     u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles);
+    check_tiles(s, tiles, st);
     GenLabelOp op{l1, l2, outDegree, activeNodes, activeNodesPointer};
     hipLaunchKernelGGL((select_scan_kernel<GenLabelOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1279,10 +1348,10 @@ void launch_generate_pr(ull n, const float *delta, float acc, const uint32_t *ou
 {
     if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
     u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles);
+    check_tiles(s, tiles, st);
     GenPrOp op{delta, acc, outDegree, activeNodes, activeNodesPointer};
     hipLaunchKernelGGL((select_scan_kernel<GenPrOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1322,17 +1391,18 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
 {
     if (a.n == 0) throw Error(ERR_ARG, "frontier over an empty partition");
     u64 tiles = (a.n + FR_TILE - 1) / FR_TILE;
-    check_tiles(s, tiles);
+    check_tiles(s, tiles, st);
     if (a.algo == ALGO_PR) {
         FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
                         a.idBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
-                           dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first);
+                           dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first,
+                           s.spinLimit);
     } else {
         FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap, a.idBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp, SCAN_BLOCK, FR_IPT>),
                            dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td,
-                           a.first);
+                           a.first, s.spinLimit);
     }
     SUBWAY_CHECK(hipGetLastError());
 }
@@ -1451,9 +1521,11 @@ void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hip
     a.entries = l.entries;
     a.chunkMap = l.chunkMap;
     a.edges = l.edges;
-    a.sc = l.sc;
+    charge_epochs(*l.scan, (u64)l.maxRounds + 1, st);
+    a.sc = l.scan->ctl;
     a.pc = l.pc;
-    a.td = l.td;
+    a.td = l.scan->td;
+    a.spinLimit = l.scan->spinLimit;
     a.maxRounds = l.maxRounds;
     a.roundBase = l.roundBase;
     a.idBase = l.idBase;
@@ -1467,10 +1539,10 @@ void launch_generate_masks(ull n, const ull *m1, const ull *m2, const uint32_t *
 {
     if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
     u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles);
+    check_tiles(s, tiles, st);
     GenMaskOp op{m1, m2, outDegree, activeNodes, activeNodesPointer};
     hipLaunchKernelGGL((select_scan_kernel<GenMaskOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1494,39 +1566,10 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
 {
     if (n == 0) throw Error(ERR_ARG, "frontier over an empty partition");
     u64 tiles = (n + FR_TILE - 1) / FR_TILE;
-    check_tiles(s, tiles);
+    check_tiles(s, tiles, st);
     FrontierMaskOp op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase};
     hipLaunchKernelGGL((select_scan_kernel<FrontierMaskOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1);
-    SUBWAY_CHECK(hipGetLastError());
-}
-
-template <int ALGO, class E>
-static void adv_batch(const Entry *entries, const ull *entryMask, const uint32_t *chunkMap, const void *edges,
-                      uint32_t *dist, ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
-{
-    hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)grid), dim3(ADV_BLOCK), 0, st, entries,
-                       entryMask, chunkMap, (const E *)edges, dist, maskOut, (u64)w0, (u64)w1, (u64)shift, ctl);
-}
-
-void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
-                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
-                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
-{
-    if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
-    if (algo == ALGO_SSSP && edgeBytes == 8)
-        adv_batch<ALGO_SSSP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                              grid, st);
-    else if (algo == ALGO_SSWP && edgeBytes == 8)
-        adv_batch<ALGO_SSWP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                              grid, st);
-    else if (algo == ALGO_BFS && edgeBytes == 8)
-        adv_batch<ALGO_BFS, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                             grid, st);
-    else if (algo == ALGO_BFS && edgeBytes == 4)
-        adv_batch<ALGO_BFS, OutEdge>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl, grid, st);
-    else
-        throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP (SSSP/SSWP on weighted edges)");
+                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1592,9 +1635,11 @@ void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch 
     a.entryMask = l.entryMask;
     a.chunkMap = l.chunkMap;
     a.edges = l.edges;
-    a.sc = l.sc;
+    charge_epochs(*l.scan, (u64)l.maxRounds + 1, st);
+    a.sc = l.scan->ctl;
     a.pc = l.pc;
-    a.td = l.td;
+    a.td = l.scan->td;
+    a.spinLimit = l.scan->spinLimit;
     a.maxRounds = l.maxRounds;
     a.idBase = l.idBase;
     void *params[] = {&a};

@@ -56,7 +56,18 @@ struct ScanScratch {
     ScanCtl *ctl = nullptr;
     TileDesc td{};
     ull maxTiles = 0;
+    // Launch epochs handed out since the descriptors were last cleared.  A
+    // descriptor word holds 22 epoch bits, so the descriptors are cleared
+    // before that many epochs have passed (a stale word could otherwise
+    // alias the running epoch).
+    ull epochsIssued = 0;
+    uint32_t spinLimit = dev::SPIN_LIMIT_DEFAULT;   // poll limit of the scans' bounded waits
 };
+
+// Error word of the scratch's control block (0: no wait gave up); copies it
+// to the host and synchronises the stream.
+uint32_t scan_wait_error(const ScanScratch &s, hipStream_t st);
+const char *wait_error_name(uint32_t code);
 
 void alloc_scan_scratch(ScanScratch &s, ull maxTiles);
 void free_scan_scratch(ScanScratch &s);
@@ -141,9 +152,8 @@ struct RoundsLaunch {
     Entry *entries;
     uint32_t *chunkMap;
     const void *edges;
-    ScanCtl *sc;
+    ScanScratch *scan;     // frontier scan scratch (its epoch budget is charged)
     dev::PersistCtl *pc;
-    TileDesc td;
     uint32_t maxRounds;
     uint32_t roundBase;
     uint32_t idBase;       // nodes == null: a whole id range starting here
@@ -170,9 +180,8 @@ struct BatchLaunch {
     ull *entryMask;
     uint32_t *chunkMap;
     const void *edges;
-    ScanCtl *sc;
+    ScanScratch *scan;
     dev::PersistCtl *pc;
-    TileDesc td;
     uint32_t maxRounds;
     uint32_t idBase;
 };
@@ -180,9 +189,6 @@ int batch_max_grid(int algo, int edgeBytes, int unroll);
 void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
                            Entry *entries, ull *entryMask, uint32_t *chunkMap, ScanScratch &s, hipStream_t st,
                            uint32_t idBase = 0);
-void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
-                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
-                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
 void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
 
 // Initial vertex state (values and both label arrays / PR value and delta).

@@ -3,6 +3,7 @@
 #ifndef SUBWAY_SCAN_TYPES_HPP
 #define SUBWAY_SCAN_TYPES_HPP
 
+#include <cstddef>
 #include <cstdint>
 
 namespace subway {
@@ -21,13 +22,20 @@ struct ScanCtl {
     u32 stop;       // frontier kernels: previous launch emitted nothing
     u64 count;      // number of selected items of the last launch
     u64 total;      // sum of the selected items' weights
+    u32 error;      // nonzero: a bounded device wait gave up (WAIT_* code);
+    u32 pad0;       // count, total, error form the [count, total, error] totals block
     u32 adv_counter;  // chunk counter of the consuming advance kernel
     u32 adv_done;
     u64 updates;    // successful relaxations (statistics)
     u64 rounds;     // launches that selected at least one item
     u64 entry_sum;  // selected items over all launches (statistics)
     u64 work_sum;   // selected weight over all launches (statistics)
+    u64 item_sum;   // items scanned over all launches (statistics)
 };
+static_assert(offsetof(ScanCtl, total) == offsetof(ScanCtl, count) + 8 &&
+                  offsetof(ScanCtl, error) == offsetof(ScanCtl, count) + 16,
+              "count, total and error form one [count, total, error] block of u64 words");
+
 
 // Control block of the persistent rounds kernel: a two-level grid barrier
 // (8 group counters on separate cache lines, then one top counter), and the
@@ -41,12 +49,19 @@ struct PersistCtl {
     u32 chunk_ctr[2];
     u32 rounds_done;   // rounds that selected work in the last launch
     u32 converged;     // 1: the last launch ended on a round that selected nothing
+    u32 error;         // nonzero: a bounded device wait gave up (WAIT_* code)
 };
 
+// Every device-side wait (lookback, grid barrier) polls at most a bounded
+// number of times; on expiry it records one of these codes in the error word
+// of its control block and its kernel runs to its end without further waits.
+constexpr u32 WAIT_LOOKBACK = 1;
+constexpr u32 WAIT_BARRIER = 2;
+constexpr u32 SPIN_LIMIT_DEFAULT = 1u << 22;   // polls (each a device-scope load + s_sleep)
+
 struct TileDesc {
-    u64 *flag;     // packed descriptors
-    u64 *agg;      // weight aggregate of the tile
-    u64 *inc;      // inclusive weight prefix through the tile
+    u64 *flag;     // per tile: epoch | state | selected count
+    u64 *inc;      // per tile: epoch | state | weight sum
 };
 
 }  // namespace dev

@@ -2,6 +2,9 @@
 // (reference: shared/subgraph_generator.cu:100-285).
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+#include <string>
+
 #include "kernels.hpp"
 #include "subway/gpu_error_check.hpp"
 #include "subway/subgraph_generator.hpp"
@@ -12,14 +15,14 @@ template <class E>
 SubgraphGenerator<E>::SubgraphGenerator(Graph<E> &graph) : scratch(new ScanScratch), h_totals(nullptr)
 {
     alloc_scan_scratch(*scratch, ((ull)graph.num_nodes + GEN_TILE - 1) / GEN_TILE);
-    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 2 * sizeof(ull), 0));
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 3 * sizeof(ull), 0));
 }
 
 template <class E>
 SubgraphGenerator<E>::SubgraphGenerator(GraphPR<E> &graph) : scratch(new ScanScratch), h_totals(nullptr)
 {
     alloc_scan_scratch(*scratch, ((ull)graph.num_nodes + GEN_TILE - 1) / GEN_TILE);
-    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 2 * sizeof(ull), 0));
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_totals, 3 * sizeof(ull), 0));
 }
 
 template <class E>
@@ -54,8 +57,13 @@ void SubgraphGenerator<E>::enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, flo
 template <class E>
 void SubgraphGenerator<E>::readback(Subgraph<E> &subgraph)
 {
-    SUBWAY_CHECK(hipMemcpyAsync(h_totals, &scratch->ctl->count, 2 * sizeof(ull), hipMemcpyDeviceToHost, stream));
+    // [count, total, error] are adjacent u64 words of the control block
+    SUBWAY_CHECK(hipMemcpyAsync(h_totals, &scratch->ctl->count, 3 * sizeof(ull), hipMemcpyDeviceToHost, stream));
     SUBWAY_CHECK(hipStreamSynchronize(stream));
+    const uint32_t err = (uint32_t)h_totals[2];
+    if (err)
+        throw subway::Error(subway::ERR_INTERNAL,
+                            std::string("generator: device wait gave up (") + subway::wait_error_name(err) + ")");
     subgraph.numActiveNodes = (uint)h_totals[0];
     subgraph.numActiveEdges = h_totals[1];
 }
