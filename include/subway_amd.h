@@ -59,7 +59,8 @@ typedef struct subway_options {
     int32_t profile;        /* 1: time every kernel launch and transfer with HIP events */
     int32_t rounds_mode;    /* async: 0 = all rounds of a partition in one cooperative launch,
                                1 = one frontier + advance launch pair per round */
-    int32_t rounds_grid;    /* workgroups of the cooperative rounds kernel; 0 = 4 per CU */
+    int32_t rounds_grid;    /* workgroups of the cooperative rounds kernel; 0 (and the cap) = one below
+                               the co-resident maximum per CU */
     int32_t max_rounds;     /* async rounds per partition visit; 0 = 4096 */
     int32_t batch;          /* 1: batched session (BFS/SSSP/SSWP async, <= 64 sources per run) */
     int32_t batch_unroll;   /* batched advance: edges in flight per wave, 1/2/4/8; 0 = 4 */

@@ -169,7 +169,7 @@ __device__ __forceinline__ bool lookback(const TileDesc &td, u32 tile, u32 epoch
             st = desc_state(wc);
         }
         if (__ballot(gaveUp)) {
-            if (lane == 0) st_relaxed(wt.err, WAIT_LOOKBACK);
+            if (lane == 0) (void)atomicCAS(wt.err, 0u, WAIT_LOOKBACK);
             return false;
         }
         unsigned long long incm = __ballot(st == ST_INC);

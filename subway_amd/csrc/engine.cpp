@@ -191,13 +191,6 @@ public:
             if (o_.algo != BFS && o_.algo != SSSP && o_.algo != SSWP)
                 throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
             if (o_.mode != ASYNC) throw Error(ERR_ARG, "batched runs are async");
-            // A vertex whose edges exceed one buffer would need the chunked
-            // (windowed) batched advance, which is switched off until its
-            // time-out at small budgets is explained.
-            if ((ull)g_->csr->maxDegree > maxPart_)
-                throw Error(ERR_BUDGET, "batched runs need every vertex's edges to fit one buffer: max out-degree " +
-                                            std::to_string(g_->csr->maxDegree) + " > " + std::to_string(maxPart_) +
-                                            " edges per buffer; raise the budget");
             const ull n = g_->num_nodes;
             SUBWAY_CHECK(hipMalloc(&d_dist_, n * BATCH_LANES * sizeof(uint32_t)));
             SUBWAY_CHECK(hipMalloc(&d_m1_, n * sizeof(ull)));
@@ -398,10 +391,23 @@ void Engine<E, G>::checkWaits()
     for (int i = 0; i < 3; i++) {
         if (h_err_[i]) {
             broken_ = true;
-            throw Error(ERR_INTERNAL, std::string("device wait gave up in the ") + where[i] + " (" +
-                                          wait_error_name(h_err_[i]) + ", " + std::to_string(fr_.spinLimit) +
-                                          " polls) at global iteration " +
-                                          std::to_string(st_.globalIterations) + "; the session must be rebuilt");
+            const std::string what = h_err_[i] == dev::WAIT_DATA
+                                         ? std::string("device data check failed in the ") + where[i] + " (" +
+                                               wait_error_name(h_err_[i]) + ")"
+                                         : std::string("device wait gave up in the ") + where[i] + " (" +
+                                               wait_error_name(h_err_[i]) + ", " +
+                                               std::to_string(fr_.spinLimit) + " polls)";
+            std::string diag;
+            if (i == 2 && h_err_[i] == dev::WAIT_BARRIER) {
+                dev::PersistCtl pc;
+                SUBWAY_CHECK(hipMemcpy(&pc, pc_, sizeof(pc), hipMemcpyDeviceToHost));
+                diag = " [round " + std::to_string(pc.diag[0] >> 1) + (pc.diag[0] & 1 ? " advance" : " frontier") +
+                       " barrier: " + std::to_string(pc.diag[1]) + " of " + std::to_string(pc.diag[4]) +
+                       " workgroups started, group counter " + std::to_string(pc.diag[2]) + ", top counter " +
+                       std::to_string(pc.diag[3]) + "]";
+            }
+            throw Error(ERR_INTERNAL, what + diag + " at global iteration " + std::to_string(st_.globalIterations) +
+                                          "; the session must be rebuilt");
         }
     }
 }
@@ -806,12 +812,20 @@ void Engine<E, G>::compute(size_t idx)
     }
 }
 
-// Batched partition: all rounds for every source in one cooperative launch.
-// Oversized vertices are rejected when the batched engine is set up.
+// Batched partition: all rounds for every source in one cooperative launch;
+// an oversized vertex gets one masked frontier and windowed advances.
 template <class E, class G>
 void Engine<E, G>::computeBatch(const Unit &u, int b)
 {
-    if (u.hub) throw Error(ERR_INTERNAL, "batched run reached an oversized vertex");
+    if (u.hub) {
+        if (u.chunk == 0)
+            launch_frontier_masks(u.n, u.nodes, u.ptr, u.fromEdge, d_m1_, d_m2_, entries_, d_entryMask_, chunkMap_,
+                                  fr_, sc_, u.idBase);
+        const ull w0 = u.chunk * maxPart_;
+        launch_advance_batch_window(o_.algo, (int)sizeof(E), entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
+                                    d_dist_, d_m2_, w0, w0 + u.count, w0, fr_.ctl, advGrid_, sc_);
+        return;
+    }
     BatchLaunch l{};
     l.n = u.n;
     l.nodes = u.nodes;

@@ -571,7 +571,7 @@ struct Relax<ALGO_PR> {
 template <int ALGO, class E>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, float *delta, u64 count, u64 W, u64 w0,
-                                             u64 wend, u64 shift)
+                                             u64 wend, u64 shift, u32 *err)
 {
     __shared__ u32 s_pre[ADV_CHUNK + 1];
     __shared__ u32 s_loc[ADV_CHUNK + 1];
@@ -582,14 +582,21 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     const u64 pend = (cbeg + ADV_CHUNK) < wend ? (cbeg + ADV_CHUNK) : wend;
     const u64 k0 = chunkMap[j];
     const u64 k1 = (cbeg + ADV_CHUNK < W) ? chunkMap[j + 1] : count - 1;
-    const int nk = (int)(k1 - k0 + 1);
+    // the chunk's entries are [k0, k1], k1 < count, at most ADV_CHUNK + 1 of
+    // them; anything else is frontier output that is not visible (reported,
+    // never dereferenced)
+    const bool bad = k0 > k1 || k1 >= count || k1 - k0 > (u64)ADV_CHUNK;
+    const int nk = bad ? 0 : (int)(k1 - k0 + 1);
     for (int t = tid; t < nk; t += ADV_BLOCK) {
         uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
         s_loc[t] = e.x;
         s_pre[t] = e.y;
         s_val[t] = e.z;
     }
-    __syncthreads();
+    if (__syncthreads_or(bad)) {
+        if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
+        return 0;
+    }
     u32 updates = 0;
     for (u64 p = pbeg + tid; p < pend; p += ADV_BLOCK) {
         const u32 pp = (u32)p;
@@ -632,7 +639,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
         __syncthreads();
         if (c >= nchunks) break;
         updates += advance_chunk<ALGO, E>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0, wend,
-                                          shift);
+                                          shift, &ctl->error);
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -653,9 +660,14 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 // The wait is bounded (`limit` polls, WAIT_BARRIER recorded on expiry).
 // Returns false on every workgroup once any wait of the launch has given up
 // (checked after the barrier, so all workgroups that pass one barrier agree).
-__device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit)
+__device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u32 where)
 {
     __shared__ u32 s_fail;
+    // __syncthreads() waits for LDS traffic only: every wave first waits for
+    // its own global stores to reach its XCD's L2, so that thread 0's
+    // agent-scope release (an L2 write-back) covers the whole workgroup's
+    // writes before it arrives
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const u32 g = gen;
@@ -688,7 +700,13 @@ __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit)
         }
         gen = g + 1;
         __threadfence();
-        if (gaveUp && ld_relaxed(&pc->error) == 0) st_relaxed(&pc->error, WAIT_BARRIER);
+        if (gaveUp && atomicCAS(&pc->error, 0u, WAIT_BARRIER) == 0u) {
+            pc->diag[0] = where;
+            pc->diag[1] = ld_relaxed(&pc->started);
+            pc->diag[2] = ld_relaxed(&pc->bar_sub[grp * 16]);
+            pc->diag[3] = ld_relaxed(&pc->bar_top[0]);
+            pc->diag[4] = gridDim.x;
+        }
         s_fail = (gaveUp || ld_relaxed(&pc->error) != 0) ? 1u : 0u;
     }
     __syncthreads();
@@ -729,6 +747,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
     if (tid == 0) {
         s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         s_epoch = ld_relaxed(&a.sc->epoch);
+        atomicAdd(&a.pc->started, 1u);
     }
     __syncthreads();
     u32 gen = s_gen;
@@ -764,7 +783,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
         }
         // a wait that gave up leaves the launch's results unusable; every
         // workgroup leaves here and the host reports the error word
-        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
@@ -784,14 +803,15 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk<ALGO, E>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
-                                              a.delta, count, W, 0, W, 0);
+                                              a.delta, count, W, 0, W, 0, wt.err);
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
     if (blockIdx.x == 0 && tid == 0) {
         st_relaxed(&a.sc->epoch, epoch0 + r + conv);
+        st_relaxed(&a.pc->started, 0u);   // every workgroup has passed the last barrier
         a.pc->rounds_done = r;
         a.pc->converged = conv;
         st_relaxed(&a.pc->tile_ctr[0], 0u);
@@ -839,7 +859,7 @@ __device__ __forceinline__ u32 edge_weight(const E &e)
 template <int ALGO, class E, int U>
 __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, const u64 *entryMask,
                                                    const u32 *chunkMap, const E *edges, u32 *dist, u64 *maskOut,
-                                                   u64 count, u64 W, u64 w0, u64 wend, u64 shift)
+                                                   u64 count, u64 W, u64 w0, u64 wend, u64 shift, u32 *err)
 {
     __shared__ u32 b_pre[BATCH_CHUNK + 1];
     __shared__ u32 b_loc[BATCH_CHUNK + 1];
@@ -851,7 +871,8 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
     const u64 cend = (cfirst + BATCH_CHUNK) < wend ? (cfirst + BATCH_CHUNK) : wend;
     const u64 k0 = chunkMap[j];
     const u64 k1 = (cfirst + BATCH_CHUNK < W) ? chunkMap[j + 1] : count - 1;
-    const int nk = (int)(k1 - k0 + 1);
+    const bool bad = k0 > k1 || k1 >= count || k1 - k0 > (u64)BATCH_CHUNK;
+    const int nk = bad ? 0 : (int)(k1 - k0 + 1);
     for (int t = tid; t < nk; t += ADV_BLOCK) {
         uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
         b_loc[t] = e.x;
@@ -859,7 +880,10 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
         b_id[t] = e.z;
         b_msk[t] = entryMask[k0 + t];
     }
-    __syncthreads();
+    if (__syncthreads_or(bad)) {
+        if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
+        return 0;
+    }
     constexpr int NW = ADV_BLOCK / WAVE;
     const u64 span = cend > cbeg ? (cend - cbeg + NW - 1) / NW : 0;
     const u64 q0 = cbeg + (u64)wid * span;
@@ -955,6 +979,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
     if (tid == 0) {
         s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         s_epoch = ld_relaxed(&a.sc->epoch);
+        atomicAdd(&a.pc->started, 1u);
     }
     __syncthreads();
     u32 gen = s_gen;
@@ -978,7 +1003,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
                                                                wt))
                 break;
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
@@ -997,14 +1022,15 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk_batch<ALGO, E, U>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
-                                                       a.dist, labOut, count, W, 0, W, 0);
+                                                       a.dist, labOut, count, W, 0, W, 0, wt.err);
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
     if (blockIdx.x == 0 && tid == 0) {
         st_relaxed(&a.sc->epoch, epoch0 + r + conv);
+        st_relaxed(&a.pc->started, 0u);   // every workgroup has passed the last barrier
         a.pc->rounds_done = r;
         a.pc->converged = conv;
         st_relaxed(&a.pc->tile_ctr[0], 0u);
@@ -1012,6 +1038,30 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
         st_relaxed(&a.pc->chunk_ctr[0], 0u);
         st_relaxed(&a.pc->chunk_ctr[1], 0u);
     }
+}
+
+// Standalone batched advance over the work window [w0, w1) (a vertex larger
+// than a buffer, streamed chunk by chunk).
+template <int ALGO, class E>
+__global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *entries, const u64 *entryMask,
+                                                                  const u32 *chunkMap, const E *edges, u32 *dist,
+                                                                  u64 *maskOut, u64 w0, u64 w1, u64 shift,
+                                                                  ScanCtl *ctl)
+{
+    // a frontier scan whose wait gave up left no usable entries
+    if (ld_relaxed(&ctl->error)) return;
+    const u64 count = ctl->count;
+    const u64 W = ctl->total;
+    const u64 wend = W < w1 ? W : w1;
+    if (count == 0 || wend <= w0) return;
+    const u64 j0 = w0 / BATCH_CHUNK;
+    const u64 j1 = (wend + BATCH_CHUNK - 1) / BATCH_CHUNK;
+    u32 updates = 0;
+    for (u64 j = j0 + blockIdx.x; j < j1; j += gridDim.x)
+        updates += advance_chunk_batch<ALGO, E, 4>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
+                                                   w0, wend, shift, &ctl->error);
+    u32 wsum = wave_reduce(updates);
+    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
 }
 
 // Batched initial state: every distance INF (SSWP: 0), masks cleared
@@ -1286,6 +1336,7 @@ const char *wait_error_name(uint32_t code)
         case 0: return "none";
         case WAIT_LOOKBACK: return "scan lookback";
         case WAIT_BARRIER: return "grid barrier";
+        case WAIT_DATA: return "inconsistent frontier data";
         default: return "unknown wait";
     }
 }
@@ -1469,6 +1520,16 @@ void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st)
     SUBWAY_CHECK(hipGetLastError());
 }
 
+// Grid of a cooperative kernel: every workgroup must be resident at once,
+// and workgroups are placed round-robin over the XCDs, so a kernel of the
+// copy stream that is resident when the grid starts (a gather) can hold the
+// slots a full-occupancy grid needs on its XCD for as long as the grid waits
+// at its first barrier.  One workgroup slot per CU is left free.
+static int coresident_grid(int perCU, int cus)
+{
+    return (perCU > 1 ? perCU - 1 : perCU) * cus;
+}
+
 template <int ALGO, class E>
 static const void *rounds_fn()
 {
@@ -1501,7 +1562,7 @@ int rounds_max_grid(int algo, int edgeBytes)
     hipDeviceProp_t prop;
     SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
     if (!prop.cooperativeLaunch) return 0;
-    return perCU * prop.multiProcessorCount;
+    return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
 void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st)
@@ -1573,6 +1634,35 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
     SUBWAY_CHECK(hipGetLastError());
 }
 
+template <int ALGO, class E>
+static void adv_batch(const Entry *entries, const ull *entryMask, const uint32_t *chunkMap, const void *edges,
+                      uint32_t *dist, ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)grid), dim3(ADV_BLOCK), 0, st, entries,
+                       entryMask, chunkMap, (const E *)edges, dist, maskOut, (u64)w0, (u64)w1, (u64)shift, ctl);
+}
+
+void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
+                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
+                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+{
+    if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
+    if (algo == ALGO_SSSP && edgeBytes == 8)
+        adv_batch<ALGO_SSSP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                              grid, st);
+    else if (algo == ALGO_SSWP && edgeBytes == 8)
+        adv_batch<ALGO_SSWP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                              grid, st);
+    else if (algo == ALGO_BFS && edgeBytes == 8)
+        adv_batch<ALGO_BFS, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
+                                             grid, st);
+    else if (algo == ALGO_BFS && edgeBytes == 4)
+        adv_batch<ALGO_BFS, OutEdge>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl, grid, st);
+    else
+        throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP (SSSP/SSWP on weighted edges)");
+    SUBWAY_CHECK(hipGetLastError());
+}
+
 template <int ALGO, class E, int U>
 static const void *batch_fn()
 {
@@ -1616,7 +1706,7 @@ int batch_max_grid(int algo, int edgeBytes, int unroll)
     hipDeviceProp_t prop;
     SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
     if (!prop.cooperativeLaunch) return 0;
-    return perCU * prop.multiProcessorCount;
+    return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
 void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st)

@@ -158,7 +158,8 @@ struct RoundsLaunch {
     uint32_t roundBase;
     uint32_t idBase;       // nodes == null: a whole id range starting here
 };
-// Largest co-resident grid for the rounds kernel (0: no cooperative launch).
+// Largest co-resident grid for the rounds kernel, one workgroup slot per CU
+// below full occupancy (0: no cooperative launch).
 int rounds_max_grid(int algo, int edgeBytes);
 void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st);
 
@@ -189,6 +190,11 @@ int batch_max_grid(int algo, int edgeBytes, int unroll);
 void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
                            Entry *entries, ull *entryMask, uint32_t *chunkMap, ScanScratch &s, hipStream_t st,
                            uint32_t idBase = 0);
+// Batched advance over the work window [w0, w1) of the last frontier (a
+// vertex larger than a buffer, streamed chunk by chunk).
+void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
+                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
+                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
 void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
 
 // Initial vertex state (values and both label arrays / PR value and delta).

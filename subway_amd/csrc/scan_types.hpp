@@ -50,6 +50,11 @@ struct PersistCtl {
     u32 rounds_done;   // rounds that selected work in the last launch
     u32 converged;     // 1: the last launch ended on a round that selected nothing
     u32 error;         // nonzero: a bounded device wait gave up (WAIT_* code)
+    u32 started;       // workgroups of the running launch that have started
+    // first barrier wait that gave up: round << 1 | phase (0 frontier, 1
+    // advance), workgroups started, arrivals counted at its group and top
+    // counters, grid size
+    u32 diag[5];
 };
 
 // Every device-side wait (lookback, grid barrier) polls at most a bounded
@@ -57,6 +62,7 @@ struct PersistCtl {
 // of its control block and its kernel runs to its end without further waits.
 constexpr u32 WAIT_LOOKBACK = 1;
 constexpr u32 WAIT_BARRIER = 2;
+constexpr u32 WAIT_DATA = 3;       // a frontier consumer found inconsistent entries / chunk map
 constexpr u32 SPIN_LIMIT_DEFAULT = 1u << 22;   // polls (each a device-scope load + s_sleep)
 
 struct TileDesc {

@@ -160,9 +160,6 @@ def test_pr_golden_readme(sw, gpu, mode):
 @pytest.mark.parametrize("budget", [0, 64 << 10])
 @pytest.mark.parametrize("transfer", ["auto", "zerocopy", "host"])
 def test_apps_random(sw, gpu, algo, mode, budget, transfer):
-    if (algo, mode, budget, transfer) == ("sssp", "async", 64 << 10, "auto"):
-        pytest.skip("hung on the GPU (test time-out) with no cause found by reading the code; kept off the GPU "
-                    "until one is shown (every device wait is now bounded and reports an error instead)")
     rng = np.random.default_rng(zlib.crc32(f"{algo}-{mode}-{budget}-{transfer}".encode()))
     n = 20_000
     weighted = algo in ("sssp", "sswp")
@@ -277,11 +274,11 @@ def test_errors_are_reported(sw, gpu):
 
 # ------------------------------------------------------ batched sources
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "sswp"])
-@pytest.mark.parametrize("budget", [0, 64 << 10])
+@pytest.mark.parametrize("budget", [0, 64 << 10, 32 << 10])
 @pytest.mark.parametrize("k", [1, 5, 64])
 def test_batch_matches_single_source(sw, gpu, algo, budget, k):
-    """k sources advanced together equal k single-source oracle runs (64 KiB:
-    many small partitions; the hub vertex still fits one buffer)."""
+    """k sources advanced together equal k single-source oracle runs; the
+    32 KiB budget makes the hub vertex stream in chunks."""
     g = sw.Graph.rmat(13, 16, seed=17, weighted=True)
     ptr, adj, w = g.csr()
     deg = np.diff(ptr)
@@ -294,16 +291,6 @@ def test_batch_matches_single_source(sw, gpu, algo, budget, k):
     for i, s in enumerate(sources):
         exp = O.expected(algo, ptr, adj, w, int(s))
         assert np.array_equal(vals[i], exp), (algo, budget, k, i, int(np.sum(vals[i] != exp)))
-
-
-def test_batch_rejects_vertex_larger_than_buffer(sw, gpu):
-    """Batched runs need each vertex's edges in one buffer; at 32 KiB the hub
-    vertex of this graph does not fit and the session refuses to start (the
-    chunked batched advance is switched off: it timed out at this budget)."""
-    g = sw.Graph.rmat(13, 16, seed=17, weighted=True)
-    assert int(np.diff(g.csr()[0]).max()) > (32 << 10) // 2 // 8
-    with pytest.raises(sw.SubwayError, match="max out-degree"):
-        sw.BatchSession(g, "sssp", budget=32 << 10)
 
 
 def test_batch_tuner_and_repeats(sw, gpu):
