@@ -39,6 +39,22 @@ def parse_size(s):
     return int(float(s[:-1]) * mul[s[-1].upper()]) if s[-1].upper() in mul else int(s)
 
 
+def profiled_traffic(kernel, workload_key):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by the copy-kernel calibration it records), or None."""
+    path = os.path.join(ROOT, "profiles", "sssp_async_rmat24_hbm_counters.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if parts[:2] == ["traffic_bytes_per_launch", kernel] and " ".join(parts[2:-1]) == workload_key:
+                    return int(parts[-1])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,12 +130,12 @@ def main():
 
     # Roofline of the dominant kernel.  Persistent async rounds: one
     # cooperative rounds_kernel launch per partition visit does every inner
-    # round (frontier scan + advance).  Algorithmic bytes of a launch:
-    #   per scanned partition item  5 B  (vertex id 4 + label 1)
-    #   per frontier entry         29 B  (offset 8 + value 4 + label clear 1 + entry 16)
-    #   per relaxed edge      s(E) + 4 B (edge + destination value)
-    #   per successful update       9 B  (atomic read + write 8 + label 1)
-    # Launch-per-round mode: the advance_kernel, edges and updates only.
+    # round (frontier + advance).  Algorithmic bytes, summed over the rounds
+    # of a launch: 22 B per labelled active vertex (id 4 + offset 8 + degree
+    # 4 + label r/w 2 + value 4), s(E) + 4 B per relaxed edge (edge +
+    # destination value), 9 B per successful update (atomic RMW 8 + label 1).
+    # The frontier pass over unlabelled items is not counted.  Launch-per-
+    # round mode: the advance_kernel, edges and updates only.
     rounds_ms = sum(s.ms_rounds for s in stats)
     rounds_launches = sum(s.rounds_launches for s in stats)
     work = sum(s.work_edges for s in stats)
@@ -128,12 +144,16 @@ def main():
     ents = sum(s.work_entries for s in stats)
     if rounds_launches and rounds_ms > 0:
         kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
-        alg_bytes = 5 * items + 29 * ents + (edge_bytes + 4) * work + 9 * upd
+        alg_bytes = 22 * ents + (edge_bytes + 4) * work + 9 * upd
     else:
         kern, k_ms = "advance_kernel", sum(s.ms_advance for s in stats)
         k_launches = sum(s.advance_launches for s in stats)
         alg_bytes = (edge_bytes + 4) * work + 9 * upd
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    traffic = None
+    if (kern == "rounds_kernel" and args.algo == "sssp" and args.mode == "async" and args.scale == 24
+            and args.edge_factor == 16 and budget == 1 << 30 and args.buffers == 2):
+        traffic = profiled_traffic(kern, "sssp-async rmat24 budget1GiB")
     adv_ms = sum(s.ms_advance for s in stats)
     h2d = float(np.mean([s.h2d_bytes for s in stats]))
     xfer_ms = float(np.mean([s.ms_transfer for s in stats]))
@@ -193,7 +213,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "launches": k_launches,
             "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
             "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
