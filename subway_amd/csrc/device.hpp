@@ -52,6 +52,24 @@ __device__ __forceinline__ u32 fetch_add_relaxed(u32 *p, u32 v)
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Memory policy of data exchanged between workgroups inside one launch.
+// COH: agent-scope relaxed atomic loads and stores, performed at the device's
+// coherence point (the XCDs' L2 caches are not coherent with each other), so
+// a grid barrier needs only completion waits, no L2 write-back or
+// invalidate.  Without COH: plain accesses, ordered by kernel boundaries.
+template <bool COH, class T>
+__device__ __forceinline__ T mld(const T *p)
+{
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool COH, class T>
+__device__ __forceinline__ void mst(T *p, T v)
+{
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
 // ------------------------------------------------------------- wave scans
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

@@ -202,8 +202,8 @@ struct GenLabelOp {
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
         activeNodesPointer[cnt] = sum;
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
     }
 };
 
@@ -257,8 +257,8 @@ struct GenPrOp {
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
         activeNodesPointer[cnt] = sum;
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
     }
 };
 
@@ -268,20 +268,45 @@ struct FrItem {
     u32 val;
 };
 
-template <int CHUNK = ADV_CHUNK>
+template <int CHUNK, bool COH>
 __device__ __forceinline__ void write_entry(Entry *entries, u32 *chunkMap, u64 k, u64 pre,
                                             u32 loc, u32 val, u64 deg)
 {
-    uint4 e = make_uint4(loc, (u32)pre, val, 0u);
-    *reinterpret_cast<uint4 *>(entries + k) = e;
+    if constexpr (COH) {
+        u64 *q = reinterpret_cast<u64 *>(entries + k);
+        mst<true>(q, (u64)loc | ((u64)(u32)pre << 32));
+        mst<true>(q + 1, (u64)val);
+    } else {
+        *reinterpret_cast<uint4 *>(entries + k) = make_uint4(loc, (u32)pre, val, 0u);
+    }
     // chunk j starts inside [pre, pre + deg): record the entry that holds it
     u64 j = (pre + CHUNK - 1) / CHUNK;
     u64 jend = (pre + deg - 1) / CHUNK;
-    for (; j <= jend; j++) chunkMap[j] = (u32)k;
+    for (; j <= jend; j++) mst<COH>(&chunkMap[j], (u32)k);
+}
+
+// Entry fields (loc, prefix, val) of entries[k].
+template <bool COH>
+__device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc, u32 &pre, u32 &val)
+{
+    if constexpr (COH) {
+        const u64 *q = reinterpret_cast<const u64 *>(entries + k);
+        const u64 a = mld<true>(q), b = mld<true>(q + 1);
+        loc = (u32)a;
+        pre = (u32)(a >> 32);
+        val = (u32)b;
+    } else {
+        const uint4 e = *reinterpret_cast<const uint4 *>(entries + k);
+        loc = e.x;
+        pre = e.y;
+        val = e.z;
+    }
 }
 
 // Labelled vertices of the partition push their current value; their labels
-// are cleared as the reference does before its edge loop.
+// are cleared as the reference does before its edge loop.  COH: the labels
+// and values are exchanged with other workgroups of a persistent launch.
+template <bool COH>
 struct FrontierLabelOp {
     typedef FrItem Item;
     const u32 *nodes;
@@ -300,17 +325,17 @@ struct FrontierLabelOp {
             u64 i = i0 + j;
             if (i < n) {
                 u32 id = nodes ? nodes[i] : idBase + (u32)i;
-                uint8_t f = labA[id];
-                if (labB) f |= labB[id];
+                uint8_t f = mld<COH>(&labA[id]);
+                if (labB) f |= mld<COH>(&labB[id]);
                 if (f) {
-                    labA[id] = 0;
-                    if (labB) labB[id] = 0;
+                    mst<COH>(&labA[id], (uint8_t)0);
+                    if (labB) mst<COH>(&labB[id], (uint8_t)0);
                     u64 p0 = ptr[i], p1 = ptr[i + 1];
                     if (p1 > p0) {
                         mask |= 1u << j;
                         w[j] = p1 - p0;
                         it[j].loc = (u32)(p0 - fromEdge);
-                        it[j].val = value[id];
+                        it[j].val = mld<COH>(&value[id]);
                     }
                 }
             }
@@ -318,12 +343,12 @@ struct FrontierLabelOp {
     }
     __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
     {
-        write_entry(entries, chunkMap, k, pre, it.loc, it.val, deg);
+        write_entry<ADV_CHUNK, COH>(entries, chunkMap, k, pre, it.loc, it.val, deg);
     }
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
         ctl->stop = (cnt == 0) ? 1u : 0u;
         if (cnt) ctl->rounds += 1;
         ctl->entry_sum += cnt;
@@ -333,6 +358,7 @@ struct FrontierLabelOp {
 
 // PageRank: delta > acc selects; value absorbs delta and the share is
 // (float)((d / deg) * 0.85) as in shared/gpu_kernels.cu:233-239.
+template <bool COH>
 struct FrontierPrOp {
     typedef FrItem Item;
     const u32 *nodes;
@@ -352,10 +378,10 @@ struct FrontierPrOp {
             u64 i = i0 + j;
             if (i < n) {
                 u32 id = nodes ? nodes[i] : idBase + (u32)i;
-                float d = delta[id];
+                float d = mld<COH>(&delta[id]);
                 if (d > acc) {
-                    value[id] += d;
-                    delta[id] = 0.0f;
+                    mst<COH>(&value[id], mld<COH>(&value[id]) + d);
+                    mst<COH>(&delta[id], 0.0f);
                     u64 p0 = ptr[i], p1 = ptr[i + 1];
                     if (p1 > p0) {
                         float share = (float)((double)(d / (float)(p1 - p0)) * 0.85);
@@ -370,12 +396,12 @@ struct FrontierPrOp {
     }
     __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
     {
-        write_entry(entries, chunkMap, k, pre, it.loc, it.val, deg);
+        write_entry<ADV_CHUNK, COH>(entries, chunkMap, k, pre, it.loc, it.val, deg);
     }
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
         ctl->stop = (cnt == 0) ? 1u : 0u;
         if (cnt) ctl->rounds += 1;
         ctl->entry_sum += cnt;
@@ -414,8 +440,8 @@ struct GenMaskOp {
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
         activeNodesPointer[cnt] = sum;
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
     }
 };
 
@@ -427,6 +453,7 @@ struct FrMaskItem {
 
 // A vertex with any source bit set pushes for those sources; its masks are
 // cleared (the batched form of the label test/clear).
+template <bool COH>
 struct FrontierMaskOp {
     typedef FrMaskItem Item;
     const u32 *nodes;
@@ -445,11 +472,11 @@ struct FrontierMaskOp {
             u64 i = i0 + j;
             if (i < n) {
                 u32 id = nodes ? nodes[i] : idBase + (u32)i;
-                u64 m = labA[id];
-                if (labB) m |= labB[id];
+                u64 m = mld<COH>(&labA[id]);
+                if (labB) m |= mld<COH>(&labB[id]);
                 if (m) {
-                    labA[id] = 0;
-                    if (labB) labB[id] = 0;
+                    mst<COH>(&labA[id], 0ull);
+                    if (labB) mst<COH>(&labB[id], 0ull);
                     u64 p0 = ptr[i], p1 = ptr[i + 1];
                     if (p1 > p0) {
                         mask |= 1u << j;
@@ -464,13 +491,13 @@ struct FrontierMaskOp {
     }
     __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
     {
-        write_entry<BATCH_CHUNK>(entries, chunkMap, k, pre, it.loc, it.id, deg);
-        entryMask[k] = it.m;
+        write_entry<BATCH_CHUNK, COH>(entries, chunkMap, k, pre, it.loc, it.id, deg);
+        mst<COH>(&entryMask[k], it.m);
     }
     __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *ctl) const
     {
-        ctl->count = cnt;
-        ctl->total = sum;
+        st_relaxed(&ctl->count, cnt);
+        st_relaxed(&ctl->total, sum);
         ctl->stop = (cnt == 0) ? 1u : 0u;
         if (cnt) ctl->rounds += 1;
         ctl->entry_sum += cnt;
@@ -485,7 +512,7 @@ struct Relax;
 // cand = value[u] + 1, keep min   (shared/gpu_kernels.cu:42-56)
 template <>
 struct Relax<ALGO_BFS> {
-    template <class E>
+    template <bool COH, class E>
     __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, float *)
     {
         u32 cand = val + 1;
@@ -493,7 +520,7 @@ struct Relax<ALGO_BFS> {
         if (cand < value[v]) {
             u32 prev = atomicMin(&value[v], cand);
             if (cand < prev) {
-                lab[v] = 1;
+                mst<COH>(&lab[v], (uint8_t)1);
                 return true;
             }
         }
@@ -503,14 +530,14 @@ struct Relax<ALGO_BFS> {
 // cand = value[u], keep min       (shared/gpu_kernels.cu:90-106)
 template <>
 struct Relax<ALGO_CC> {
-    template <class E>
+    template <bool COH, class E>
     __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, float *)
     {
         u32 v = e.end;
         if (val < value[v]) {
             u32 prev = atomicMin(&value[v], val);
             if (val < prev) {
-                lab[v] = 1;
+                mst<COH>(&lab[v], (uint8_t)1);
                 return true;
             }
         }
@@ -520,6 +547,7 @@ struct Relax<ALGO_CC> {
 // cand = value[u] + w, keep min   (shared/gpu_kernels.cu:143-157)
 template <>
 struct Relax<ALGO_SSSP> {
+    template <bool COH>
     __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
                                                  uint8_t *lab, float *)
     {
@@ -528,7 +556,7 @@ struct Relax<ALGO_SSSP> {
         if (cand < value[v]) {
             u32 prev = atomicMin(&value[v], cand);
             if (cand < prev) {
-                lab[v] = 1;
+                mst<COH>(&lab[v], (uint8_t)1);
                 return true;
             }
         }
@@ -538,6 +566,7 @@ struct Relax<ALGO_SSSP> {
 // cand = min(value[u], w), keep max (shared/gpu_kernels.cu:193-207)
 template <>
 struct Relax<ALGO_SSWP> {
+    template <bool COH>
     __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
                                                  uint8_t *lab, float *)
     {
@@ -546,7 +575,7 @@ struct Relax<ALGO_SSWP> {
         if (cand > value[v]) {
             u32 prev = atomicMax(&value[v], cand);
             if (cand > prev) {
-                lab[v] = 1;
+                mst<COH>(&lab[v], (uint8_t)1);
                 return true;
             }
         }
@@ -556,7 +585,7 @@ struct Relax<ALGO_SSWP> {
 // delta[v] += share             (shared/gpu_kernels.cu:241-244)
 template <>
 struct Relax<ALGO_PR> {
-    template <class E>
+    template <bool COH, class E>
     __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *, uint8_t *, float *delta)
     {
         unsafeAtomicAdd(&delta[e.end], __uint_as_float(val));
@@ -568,7 +597,7 @@ struct Relax<ALGO_PR> {
 // [w0, wend): the chunk's entries are staged in LDS and every position finds
 // its entry by binary search.  Returns the successful relaxations of the
 // calling thread.
-template <int ALGO, class E>
+template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, float *delta, u64 count, u64 W, u64 w0,
                                              u64 wend, u64 shift, u32 *err)
@@ -580,19 +609,14 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     const u64 cbeg = j * ADV_CHUNK;
     const u64 pbeg = cbeg > w0 ? cbeg : w0;
     const u64 pend = (cbeg + ADV_CHUNK) < wend ? (cbeg + ADV_CHUNK) : wend;
-    const u64 k0 = chunkMap[j];
-    const u64 k1 = (cbeg + ADV_CHUNK < W) ? chunkMap[j + 1] : count - 1;
+    const u64 k0 = mld<COH>(&chunkMap[j]);
+    const u64 k1 = (cbeg + ADV_CHUNK < W) ? mld<COH>(&chunkMap[j + 1]) : count - 1;
     // the chunk's entries are [k0, k1], k1 < count, at most ADV_CHUNK + 1 of
     // them; anything else is frontier output that is not visible (reported,
     // never dereferenced)
     const bool bad = k0 > k1 || k1 >= count || k1 - k0 > (u64)ADV_CHUNK;
     const int nk = bad ? 0 : (int)(k1 - k0 + 1);
-    for (int t = tid; t < nk; t += ADV_BLOCK) {
-        uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
-        s_loc[t] = e.x;
-        s_pre[t] = e.y;
-        s_val[t] = e.z;
-    }
+    for (int t = tid; t < nk; t += ADV_BLOCK) read_entry<COH>(entries, k0 + t, s_loc[t], s_pre[t], s_val[t]);
     if (__syncthreads_or(bad)) {
         if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
         return 0;
@@ -608,7 +632,7 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
         }
         const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
         const E e = edges[eidx];
-        if (Relax<ALGO>::apply(s_val[lo], e, value, lab, delta)) updates++;
+        if (Relax<ALGO>::template apply<COH>(s_val[lo], e, value, lab, delta)) updates++;
     }
     __syncthreads();
     return updates;
@@ -638,7 +662,8 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
         const u64 c = s_chunk;
         __syncthreads();
         if (c >= nchunks) break;
-        updates += advance_chunk<ALGO, E>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0, wend,
+        updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
+                                                 wend,
                                           shift, &ctl->error);
     }
     u32 wsum = wave_reduce(updates);
@@ -654,19 +679,19 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 
 // ================================================== persistent rounds
 // Two-level grid barrier for a cooperative launch (every workgroup resident).
-// Thread 0 of each workgroup releases the workgroup's writes at agent scope,
-// arrives at its group counter, the last of a group arrives at the top
-// counter, and the last overall bumps the generation the others spin on.
+// Everything the workgroups of a persistent launch exchange is read and
+// written with agent-scope atomics (the COH memory policy), so the barrier
+// orders only their completion: every wave waits for its outstanding memory
+// operations, then thread 0 arrives at its group counter, the last of a
+// group arrives at the top counter, and the last overall bumps the
+// generation the others poll.  No L2 write-back or invalidate is needed
+// (with them a barrier of 768 workgroups costs about 60 us instead of 4).
 // The wait is bounded (`limit` polls, WAIT_BARRIER recorded on expiry).
 // Returns false on every workgroup once any wait of the launch has given up
 // (checked after the barrier, so all workgroups that pass one barrier agree).
 __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u32 where)
 {
     __shared__ u32 s_fail;
-    // __syncthreads() waits for LDS traffic only: every wave first waits for
-    // its own global stores to reach its XCD's L2, so that thread 0's
-    // agent-scope release (an L2 write-back) covers the whole workgroup's
-    // writes before it arrives
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -674,22 +699,21 @@ __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u
         const u32 ngroups = gridDim.x < 8u ? gridDim.x : 8u;
         const u32 grp = blockIdx.x % ngroups;
         const u32 grpSize = (gridDim.x - grp + ngroups - 1) / ngroups;
-        __threadfence();
         bool releaser = false;
-        u32 a = __hip_atomic_fetch_add(&pc->bar_sub[grp * 16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        u32 a = fetch_add_relaxed(&pc->bar_sub[grp * 16], 1u) + 1;
         if (a == grpSize) {
-            __hip_atomic_store(&pc->bar_sub[grp * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            u32 t = __hip_atomic_fetch_add(&pc->bar_top[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            st_relaxed(&pc->bar_sub[grp * 16], 0u);
+            u32 t = fetch_add_relaxed(&pc->bar_top[0], 1u) + 1;
             if (t == ngroups) {
-                __hip_atomic_store(&pc->bar_top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&pc->bar_gen[0], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                st_relaxed(&pc->bar_top[0], 0u);
+                st_relaxed(&pc->bar_gen[0], g + 1);
                 releaser = true;
             }
         }
         bool gaveUp = false;
         if (!releaser) {
             u32 polls = 0;
-            while (__hip_atomic_load(&pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+            while (ld_relaxed(&pc->bar_gen[0]) == g) {
                 ++polls;
                 if (polls > limit || ((polls & 63u) == 0 && ld_relaxed(&pc->error) != 0)) {
                     gaveUp = true;
@@ -699,7 +723,6 @@ __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u
             }
         }
         gen = g + 1;
-        __threadfence();
         if (gaveUp && atomicCAS(&pc->error, 0u, WAIT_BARRIER) == 0u) {
             pc->diag[0] = where;
             pc->diag[1] = ld_relaxed(&pc->started);
@@ -745,7 +768,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
     __shared__ u32 s_claim, s_gen, s_epoch;
     const int tid = threadIdx.x;
     if (tid == 0) {
-        s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_gen = ld_relaxed(&a.pc->bar_gen[0]);
         s_epoch = ld_relaxed(&a.sc->epoch);
         atomicAdd(&a.pc->started, 1u);
     }
@@ -768,15 +791,15 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             __syncthreads();
             if (t >= numTiles) break;
             if constexpr (ALGO == ALGO_PR) {
-                FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
+                FrontierPrOp<true> op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
                                 a.idBase};
-                if (!scan_tile<FrontierPrOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
+                if (!scan_tile<FrontierPrOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                  wt))
                     break;
             } else {
-                FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
+                FrontierLabelOp<true> op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
                                    a.entries, a.chunkMap, a.idBase};
-                if (!scan_tile<FrontierLabelOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td,
+                if (!scan_tile<FrontierLabelOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td,
                                                                     a.sc, wt))
                     break;
             }
@@ -802,7 +825,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             const u64 c = s_claim;
             __syncthreads();
             if (c >= nchunks) break;
-            updates += advance_chunk<ALGO, E>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
+            updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
                                               a.delta, count, W, 0, W, 0, wt.err);
         }
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
@@ -856,7 +879,7 @@ __device__ __forceinline__ u32 edge_weight(const E &e)
 // lane s relaxes source s when bit s of the pushing vertex's mask is set.
 // dist is vertex-major, 64 u32 per vertex, so a wave's row access is one
 // coalesced 256-byte transaction.
-template <int ALGO, class E, int U>
+template <int ALGO, class E, int U, bool COH>
 __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, const u64 *entryMask,
                                                    const u32 *chunkMap, const E *edges, u32 *dist, u64 *maskOut,
                                                    u64 count, u64 W, u64 w0, u64 wend, u64 shift, u32 *err)
@@ -869,16 +892,13 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
     const u64 cfirst = j * BATCH_CHUNK;
     const u64 cbeg = cfirst > w0 ? cfirst : w0;
     const u64 cend = (cfirst + BATCH_CHUNK) < wend ? (cfirst + BATCH_CHUNK) : wend;
-    const u64 k0 = chunkMap[j];
-    const u64 k1 = (cfirst + BATCH_CHUNK < W) ? chunkMap[j + 1] : count - 1;
+    const u64 k0 = mld<COH>(&chunkMap[j]);
+    const u64 k1 = (cfirst + BATCH_CHUNK < W) ? mld<COH>(&chunkMap[j + 1]) : count - 1;
     const bool bad = k0 > k1 || k1 >= count || k1 - k0 > (u64)BATCH_CHUNK;
     const int nk = bad ? 0 : (int)(k1 - k0 + 1);
     for (int t = tid; t < nk; t += ADV_BLOCK) {
-        uint4 e = *reinterpret_cast<const uint4 *>(entries + k0 + t);
-        b_loc[t] = e.x;
-        b_pre[t] = e.y;
-        b_id[t] = e.z;
-        b_msk[t] = entryMask[k0 + t];
+        read_entry<COH>(entries, k0 + t, b_loc[t], b_pre[t], b_id[t]);
+        b_msk[t] = mld<COH>(&entryMask[k0 + t]);
     }
     if (__syncthreads_or(bad)) {
         if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
@@ -913,7 +933,7 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
                     while (k + 1 < nk && b_pre[k + 1] <= (u32)pp) k++;
                     if (k != kcur) {
                         kcur = k;
-                        du = dist[(u64)b_id[k] * WAVE + lane];
+                        du = mld<COH>(&dist[(u64)b_id[k] * WAVE + lane]);
                     }
                     const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
                     const E e = edges[eidx];
@@ -977,7 +997,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
     __shared__ u32 s_claim, s_gen, s_epoch;
     const int tid = threadIdx.x;
     if (tid == 0) {
-        s_gen = __hip_atomic_load(&a.pc->bar_gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_gen = ld_relaxed(&a.pc->bar_gen[0]);
         s_epoch = ld_relaxed(&a.sc->epoch);
         atomicAdd(&a.pc->started, 1u);
     }
@@ -997,9 +1017,9 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             const u32 t = s_claim;
             __syncthreads();
             if (t >= numTiles) break;
-            FrontierMaskOp op{a.nodes, a.ptr, a.fromEdge, labIn, r == 0 ? a.m2 : nullptr, a.entries, a.entryMask,
+            FrontierMaskOp<true> op{a.nodes, a.ptr, a.fromEdge, labIn, r == 0 ? a.m2 : nullptr, a.entries, a.entryMask,
                               a.chunkMap, a.idBase};
-            if (!scan_tile<FrontierMaskOp, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
+            if (!scan_tile<FrontierMaskOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                wt))
                 break;
         }
@@ -1021,7 +1041,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             const u64 c = s_claim;
             __syncthreads();
             if (c >= nchunks) break;
-            updates += advance_chunk_batch<ALGO, E, U>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
+            updates += advance_chunk_batch<ALGO, E, U, true>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
                                                        a.dist, labOut, count, W, 0, W, 0, wt.err);
         }
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
@@ -1058,7 +1078,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *e
     const u64 j1 = (wend + BATCH_CHUNK - 1) / BATCH_CHUNK;
     u32 updates = 0;
     for (u64 j = j0 + blockIdx.x; j < j1; j += gridDim.x)
-        updates += advance_chunk_batch<ALGO, E, 4>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
+        updates += advance_chunk_batch<ALGO, E, 4, false>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
                                                    w0, wend, shift, &ctl->error);
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -1444,14 +1464,14 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
     u64 tiles = (a.n + FR_TILE - 1) / FR_TILE;
     check_tiles(s, tiles, st);
     if (a.algo == ALGO_PR) {
-        FrontierPrOp op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
+        FrontierPrOp<false> op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
                         a.idBase};
-        hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
+        hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                            dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first,
                            s.spinLimit);
     } else {
-        FrontierLabelOp op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap, a.idBase};
-        hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp, SCAN_BLOCK, FR_IPT>),
+        FrontierLabelOp<false> op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap, a.idBase};
+        hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp<false>, SCAN_BLOCK, FR_IPT>),
                            dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td,
                            a.first, s.spinLimit);
     }
@@ -1628,8 +1648,8 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
     if (n == 0) throw Error(ERR_ARG, "frontier over an empty partition");
     u64 tiles = (n + FR_TILE - 1) / FR_TILE;
     check_tiles(s, tiles, st);
-    FrontierMaskOp op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase};
-    hipLaunchKernelGGL((select_scan_kernel<FrontierMaskOp, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
+    FrontierMaskOp<false> op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase};
+    hipLaunchKernelGGL((select_scan_kernel<FrontierMaskOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                        dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());
 }
