@@ -107,14 +107,16 @@ def main():
         log(f"copy probe: {args.probe_copy_mb} MiB read + written in {ms:.3f} ms")
     peak = sw.h2d_peak(256 << 20, 10)
     log(f"pinned H2D peak {peak:.1f} GB/s")
+    # results stay in the session's host value array (graph.value); the
+    # timed steps read them in place
     for i in range(args.warmup):
-        _, st = sess.run(src)
+        _, st = sess.run(src, copy=False)
         log(f"warmup {i}: {st.ms_total:.1f} ms, {st.global_iterations} global iterations")
 
     stats = []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        vals, st = sess.run(src)
+        vals, st = sess.run(src, copy=False)
         stats.append(st)
         log(f"step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
             f"H2D {st.h2d_bytes / 1e9:.2f} GB, rounds {st.ms_rounds:.1f} ms ({st.rounds_launches} launches), "

@@ -51,6 +51,7 @@ struct RunOptions {
     int batchUnroll = 0;    // batched advance: edges in flight per wave (1, 2, 4, 8); 0 = 4
     float theta = 0.5f;     // learned policy without a model: whole-range density threshold
     uint32_t spinLimit = 0; // polls before a device wait gives up (the run then fails); 0 = default
+    int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
 };
 
 struct IterRecord {
@@ -95,6 +96,7 @@ public:
     virtual ~Session() = default;
     virtual RunStats run(uint source) = 0;   // the source is ignored by CC and PR
     virtual void values(void *out) = 0;      // result of the last run (u32 or float per vertex)
+    virtual const void *valuesPtr() = 0;     // the graph's host value array (last run's results)
     virtual ull numNodes() const = 0;
     // Batched sessions (RunOptions::batch): k <= 64 sources per run; rows()
     // copies the n x 64 vertex-major values of the last run (lane i = sources[i]).

@@ -16,6 +16,7 @@
 #define SUBWAY_GRAPH_HPP
 
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "globals.hpp"
@@ -35,8 +36,19 @@ struct HostCsr {
     uint maxDegree = 0;
     bool large = false;     // read from / destined for the 64-bit file variant
     bool pinned = false;    // edges live in pinned, device-mapped memory
+    // Weighted graphs: every edge as one 32-bit word, destination | weight <<
+    // packShift (pinned, device-mapped), built on first use when the vertex
+    // ids fit packShift bits and every weight the remaining 32 - packShift.
+    uint32_t *packed = nullptr;
+    uint32_t *d_packed = nullptr;   // device address of `packed`
+    uint32_t packShift = 0;         // 0: no packed copy (unweighted, or ids + weight exceed 32 bits)
+    uint32_t ensurePacked();        // builds the packed copy once; returns packShift
     HostCsr() = default;
     HostCsr(const HostCsr &) = delete;
+private:
+    std::mutex packMu_;
+    bool packTried_ = false;
+public:
     HostCsr &operator=(const HostCsr &) = delete;
     ~HostCsr();
     void allocate(ull n, ull m);   // allocates ptr, edges, degree

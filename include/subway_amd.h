@@ -68,7 +68,10 @@ typedef struct subway_options {
                                least theta move whole (default 0.5; > 1 never) */
     uint32_t spin_limit;    /* polls before a device-side wait (scan lookback, grid barrier) gives up and
                                the run fails with SUBWAY_ERR_INTERNAL; 0 = 4194304 */
-    int32_t reserved[6];
+    int32_t unpacked_edges; /* weighted graphs: 0 (default) moves and buffers each edge as one 32-bit word,
+                               destination | weight << ceil(log2 V), when every weight fits the remaining
+                               bits; 1 keeps 8-byte {end, w8} edges */
+    int32_t reserved[5];
 } subway_options;
 
 typedef struct subway_stats {
@@ -143,6 +146,11 @@ int32_t subway_run(const subway_graph *g, const subway_options *opt, void *value
 int32_t subway_session_create(const subway_graph *g, const subway_options *opt, subway_session **out);
 int32_t subway_session_run(subway_session *s, uint32_t source, void *values, subway_stats *stats,
                            subway_iter_record *trace, uint32_t trace_cap);
+/* The session graph's host value array (graph.value of the reference: u32,
+ * or float for PR, one per vertex), holding the last run's results; valid
+ * until the next run or the session's end.  Lets a caller read results
+ * without the copy `values` makes (pass values = NULL to run). */
+const void *subway_session_values(const subway_session *s);
 void subway_session_free(subway_session *s);
 /* Batched session (options.batch = 1): k <= 64 sources advance together in
  * one cooperative kernel per transferred partition.  rows: n x 64 uint32,

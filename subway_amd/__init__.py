@@ -48,7 +48,7 @@ class _Options(ctypes.Structure):
         ("verbose", ctypes.c_int32), ("advance_grid", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
-        ("spin_limit", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 6),
+        ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -195,7 +195,7 @@ ROUNDS = {"persistent": 0, "per-round": 1}
 
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0):
+             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -215,6 +215,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.max_rounds = max_rounds
     o.theta = theta
     o.spin_limit = spin_limit
+    o.unpacked_edges = 0 if packed else 1
     return o
 
 
@@ -236,21 +237,34 @@ class Session:
     def __init__(self, graph: Graph, algo: str, mode: str = "sync", budget: int = 0, buffers: int = 2,
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
-                 rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0):
+                 rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
+                 packed: bool = True):
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit)
+                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        self._rec = (_IterRecord * _TRACE_CAP)()
 
-    def run(self, source: int = 0):
-        vals = np.empty(self.graph.n, dtype=np.float32 if self.algo == "pr" else np.uint32)
+    def run(self, source: int = 0, copy: bool = True):
+        """One query.  Returns (values, RunStats).  With copy=False the values
+        are a read-only view of the session's host result array, valid until
+        the next run (no copy of V values)."""
         st = _Stats()
-        rec = (_IterRecord * _TRACE_CAP)()
-        check(lib.subway_session_run(self._h, source, vals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st),
-                                     rec, _TRACE_CAP))
-        return vals, _stats(st, rec)
+        dt = np.float32 if self.algo == "pr" else np.uint32
+        if copy:
+            vals = np.empty(self.graph.n, dtype=dt)
+            out = vals.ctypes.data_as(ctypes.c_void_p)
+        else:
+            out = None
+        check(lib.subway_session_run(self._h, source, out, ctypes.byref(st), self._rec, _TRACE_CAP))
+        if not copy:
+            ptr = lib.subway_session_values(self._h)
+            buf = (ctypes.c_uint8 * (self.graph.n * 4)).from_address(ptr)
+            vals = np.frombuffer(buf, dtype=dt)
+            vals.flags.writeable = False
+        return vals, _stats(st, self._rec)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -268,10 +282,10 @@ class BatchSession:
 
     def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",
                  unroll: int = 0, rounds_grid: int = 0, profile: bool = False, policy: str = "plain",
-                 theta: float = 0.5):
+                 theta: float = 0.5, packed: bool = True):
         self.graph, self.algo = graph, algo
         o = _options(algo, "async", 0, budget, buffers, 0.01, 0.15, transfer, policy, False, 0, profile,
-                     "persistent", rounds_grid, 0, theta)
+                     "persistent", rounds_grid, 0, theta, 0, packed)
         o.batch = 1
         o.batch_unroll = unroll
         h = ctypes.c_void_p()
@@ -308,7 +322,7 @@ class BatchSession:
 def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
-        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0):
+        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
@@ -316,7 +330,7 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     device memory, the reference's sizing).
     """
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit)
+                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

@@ -224,6 +224,7 @@ static RunOptions to_options(const subway_options *o)
     r.roundsGrid = o->rounds_grid;
     if (o->max_rounds > 0) r.maxRounds = o->max_rounds;
     r.spinLimit = o->spin_limit;
+    r.unpackedEdges = o->unpacked_edges;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -307,6 +308,8 @@ int32_t subway_session_run(subway_session *s, uint32_t source, void *values, sub
         fill_stats(r, stats, trace, trace_cap);
     });
 }
+
+const void *subway_session_values(const subway_session *s) { return s ? s->impl->valuesPtr() : nullptr; }
 
 void subway_session_free(subway_session *s) { delete s; }
 

@@ -149,6 +149,16 @@ public:
         SUBWAY_CHECK(hipEventCreate(&evX0_));
         SUBWAY_CHECK(hipEventCreate(&evX1_));
         maxPart_ = sub_->max_partition_size;
+        if constexpr (std::is_same<E, OutEdgeWeighted>::value) {
+            // weighted edges travel and sit in the buffers as one word each
+            // when ids and weights fit: twice the edges per buffer, half the
+            // bytes per transferred edge
+            if (!o_.unpackedEdges && (packShift_ = g_->csr->ensurePacked()) != 0) {
+                fmt_ = EDGE_PACKED;
+                eb_ = sizeof(uint32_t);
+                maxPart_ = std::min<ull>(maxPart_ * (sizeof(E) / sizeof(uint32_t)), 0x7FFFF800ull);
+            }
+        }
         alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
         ull maxEntries = std::min<ull>((ull)g_->num_nodes, maxPart_) + 1;
         SUBWAY_CHECK(hipMalloc(&entries_, maxEntries * sizeof(Entry)));
@@ -171,7 +181,7 @@ public:
             return;
         }
         if (o_.persistent) {
-            int mx = rounds_max_grid(o_.algo, (int)sizeof(E));
+            int mx = rounds_max_grid(o_.algo, fmt_);
             roundsGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
             if (roundsGrid_ > 0) {
                 SUBWAY_CHECK(hipMalloc(&pc_, sizeof(dev::PersistCtl)));
@@ -213,7 +223,7 @@ public:
         SUBWAY_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
-        const int mx = batch_max_grid(o_.algo, (int)sizeof(E), u);
+        const int mx = batch_max_grid(o_.algo, fmt_, u);
         if (mx <= 0) throw Error(ERR_ARG, "this device cannot run the cooperative batched rounds kernel");
         batchUnroll_ = u;
         batchGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
@@ -276,7 +286,10 @@ private:
     ScanCtl *h_ctl_ = nullptr;
     uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
     bool broken_ = false;         // a device wait gave up: the device state is no longer trusted
-    ull maxPart_ = 0;
+    ull maxPart_ = 0;       // edges per device buffer (in the buffer's edge format)
+    int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
+    int eb_ = (int)sizeof(E);           // bytes per transferred / buffered edge
+    uint32_t packShift_ = 0;            // EDGE_PACKED: destination bits
     int advGrid_ = 0;
     RunStats st_;
     std::vector<Unit> units_;
@@ -299,6 +312,11 @@ private:
     void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf);
     void asyncRounds(const Unit &un, int buf);
     void hostGather(const Unit &u, int b);
+    // host edge array in the transfer format (pinned)
+    const char *hostEdges() const
+    {
+        return fmt_ == EDGE_PACKED ? (const char *)g_->csr->packed : (const char *)g_->edgeList;
+    }
     // items of a compacted unit: a slice of the active list
     void setItems(Unit &u, ull fromNode, ull cnt, ull fromEdge)
     {
@@ -597,23 +615,24 @@ void Engine<E, G>::issue(size_t idx)
     SUBWAY_CHECK(hipStreamWaitEvent(sx_, evFree_[b], 0));
     if (u.count) {
         if (o_.profile) pXfer_.begin(sx_);
-        E *dst = sub_->d_buffers[b];
+        void *dst = sub_->d_buffers[b];
+        const size_t bytes = (size_t)u.count * (size_t)eb_;
         if (u.dma) {
-            SUBWAY_CHECK(hipMemcpyAsync(dst, g_->edgeList + u.hostEdge, (size_t)u.count * sizeof(E),
-                                        hipMemcpyHostToDevice, sx_));
-            st_.h2dDmaBytes += u.count * sizeof(E);
+            SUBWAY_CHECK(hipMemcpyAsync(dst, hostEdges() + u.hostEdge * (ull)eb_, bytes, hipMemcpyHostToDevice,
+                                        sx_));
+            st_.h2dDmaBytes += bytes;
         } else if (o_.transfer == TRANSFER_HOST) {
             hostGather(u, b);
-            SUBWAY_CHECK(hipMemcpyAsync(dst, sub_->stagingBuffer(b), (size_t)u.count * sizeof(E),
-                                        hipMemcpyHostToDevice, sx_));
+            SUBWAY_CHECK(hipMemcpyAsync(dst, sub_->stagingBuffer(b), bytes, hipMemcpyHostToDevice, sx_));
             SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
-            st_.h2dDmaBytes += u.count * sizeof(E);
+            st_.h2dDmaBytes += bytes;
         } else {
-            launch_gather((int)sizeof(E), sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend,
-                          g_->d_nodePointer, g_->d_edgeList, dst, u.edgeBase, u.count, sx_);
-            st_.h2dGatherBytes += u.count * sizeof(E);
+            const void *mapped = fmt_ == EDGE_PACKED ? (const void *)g_->csr->d_packed : (const void *)g_->d_edgeList;
+            launch_gather(eb_, sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend, g_->d_nodePointer,
+                          mapped, dst, u.edgeBase, u.count, sx_);
+            st_.h2dGatherBytes += bytes;
         }
-        st_.h2dBytes += u.count * sizeof(E);
+        st_.h2dBytes += bytes;
         if (o_.profile) pXfer_.end(sx_);
     }
     SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));
@@ -635,11 +654,12 @@ void Engine<E, G>::hostGather(const Unit &u, int b)
     }
     // the staging buffer is reused only after the copy out of it finished
     SUBWAY_CHECK(hipEventSynchronize(evStaged_[b]));
-    E *stage = sub_->stagingBuffer(b);
+    char *stage = (char *)sub_->stagingBuffer(b);
     const uint *ids = h_ids_ + u.kbeg;
     const ull cnt = u.kend - u.kbeg;
     const ull *np = g_->nodePointer;
-    const E *src = g_->edgeList;
+    const char *src = hostEdges();
+    const size_t esz = (size_t)eb_;
     const int T = std::max(1, std::min(host_threads(), 64));
     const ull nch = std::min<ull>(cnt, (ull)T * 8);
     std::vector<ull> chunkEdges(nch + 1, 0);
@@ -657,7 +677,7 @@ void Engine<E, G>::hostGather(const Unit &u, int b)
             ull v0 = cnt * c / nch, v1 = cnt * (c + 1) / nch, at = chunkEdges[c];
             for (ull i = v0; i < v1; i++) {
                 const ull a = np[ids[i]], d = np[ids[i] + 1] - a;
-                std::memcpy(stage + at, src + a, d * sizeof(E));
+                std::memcpy(stage + at * esz, src + a * esz, d * esz);
                 at += d;
             }
         }
@@ -696,7 +716,8 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf)
 {
     AdvanceArgs a{};
     a.algo = o_.algo;
-    a.edgeBytes = (int)sizeof(E);
+    a.edgeFmt = fmt_;
+    a.packShift = packShift_;
     a.entries = entries_;
     a.chunkMap = chunkMap_;
     a.edges = sub_->d_buffers[buf];
@@ -750,7 +771,8 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.maxRounds = (uint32_t)std::max(1, roundsCap_);
         l.roundBase = 0;
         if (o_.profile) pRounds_.begin(sc_);
-        launch_rounds(o_.algo, (int)sizeof(E), l, roundsGrid_, sc_);
+        l.packShift = packShift_;
+        launch_rounds(o_.algo, fmt_, l, roundsGrid_, sc_);
         if (o_.profile) pRounds_.end(sc_);
         return;
     }
@@ -822,7 +844,7 @@ void Engine<E, G>::computeBatch(const Unit &u, int b)
             launch_frontier_masks(u.n, u.nodes, u.ptr, u.fromEdge, d_m1_, d_m2_, entries_, d_entryMask_, chunkMap_,
                                   fr_, sc_, u.idBase);
         const ull w0 = u.chunk * maxPart_;
-        launch_advance_batch_window(o_.algo, (int)sizeof(E), entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
+        launch_advance_batch_window(o_.algo, fmt_, packShift_, entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
                                     d_dist_, d_m2_, w0, w0 + u.count, w0, fr_.ctl, advGrid_, sc_);
         return;
     }
@@ -843,7 +865,8 @@ void Engine<E, G>::computeBatch(const Unit &u, int b)
     l.pc = pc_;
     l.maxRounds = (uint32_t)std::max(1, roundsCap_);
     if (o_.profile) pRounds_.begin(sc_);
-    launch_rounds_batch(o_.algo, (int)sizeof(E), batchUnroll_, l, batchGrid_, sc_);
+    l.packShift = packShift_;
+    launch_rounds_batch(o_.algo, fmt_, batchUnroll_, l, batchGrid_, sc_);
     if (o_.profile) pRounds_.end(sc_);
 }
 
@@ -1009,8 +1032,13 @@ public:
     void values(void *out) override
     {
         G &g = eng_.graph();
-        std::memcpy(out, g.value, (size_t)g.num_nodes * (algo_ == PR ? sizeof(float) : sizeof(uint)));
+        const size_t esz = algo_ == PR ? sizeof(float) : sizeof(uint);
+        const char *src = (const char *)g.value;
+        parallel_for(g.num_nodes, [&](uint64_t b, uint64_t e, int) {
+            std::memcpy((char *)out + b * esz, src + b * esz, (e - b) * esz);
+        });
     }
+    const void *valuesPtr() override { return eng_.graph().value; }
     ull numNodes() const override { return const_cast<SessionImpl *>(this)->eng_.graph().num_nodes; }
     RunStats runBatch(const uint *sources, int k) override { return eng_.runBatch(sources, k); }
     void rows(uint *out) override

@@ -19,6 +19,7 @@
 #include <fstream>
 #include <iostream>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "subway/globals.hpp"
@@ -46,6 +47,45 @@ HostCsr<E>::~HostCsr()
     if (edges) {
         if (pinned) (void)hipHostFree(edges);
         else std::free(edges);
+    }
+    if (packed) (void)hipHostFree(packed);
+}
+
+template <class E>
+uint32_t HostCsr<E>::ensurePacked()
+{
+    if constexpr (!std::is_same<E, OutEdgeWeighted>::value) {
+        return 0;
+    } else {
+        std::lock_guard<std::mutex> lock(packMu_);
+        if (packTried_) return packShift;
+        packTried_ = true;
+        if (!pinned || m == 0) return 0;
+        uint32_t db = 1;
+        while (db < 32 && (1ull << db) < n) db++;   // ids 0 .. n-1 fit db bits
+        if (db >= 32) return 0;
+        const int T = host_threads();
+        std::vector<uint32_t> mx((size_t)T, 0);
+        parallel_for(m, [&](uint64_t b, uint64_t e, int t) {
+            uint32_t x = 0;
+            for (uint64_t i = b; i < e; i++) x = std::max(x, edges[i].w8);
+            mx[(size_t)t] = std::max(mx[(size_t)t], x);
+        }, T);
+        const uint32_t maxW = *std::max_element(mx.begin(), mx.end());
+        if ((uint64_t)maxW >> (32 - db)) return 0;   // a weight needs more than 32 - db bits
+        uint32_t *p = nullptr;
+        if (hipHostMalloc((void **)&p, (size_t)m * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocPortable) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            return 0;   // no room for the packed copy: transfers keep the 8-byte edges
+        }
+        parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+            for (uint64_t i = b; i < e; i++) p[i] = edges[i].end | (edges[i].w8 << db);
+        }, T);
+        packed = p;
+        SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_packed, packed, 0));
+        packShift = db;
+        return db;
     }
 }
 

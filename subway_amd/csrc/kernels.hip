@@ -593,6 +593,32 @@ struct Relax<ALGO_PR> {
     }
 };
 
+// Storage format -> logical edge of a partition buffer.
+template <class S>
+struct Codec;
+template <>
+struct Codec<OutEdge> {
+    typedef OutEdge T;
+    __device__ __forceinline__ static T get(const OutEdge *p, u64 i, u32) { return p[i]; }
+};
+template <>
+struct Codec<OutEdgeWeighted> {
+    typedef OutEdgeWeighted T;
+    __device__ __forceinline__ static T get(const OutEdgeWeighted *p, u64 i, u32) { return p[i]; }
+};
+template <>
+struct Codec<PackedEdge> {
+    typedef OutEdgeWeighted T;
+    __device__ __forceinline__ static T get(const PackedEdge *p, u64 i, u32 sh)
+    {
+        const u32 x = p[i].bits;
+        T e;
+        e.end = x & ((1u << sh) - 1u);
+        e.w8 = x >> sh;
+        return e;
+    }
+};
+
 // Relaxes the edges of one work chunk [cbeg, cbeg + ADV_CHUNK) clipped to
 // [w0, wend): the chunk's entries are staged in LDS and every position finds
 // its entry by binary search.  Returns the successful relaxations of the
@@ -600,7 +626,7 @@ struct Relax<ALGO_PR> {
 template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, float *delta, u64 count, u64 W, u64 w0,
-                                             u64 wend, u64 shift, u32 *err)
+                                             u64 wend, u64 shift, u32 psh, u32 *err)
 {
     __shared__ u32 s_pre[ADV_CHUNK + 1];
     __shared__ u32 s_loc[ADV_CHUNK + 1];
@@ -631,7 +657,7 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
             else hi = mid - 1;
         }
         const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-        const E e = edges[eidx];
+        const auto e = Codec<E>::get(edges, eidx, psh);
         if (Relax<ALGO>::template apply<COH>(s_val[lo], e, value, lab, delta)) updates++;
     }
     __syncthreads();
@@ -642,7 +668,7 @@ template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries, const u32 *chunkMap,
                                                             const E *edges, u32 *value, uint8_t *lab,
                                                             float *delta, u64 w0, u64 w1, u64 shift,
-                                                            ScanCtl *ctl)
+                                                            u32 psh, ScanCtl *ctl)
 {
     __shared__ u32 s_chunk;
     const int tid = threadIdx.x;
@@ -663,8 +689,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
         __syncthreads();
         if (c >= nchunks) break;
         updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
-                                                 wend,
-                                          shift, &ctl->error);
+                                                 wend, shift, psh, &ctl->error);
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -755,6 +780,7 @@ struct RoundsArgs {
     u32 roundBase;         // round index (within the partition) of this launch's first round
     u32 idBase;            // nodes == null: items are the id range idBase ..
     u32 spinLimit;         // poll limit of every wait of the launch
+    u32 packShift;         // EDGE_PACKED: destination bits
 };
 
 // All async inner rounds of one resident partition in one cooperative
@@ -826,7 +852,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
-                                              a.delta, count, W, 0, W, 0, wt.err);
+                                              a.delta, count, W, 0, W, 0, a.packShift, wt.err);
         }
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
     }
@@ -882,7 +908,7 @@ __device__ __forceinline__ u32 edge_weight(const E &e)
 template <int ALGO, class E, int U, bool COH>
 __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, const u64 *entryMask,
                                                    const u32 *chunkMap, const E *edges, u32 *dist, u64 *maskOut,
-                                                   u64 count, u64 W, u64 w0, u64 wend, u64 shift, u32 *err)
+                                                   u64 count, u64 W, u64 w0, u64 wend, u64 shift, u32 psh, u32 *err)
 {
     __shared__ u32 b_pre[BATCH_CHUNK + 1];
     __shared__ u32 b_loc[BATCH_CHUNK + 1];
@@ -936,7 +962,7 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
                         du = mld<COH>(&dist[(u64)b_id[k] * WAVE + lane]);
                     }
                     const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
-                    const E e = edges[eidx];
+                    const auto e = Codec<E>::get(edges, eidx, psh);
                     vv[u] = e.end;
                     ww[u] = edge_weight(e);
                     mm[u] = b_msk[k];
@@ -987,6 +1013,7 @@ struct BatchArgs {
     u32 maxRounds;
     u32 idBase;
     u32 spinLimit;
+    u32 packShift;
 };
 
 // All async rounds of one resident partition for up to 64 sources at once,
@@ -1042,7 +1069,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk_batch<ALGO, E, U, true>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
-                                                       a.dist, labOut, count, W, 0, W, 0, wt.err);
+                                                       a.dist, labOut, count, W, 0, W, 0, a.packShift, wt.err);
         }
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
     }
@@ -1066,7 +1093,7 @@ template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *entries, const u64 *entryMask,
                                                                   const u32 *chunkMap, const E *edges, u32 *dist,
                                                                   u64 *maskOut, u64 w0, u64 w1, u64 shift,
-                                                                  ScanCtl *ctl)
+                                                                  u32 psh, ScanCtl *ctl)
 {
     // a frontier scan whose wait gave up left no usable entries
     if (ld_relaxed(&ctl->error)) return;
@@ -1079,7 +1106,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *e
     u32 updates = 0;
     for (u64 j = j0 + blockIdx.x; j < j1; j += gridDim.x)
         updates += advance_chunk_batch<ALGO, E, 4, false>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
-                                                   w0, wend, shift, &ctl->error);
+                                                   w0, wend, shift, psh, &ctl->error);
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
 }
@@ -1483,33 +1510,34 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 {
     hipLaunchKernelGGL((advance_kernel<ALGO, E>), dim3((unsigned)a.grid), dim3(ADV_BLOCK), 0, st,
                        a.entries, a.chunkMap, (const E *)a.edges, a.value, a.labelOut, a.delta,
-                       (u64)a.w0, (u64)a.w1, (u64)a.shift, ctl);
+                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packShift, ctl);
 }
+
+// Launch `L<ALGO, storage>` for the buffer's edge format; SSSP and SSWP need
+// a weighted format.
+#define SUBWAY_BY_FORMAT(L, ALGO, fmt, ...)                                                            \
+    do {                                                                                               \
+        if ((fmt) == EDGE_PACKED) {                                                                    \
+            L<ALGO, PackedEdge>(__VA_ARGS__);                                                          \
+        } else if ((fmt) == EDGE_W64) {                                                                \
+            L<ALGO, OutEdgeWeighted>(__VA_ARGS__);                                                     \
+        } else if constexpr (ALGO != ALGO_SSSP && ALGO != ALGO_SSWP) {                                 \
+            if ((fmt) != EDGE_U32) throw Error(ERR_ARG, "unknown edge format");                        \
+            L<ALGO, OutEdge>(__VA_ARGS__);                                                             \
+        } else {                                                                                       \
+            throw Error(ERR_ARG, "this edge format does not carry the weights the algorithm needs");   \
+        }                                                                                              \
+    } while (0)
 
 void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 {
     if (a.grid <= 0) throw Error(ERR_ARG, "advance grid must be positive");
     switch (a.algo) {
-        case ALGO_BFS:
-            if (a.edgeBytes == 4) adv<ALGO_BFS, OutEdge>(a, ctl, st);
-            else adv<ALGO_BFS, OutEdgeWeighted>(a, ctl, st);
-            break;
-        case ALGO_CC:
-            if (a.edgeBytes == 4) adv<ALGO_CC, OutEdge>(a, ctl, st);
-            else adv<ALGO_CC, OutEdgeWeighted>(a, ctl, st);
-            break;
-        case ALGO_SSSP:
-            if (a.edgeBytes != 8) throw Error(ERR_ARG, "SSSP needs weighted edges");
-            adv<ALGO_SSSP, OutEdgeWeighted>(a, ctl, st);
-            break;
-        case ALGO_SSWP:
-            if (a.edgeBytes != 8) throw Error(ERR_ARG, "SSWP needs weighted edges");
-            adv<ALGO_SSWP, OutEdgeWeighted>(a, ctl, st);
-            break;
-        case ALGO_PR:
-            if (a.edgeBytes == 4) adv<ALGO_PR, OutEdge>(a, ctl, st);
-            else adv<ALGO_PR, OutEdgeWeighted>(a, ctl, st);
-            break;
+        case ALGO_BFS: SUBWAY_BY_FORMAT(adv, ALGO_BFS, a.edgeFmt, a, ctl, st); break;
+        case ALGO_CC: SUBWAY_BY_FORMAT(adv, ALGO_CC, a.edgeFmt, a, ctl, st); break;
+        case ALGO_SSSP: SUBWAY_BY_FORMAT(adv, ALGO_SSSP, a.edgeFmt, a, ctl, st); break;
+        case ALGO_SSWP: SUBWAY_BY_FORMAT(adv, ALGO_SSWP, a.edgeFmt, a, ctl, st); break;
+        case ALGO_PR: SUBWAY_BY_FORMAT(adv, ALGO_PR, a.edgeFmt, a, ctl, st); break;
         default: throw Error(ERR_ARG, "unknown algorithm");
     }
     SUBWAY_CHECK(hipGetLastError());
@@ -1551,30 +1579,28 @@ static int coresident_grid(int perCU, int cus)
 }
 
 template <int ALGO, class E>
-static const void *rounds_fn()
+static void rounds_fn(const void **f)
 {
-    return reinterpret_cast<const void *>(&rounds_kernel<ALGO, E>);
+    *f = reinterpret_cast<const void *>(&rounds_kernel<ALGO, E>);
 }
 
-static const void *rounds_fn_for(int algo, int edgeBytes)
+static const void *rounds_fn_for(int algo, int fmt)
 {
+    const void *f = nullptr;
     switch (algo) {
-        case ALGO_BFS: return edgeBytes == 4 ? rounds_fn<ALGO_BFS, OutEdge>() : rounds_fn<ALGO_BFS, OutEdgeWeighted>();
-        case ALGO_CC: return edgeBytes == 4 ? rounds_fn<ALGO_CC, OutEdge>() : rounds_fn<ALGO_CC, OutEdgeWeighted>();
-        case ALGO_SSSP:
-            if (edgeBytes != 8) throw Error(ERR_ARG, "SSSP needs weighted edges");
-            return rounds_fn<ALGO_SSSP, OutEdgeWeighted>();
-        case ALGO_SSWP:
-            if (edgeBytes != 8) throw Error(ERR_ARG, "SSWP needs weighted edges");
-            return rounds_fn<ALGO_SSWP, OutEdgeWeighted>();
-        case ALGO_PR: return edgeBytes == 4 ? rounds_fn<ALGO_PR, OutEdge>() : rounds_fn<ALGO_PR, OutEdgeWeighted>();
+        case ALGO_BFS: SUBWAY_BY_FORMAT(rounds_fn, ALGO_BFS, fmt, &f); break;
+        case ALGO_CC: SUBWAY_BY_FORMAT(rounds_fn, ALGO_CC, fmt, &f); break;
+        case ALGO_SSSP: SUBWAY_BY_FORMAT(rounds_fn, ALGO_SSSP, fmt, &f); break;
+        case ALGO_SSWP: SUBWAY_BY_FORMAT(rounds_fn, ALGO_SSWP, fmt, &f); break;
+        case ALGO_PR: SUBWAY_BY_FORMAT(rounds_fn, ALGO_PR, fmt, &f); break;
         default: throw Error(ERR_ARG, "unknown algorithm");
     }
+    return f;
 }
 
-int rounds_max_grid(int algo, int edgeBytes)
+int rounds_max_grid(int algo, int edgeFmt)
 {
-    const void *f = rounds_fn_for(algo, edgeBytes);
+    const void *f = rounds_fn_for(algo, edgeFmt);
     int perCU = 0;
     SUBWAY_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, f, ADV_BLOCK, 0));
     int dev = 0;
@@ -1585,7 +1611,7 @@ int rounds_max_grid(int algo, int edgeBytes)
     return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
-void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st)
+void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipStream_t st)
 {
     if (l.n == 0) throw Error(ERR_ARG, "rounds over an empty partition");
     if (grid <= 0) throw Error(ERR_ARG, "rounds grid must be positive");
@@ -1610,8 +1636,9 @@ void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hip
     a.maxRounds = l.maxRounds;
     a.roundBase = l.roundBase;
     a.idBase = l.idBase;
+    a.packShift = l.packShift;
     void *params[] = {&a};
-    SUBWAY_CHECK(hipLaunchCooperativeKernel(rounds_fn_for(algo, edgeBytes), dim3((unsigned)grid), dim3(ADV_BLOCK),
+    SUBWAY_CHECK(hipLaunchCooperativeKernel(rounds_fn_for(algo, edgeFmt), dim3((unsigned)grid), dim3(ADV_BLOCK),
                                             params, 0, st));
 }
 
@@ -1654,32 +1681,39 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
     SUBWAY_CHECK(hipGetLastError());
 }
 
+struct BatchWindow {
+    const Entry *entries;
+    const ull *entryMask;
+    const uint32_t *chunkMap;
+    const void *edges;
+    uint32_t *dist;
+    ull *maskOut;
+    ull w0, w1, shift;
+    uint32_t packShift;
+    ScanCtl *ctl;
+    int grid;
+};
+
 template <int ALGO, class E>
-static void adv_batch(const Entry *entries, const ull *entryMask, const uint32_t *chunkMap, const void *edges,
-                      uint32_t *dist, ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+static void adv_batch(const BatchWindow &w, hipStream_t st)
 {
-    hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)grid), dim3(ADV_BLOCK), 0, st, entries,
-                       entryMask, chunkMap, (const E *)edges, dist, maskOut, (u64)w0, (u64)w1, (u64)shift, ctl);
+    hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)w.grid), dim3(ADV_BLOCK), 0, st, w.entries,
+                       w.entryMask, w.chunkMap, (const E *)w.edges, w.dist, w.maskOut, (u64)w.w0, (u64)w.w1,
+                       (u64)w.shift, (u32)w.packShift, w.ctl);
 }
 
-void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
-                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
-                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
+void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packShift, const Entry *entries,
+                                 const ull *entryMask, const uint32_t *chunkMap, const void *edges, uint32_t *dist,
+                                 ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
 {
     if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
-    if (algo == ALGO_SSSP && edgeBytes == 8)
-        adv_batch<ALGO_SSSP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                              grid, st);
-    else if (algo == ALGO_SSWP && edgeBytes == 8)
-        adv_batch<ALGO_SSWP, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                              grid, st);
-    else if (algo == ALGO_BFS && edgeBytes == 8)
-        adv_batch<ALGO_BFS, OutEdgeWeighted>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl,
-                                             grid, st);
-    else if (algo == ALGO_BFS && edgeBytes == 4)
-        adv_batch<ALGO_BFS, OutEdge>(entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, ctl, grid, st);
-    else
-        throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP (SSSP/SSWP on weighted edges)");
+    const BatchWindow w{entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, packShift, ctl, grid};
+    switch (algo) {
+        case ALGO_BFS: SUBWAY_BY_FORMAT(adv_batch, ALGO_BFS, edgeFmt, w, st); break;
+        case ALGO_SSSP: SUBWAY_BY_FORMAT(adv_batch, ALGO_SSSP, edgeFmt, w, st); break;
+        case ALGO_SSWP: SUBWAY_BY_FORMAT(adv_batch, ALGO_SSWP, edgeFmt, w, st); break;
+        default: throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
+    }
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1690,35 +1724,32 @@ static const void *batch_fn()
 }
 
 template <int ALGO, class E>
-static const void *batch_fn_u(int unroll)
+static void batch_fn_u(const void **f, int unroll)
 {
     switch (unroll) {
-        case 1: return batch_fn<ALGO, E, 1>();
-        case 2: return batch_fn<ALGO, E, 2>();
-        case 4: return batch_fn<ALGO, E, 4>();
-        case 8: return batch_fn<ALGO, E, 8>();
+        case 1: *f = batch_fn<ALGO, E, 1>(); break;
+        case 2: *f = batch_fn<ALGO, E, 2>(); break;
+        case 4: *f = batch_fn<ALGO, E, 4>(); break;
+        case 8: *f = batch_fn<ALGO, E, 8>(); break;
         default: throw Error(ERR_ARG, "batched unroll must be 1, 2, 4 or 8");
     }
 }
 
-static const void *batch_fn_for(int algo, int edgeBytes, int unroll)
+static const void *batch_fn_for(int algo, int fmt, int unroll)
 {
+    const void *f = nullptr;
     switch (algo) {
-        case ALGO_BFS: return edgeBytes == 4 ? batch_fn_u<ALGO_BFS, OutEdge>(unroll)
-                                             : batch_fn_u<ALGO_BFS, OutEdgeWeighted>(unroll);
-        case ALGO_SSSP:
-            if (edgeBytes != 8) throw Error(ERR_ARG, "SSSP needs weighted edges");
-            return batch_fn_u<ALGO_SSSP, OutEdgeWeighted>(unroll);
-        case ALGO_SSWP:
-            if (edgeBytes != 8) throw Error(ERR_ARG, "SSWP needs weighted edges");
-            return batch_fn_u<ALGO_SSWP, OutEdgeWeighted>(unroll);
+        case ALGO_BFS: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_BFS, fmt, &f, unroll); break;
+        case ALGO_SSSP: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_SSSP, fmt, &f, unroll); break;
+        case ALGO_SSWP: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_SSWP, fmt, &f, unroll); break;
         default: throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
     }
+    return f;
 }
 
-int batch_max_grid(int algo, int edgeBytes, int unroll)
+int batch_max_grid(int algo, int edgeFmt, int unroll)
 {
-    const void *f = batch_fn_for(algo, edgeBytes, unroll);
+    const void *f = batch_fn_for(algo, edgeFmt, unroll);
     int perCU = 0;
     SUBWAY_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, f, ADV_BLOCK, 0));
     int dev = 0;
@@ -1729,7 +1760,7 @@ int batch_max_grid(int algo, int edgeBytes, int unroll)
     return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
-void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st)
+void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l, int grid, hipStream_t st)
 {
     if (l.n == 0) throw Error(ERR_ARG, "rounds over an empty partition");
     if (grid <= 0) throw Error(ERR_ARG, "rounds grid must be positive");
@@ -1752,8 +1783,9 @@ void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch 
     a.spinLimit = l.scan->spinLimit;
     a.maxRounds = l.maxRounds;
     a.idBase = l.idBase;
+    a.packShift = l.packShift;
     void *params[] = {&a};
-    SUBWAY_CHECK(hipLaunchCooperativeKernel(batch_fn_for(algo, edgeBytes, unroll), dim3((unsigned)grid),
+    SUBWAY_CHECK(hipLaunchCooperativeKernel(batch_fn_for(algo, edgeFmt, unroll), dim3((unsigned)grid),
                                             dim3(ADV_BLOCK), params, 0, st));
 }
 

@@ -34,6 +34,20 @@ constexpr int GATHER_CHUNK = 4096;
 // One worklist entry per selected vertex: where its edges sit in the device
 // buffer, where its edges start in work space, and the value it pushes
 // (uint32 value, or float bits of the PageRank share).
+// Device formats of the edges in a partition buffer.  The unpacked formats'
+// values are their sizes in bytes.  EDGE_PACKED stores a weighted edge in one
+// word, destination | weight << packShift, for graphs whose vertex ids fit
+// packShift bits and whose weights fit the remaining 32 - packShift bits.
+enum EdgeFormat : int {
+    EDGE_U32 = 4,      // OutEdge
+    EDGE_W64 = 8,      // OutEdgeWeighted
+    EDGE_PACKED = 5,   // PackedEdge
+};
+struct PackedEdge {
+    uint32_t bits;
+};
+inline int edge_format_bytes(int fmt) { return fmt == EDGE_W64 ? 8 : 4; }
+
 struct Entry {
     uint32_t edge_local;
     uint32_t prefix;
@@ -126,7 +140,8 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
 // ---- per-round advance (edge relaxation) kernel
 struct AdvanceArgs {
     int algo;
-    int edgeBytes;                   // 4 (OutEdge) or 8 (OutEdgeWeighted)
+    int edgeFmt;                     // EdgeFormat of the buffer
+    uint32_t packShift;              // EDGE_PACKED: destination bits
     const Entry *entries;
     const uint32_t *chunkMap;
     const void *edges;               // device edge buffer
@@ -157,11 +172,12 @@ struct RoundsLaunch {
     uint32_t maxRounds;
     uint32_t roundBase;
     uint32_t idBase;       // nodes == null: a whole id range starting here
+    uint32_t packShift;    // EDGE_PACKED: destination bits
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
 // below full occupancy (0: no cooperative launch).
-int rounds_max_grid(int algo, int edgeBytes);
-void launch_rounds(int algo, int edgeBytes, const RoundsLaunch &l, int grid, hipStream_t st);
+int rounds_max_grid(int algo, int edgeFmt);
+void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipStream_t st);
 
 // ---- batched k-source runs (k <= 64): dist[V][64], 64-bit masks
 constexpr int BATCH_LANES = 64;
@@ -185,17 +201,18 @@ struct BatchLaunch {
     dev::PersistCtl *pc;
     uint32_t maxRounds;
     uint32_t idBase;
+    uint32_t packShift;
 };
-int batch_max_grid(int algo, int edgeBytes, int unroll);
+int batch_max_grid(int algo, int edgeFmt, int unroll);
 void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
                            Entry *entries, ull *entryMask, uint32_t *chunkMap, ScanScratch &s, hipStream_t st,
                            uint32_t idBase = 0);
 // Batched advance over the work window [w0, w1) of the last frontier (a
 // vertex larger than a buffer, streamed chunk by chunk).
-void launch_advance_batch_window(int algo, int edgeBytes, const Entry *entries, const ull *entryMask,
-                                 const uint32_t *chunkMap, const void *edges, uint32_t *dist, ull *maskOut, ull w0,
-                                 ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
-void launch_rounds_batch(int algo, int edgeBytes, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
+void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packShift, const Entry *entries,
+                                 const ull *entryMask, const uint32_t *chunkMap, const void *edges, uint32_t *dist,
+                                 ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
+void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
 
 // Initial vertex state (values and both label arrays / PR value and delta).
 void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
