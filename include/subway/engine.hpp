@@ -63,6 +63,13 @@ struct IterRecord {
     ull rounds;
     ull updates;
     double msIteration;
+    // learned policy: the iteration's features and network heads, and the
+    // schedule applied (whole-range density threshold, inner-round cap)
+    float features[12] = {};
+    float heads[3] = {};
+    float theta = 0.f;
+    uint roundsCap = 0;
+    uint onPlain = 0;          // 1: the plain policy scheduled this iteration (fallback or no network)
 };
 
 struct RunStats {
@@ -79,6 +86,8 @@ struct RunStats {
     ull workEdges = 0;         // edges relaxed: sum of frontier work over all rounds
     ull workEntries = 0;       // vertices that pushed: sum over all rounds
     ull workItems = 0;         // partition items scanned: sum over all rounds
+    uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
+    uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     ull h2dBytes = 0;          // edge bytes moved host -> device
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads
@@ -106,6 +115,9 @@ public:
     // for each candidate wave assignment and keeps the fastest.  Returns the
     // chosen edges-in-flight per wave; `timesMs` (may be null) gets 4 times.
     virtual int tuneBatch(const uint *sources, int k, double *timesMs) = 0;
+    // Scheduler network parameters (SUBWAY_MLP_PARAMS floats) for the learned
+    // policy; null removes them (fixed theta / inner-round cap instead).
+    virtual void setPolicy(const float *params) = 0;
 };
 
 template <class E>

@@ -96,6 +96,8 @@ typedef struct subway_stats {
     uint32_t global_iterations;
     uint32_t trace_len;     /* records written to the trace array */
     uint64_t work_items;    /* partition items scanned by frontier passes over all rounds */
+    uint32_t mispredictions;       /* learned policy: iterations whose active edges were > 2x off the prediction */
+    uint32_t fallback_iteration;   /* learned policy: first iteration on the plain fallback (0 = none) */
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -105,6 +107,15 @@ typedef struct subway_iter_record {
     uint64_t active_edges;
     uint64_t h2d_bytes;
     double ms_iteration;
+    /* learned policy: the iteration's 12 scheduler features and 3 network heads
+     * (next active-edge fraction, whole-range density threshold, log2(round
+     * cap) / 16), the threshold and cap applied, and 1 when the plain
+     * schedule ran the iteration */
+    float features[12];
+    float heads[3];
+    float theta;
+    uint32_t rounds_cap;
+    uint32_t on_plain;
 } subway_iter_record;
 
 /* -------- library */
@@ -151,6 +162,11 @@ int32_t subway_session_run(subway_session *s, uint32_t source, void *values, sub
  * until the next run or the session's end.  Lets a caller read results
  * without the copy `values` makes (pass values = NULL to run). */
 const void *subway_session_values(const subway_session *s);
+/* Learned policy (options.policy = SUBWAY_POLICY_LEARNED): the scheduler
+ * network's SUBWAY_MLP_PARAMS parameters (layout of subway_mlp_forward), or
+ * NULL to schedule with the options' fixed theta and round cap.  No
+ * counterpart in the reference (its schedule is fixed). */
+int32_t subway_session_set_policy(subway_session *s, const float *params);
 void subway_session_free(subway_session *s);
 /* Batched session (options.batch = 1): k <= 64 sources advance together in
  * one cooperative kernel per transferred partition.  rows: n x 64 uint32,

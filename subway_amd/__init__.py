@@ -63,7 +63,7 @@ class _Stats(ctypes.Structure):
         ("transfer_ops", ctypes.c_uint64), ("rounds_launches", ctypes.c_uint64), ("work_edges", ctypes.c_uint64),
         ("work_entries", ctypes.c_uint64),
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
-        ("work_items", ctypes.c_uint64),
+        ("work_items", ctypes.c_uint64), ("mispredictions", ctypes.c_uint32), ("fallback_iteration", ctypes.c_uint32),
     ]
 
 
@@ -71,6 +71,8 @@ class _IterRecord(ctypes.Structure):
     _fields_ = [
         ("iteration", ctypes.c_uint32), ("partitions", ctypes.c_uint32), ("active_nodes", ctypes.c_uint64),
         ("active_edges", ctypes.c_uint64), ("h2d_bytes", ctypes.c_uint64), ("ms_iteration", ctypes.c_double),
+        ("features", ctypes.c_float * 12), ("heads", ctypes.c_float * 3), ("theta", ctypes.c_float),
+        ("rounds_cap", ctypes.c_uint32), ("on_plain", ctypes.c_uint32),
     ]
 
 
@@ -188,10 +190,22 @@ class RunStats:
     work_entries: int
     global_iterations: int
     work_items: int = 0
+    mispredictions: int = 0
+    fallback_iteration: int = 0
     trace: list = field(default_factory=list)
 
 
 ROUNDS = {"persistent": 0, "per-round": 1}
+
+
+def _set_policy(h, params):
+    if params is None:
+        check(lib.subway_session_set_policy(h, None))
+        return
+    p = np.ascontiguousarray(params, np.float32)
+    if p.size != 5211:
+        raise ValueError(f"scheduler parameters: expected 5211 floats, got {p.size}")
+    check(lib.subway_session_set_policy(h, p.ctypes.data_as(ctypes.c_void_p)))
 
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
@@ -221,7 +235,9 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
 
 def _stats(st, rec):
     trace = [dict(iteration=r.iteration, partitions=r.partitions, active_nodes=r.active_nodes,
-                  active_edges=r.active_edges, h2d_bytes=r.h2d_bytes, ms=r.ms_iteration)
+                  active_edges=r.active_edges, h2d_bytes=r.h2d_bytes, ms=r.ms_iteration,
+                  features=list(r.features), heads=list(r.heads), theta=r.theta, rounds_cap=r.rounds_cap,
+                  on_plain=r.on_plain)
              for r in rec[: st.trace_len]]
     names = [f.name for f in RunStats.__dataclass_fields__.values() if f.name != "trace"]
     return RunStats(**{k: getattr(st, k) for k in names}, trace=trace)
@@ -238,7 +254,7 @@ class Session:
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
-                 packed: bool = True):
+                 packed: bool = True, params=None):
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                      profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
@@ -246,6 +262,13 @@ class Session:
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         self._rec = (_IterRecord * _TRACE_CAP)()
+        if params is not None:
+            self.set_policy(params)
+
+    def set_policy(self, params):
+        """Scheduler network parameters for policy="learned" (None: the fixed
+        theta and round cap of the options)."""
+        _set_policy(self._h, params)
 
     def run(self, source: int = 0, copy: bool = True):
         """One query.  Returns (values, RunStats).  With copy=False the values
@@ -282,7 +305,7 @@ class BatchSession:
 
     def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",
                  unroll: int = 0, rounds_grid: int = 0, profile: bool = False, policy: str = "plain",
-                 theta: float = 0.5, packed: bool = True):
+                 theta: float = 0.5, packed: bool = True, params=None):
         self.graph, self.algo = graph, algo
         o = _options(algo, "async", 0, budget, buffers, 0.01, 0.15, transfer, policy, False, 0, profile,
                      "persistent", rounds_grid, 0, theta, 0, packed)
@@ -291,6 +314,8 @@ class BatchSession:
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        if params is not None:
+            _set_policy(self._h, params)
 
     def run(self, sources, rows_out: bool = True):
         src = np.ascontiguousarray(sources, dtype=np.uint32)
@@ -322,13 +347,24 @@ class BatchSession:
 def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: int = 0, buffers: int = 2,
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
-        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True):
+        rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True,
+        params=None):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
     float32 (PR).  ``budget`` is the device edge budget in bytes (0 = 90% of
-    device memory, the reference's sizing).
+    device memory, the reference's sizing).  ``params``: scheduler network
+    for policy="learned".
     """
+    if params is not None:
+        s = Session(graph, algo, mode=mode, budget=budget, buffers=buffers, threshold=threshold, init_pr=init_pr,
+                    transfer=transfer, policy=policy, verbose=verbose, advance_grid=advance_grid, profile=profile,
+                    rounds=rounds, rounds_grid=rounds_grid, max_rounds=max_rounds, theta=theta,
+                    spin_limit=spin_limit, packed=packed, params=params)
+        try:
+            return s.run(source)
+        finally:
+            s.close()
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                  profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)

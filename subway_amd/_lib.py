@@ -55,6 +55,7 @@ _SIGS = {
     "subway_session_create": (_i32, [_vp, _vp, ctypes.POINTER(_vp)]),
     "subway_session_run": (_i32, [_vp, _u32, _vp, _vp, _vp, _u32]),
     "subway_session_values": (_vp, [_vp]),
+    "subway_session_set_policy": (_i32, [_vp, _vp]),
     "subway_session_free": (None, [_vp]),
     "subway_session_run_batch": (_i32, [_vp, _u32p, _i32, _u32p, _vp, _vp, _u32]),
     "subway_session_tune_batch": (_i32, [_vp, _u32p, _i32, _i32p, ctypes.POINTER(ctypes.c_double)]),

@@ -249,6 +249,11 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
             trace[k].active_edges = r.activeEdges;
             trace[k].h2d_bytes = r.h2dBytes;
             trace[k].ms_iteration = r.msIteration;
+            std::memcpy(trace[k].features, r.features, sizeof(trace[k].features));
+            std::memcpy(trace[k].heads, r.heads, sizeof(trace[k].heads));
+            trace[k].theta = r.theta;
+            trace[k].rounds_cap = r.roundsCap;
+            trace[k].on_plain = r.onPlain;
             k++;
         }
     }
@@ -275,6 +280,8 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->global_iterations = s.globalIterations;
     stats->trace_len = k;
     stats->work_items = s.workItems;
+    stats->mispredictions = s.mispredictions;
+    stats->fallback_iteration = s.fallbackIteration;
 }
 
 extern "C" {
@@ -310,6 +317,14 @@ int32_t subway_session_run(subway_session *s, uint32_t source, void *values, sub
 }
 
 const void *subway_session_values(const subway_session *s) { return s ? s->impl->valuesPtr() : nullptr; }
+
+int32_t subway_session_set_policy(subway_session *s, const float *params)
+{
+    return guard([&] {
+        if (!s) throw Error(ERR_ARG, "null session");
+        s->impl->setPolicy(params);
+    });
+}
 
 void subway_session_free(subway_session *s) { delete s; }
 

@@ -24,6 +24,7 @@
 
 #include "host_util.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "subway/engine.hpp"
 #include "subway/gpu_error_check.hpp"
 #include "subway/partitioner.hpp"
@@ -230,6 +231,18 @@ public:
     }
     int batchUnroll() const { return batchUnroll_; }
     RunStats runBatch(const uint *sources, int k);
+    // scheduler network of the learned policy (MLP_PARAMS floats, host); null removes it
+    void setPolicy(const float *params)
+    {
+        if (!params) {
+            if (d_params_) SUBWAY_CHECK(hipFree(d_params_));
+            d_params_ = nullptr;
+            return;
+        }
+        if (!d_bounds_) throw Error(ERR_ARG, "a scheduler network needs the learned policy");
+        if (!d_params_) SUBWAY_CHECK(hipMalloc(&d_params_, MLP_PARAMS * sizeof(float)));
+        SUBWAY_CHECK(hipMemcpy(d_params_, params, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+    }
     // rows of the last batched run: n x 64 u32, vertex-major
     const uint32_t *batchRows() const { return h_dist_; }
 
@@ -238,8 +251,10 @@ public:
         if (sc_) (void)hipStreamSynchronize(sc_);
         if (sx_) (void)hipStreamSynchronize(sx_);
         if (d_bounds_) (void)hipFree(d_bounds_);
-        if (d_rstats_) (void)hipFree(d_rstats_);
-        if (h_rstats_) (void)hipHostFree(h_rstats_);
+        if (d_pol_) (void)hipFree(d_pol_);
+        if (h_pol_) (void)hipHostFree(h_pol_);
+        if (d_pstate_) (void)hipFree(d_pstate_);
+        if (d_params_) (void)hipFree(d_params_);
         if (d_dist_) (void)hipFree(d_dist_);
         if (d_m1_) (void)hipFree(d_m1_);
         if (d_m2_) (void)hipFree(d_m2_);
@@ -331,8 +346,16 @@ private:
     // learned policy: static id ranges of at most one buffer of edges
     std::vector<uint> bounds_;
     uint32_t *d_bounds_ = nullptr;
-    ull *d_rstats_ = nullptr, *h_rstats_ = nullptr;
+    char *d_pol_ = nullptr, *h_pol_ = nullptr;   // PolicyOut, then P x 4 range stats
+    ull *h_rstats_ = nullptr;                     // range stats inside h_pol_
+    size_t polBytes_ = 0;
+    PolicyState *d_pstate_ = nullptr;
+    float *d_params_ = nullptr;                   // scheduler network; null: fixed theta / cap
+    bool onPlain_ = false;                        // learned policy fell back to the plain schedule
+    float predEdges_ = -1.f;                      // last prediction of the next active-edge fraction
+    int missStreak_ = 0;
     double theta_ = 2.0;    // density at or above which a range moves whole
+    void applyPolicy(IterRecord &rec);
     void setupRanges();
     void buildUnitsLearned();
     void pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, ull edges);
@@ -383,10 +406,26 @@ void Engine<E, G>::generateAndPartition()
         gen_->enqueue(*g_, *sub_);
     if (o_.profile) pGen_.end(sc_);
     if (d_bounds_) {
-        const int P = (int)bounds_.size() - 1;
-        launch_range_stats(sub_->d_activeNodes, sub_->d_activeNodesPointer, gen_->deviceTotals(), d_bounds_, P,
-                           d_rstats_, sc_);
-        SUBWAY_CHECK(hipMemcpyAsync(h_rstats_, d_rstats_, (size_t)P * 4 * sizeof(ull), hipMemcpyDeviceToHost, sc_));
+        // the scheduler's reduction and inference, one workgroup
+        PolicyArgs pa{};
+        pa.activeNodes = sub_->d_activeNodes;
+        pa.anp = sub_->d_activeNodesPointer;
+        pa.totals = gen_->deviceTotals();
+        pa.hist = gen_->scanScratch().ctl->hist;
+        pa.updatesNow = &fr_.ctl->updates;
+        pa.state = d_pstate_;
+        pa.bounds = d_bounds_;
+        pa.P = (int)bounds_.size() - 1;
+        pa.nodePointer = g_->d_nodePointer;
+        pa.params = onPlain_ ? nullptr : d_params_;
+        pa.V = g_->num_nodes;
+        pa.E = g_->num_edges;
+        pa.bufEdges = maxPart_;
+        pa.iteration = st_.globalIterations + 1;
+        pa.out = reinterpret_cast<PolicyOut *>(d_pol_);
+        pa.rstats = reinterpret_cast<ull *>(d_pol_ + sizeof(PolicyOut));
+        launch_policy(pa, sc_);
+        SUBWAY_CHECK(hipMemcpyAsync(h_pol_, d_pol_, polBytes_, hipMemcpyDeviceToHost, sc_));
     }
     // error words of every bounded wait so far; the partitioner synchronises
     // the stream before they are read
@@ -435,14 +474,17 @@ void Engine<E, G>::setupRanges()
 {
     const ull n = g_->num_nodes;
     const ull *np = g_->nodePointer;
+    // ranges of about an eighth of a buffer: the schedule decides per range,
+    // consecutive whole ranges share a unit
+    const ull rangeEdges = std::max<ull>(maxPart_ / 8, (ull)ADV_CHUNK);
     bounds_.assign(1, 0);
     ull v = 0;
     while (v < n) {
-        // largest t with np[t] - np[v] <= maxPart_
+        // largest t with np[t] - np[v] <= rangeEdges
         ull lo = v, hi = n;
         while (lo < hi) {
             ull mid = lo + (hi - lo + 1) / 2;
-            if (np[mid] - np[v] <= maxPart_) lo = mid;
+            if (np[mid] - np[v] <= rangeEdges) lo = mid;
             else hi = mid - 1;
         }
         v = (lo == v) ? v + 1 : lo;  // a vertex larger than a buffer is a range of its own
@@ -451,8 +493,13 @@ void Engine<E, G>::setupRanges()
     const size_t P = bounds_.size() - 1;
     SUBWAY_CHECK(hipMalloc(&d_bounds_, bounds_.size() * sizeof(uint32_t)));
     SUBWAY_CHECK(hipMemcpy(d_bounds_, bounds_.data(), bounds_.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    SUBWAY_CHECK(hipMalloc(&d_rstats_, P * 4 * sizeof(ull)));
-    SUBWAY_CHECK(hipHostMalloc((void **)&h_rstats_, P * 4 * sizeof(ull), 0));
+    polBytes_ = sizeof(PolicyOut) + P * 4 * sizeof(ull);
+    SUBWAY_CHECK(hipMalloc(&d_pol_, polBytes_));
+    SUBWAY_CHECK(hipHostMalloc((void **)&h_pol_, polBytes_, 0));
+    h_rstats_ = reinterpret_cast<ull *>(h_pol_ + sizeof(PolicyOut));
+    SUBWAY_CHECK(hipMalloc(&d_pstate_, sizeof(PolicyState)));
+    SUBWAY_CHECK(hipMemset(d_pstate_, 0, sizeof(PolicyState)));
+    SUBWAY_CHECK(hipDeviceSynchronize());
 }
 
 template <class E, class G>
@@ -472,11 +519,56 @@ void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, u
     }
 }
 
-// Learned schedule: ranges dense enough (active edges / range edges >=
-// theta_) move whole by DMA and their rounds cover every vertex of the
-// range; the other ranges' active slices are packed into compacted units of
-// at most one buffer (DMA when they form a contiguous fully active run,
-// else gathered).
+// Learned policy, per global iteration, after the policy kernel: the
+// network's heads give the next iteration's active-edge fraction, the
+// whole-range density threshold and the inner-round cap.  The plain policy is
+// the fallback and the judge: a prediction more than 2x off the active edges
+// the iteration then shows is a misprediction, and two in a row hand the rest
+// of the run to the plain schedule.  Without parameters the options' fixed
+// theta and cap apply.
+template <class E, class G>
+void Engine<E, G>::applyPolicy(IterRecord &rec)
+{
+    const PolicyOut &po = *reinterpret_cast<const PolicyOut *>(h_pol_);
+    std::memcpy(rec.features, po.feat, sizeof(rec.features));
+    std::memcpy(rec.heads, po.head, sizeof(rec.heads));
+    if (d_params_ && !onPlain_) {
+        const float actual = po.feat[2];
+        if (predEdges_ >= 0.f) {
+            const float r = (predEdges_ + 1e-6f) / (actual + 1e-6f);
+            if (r > 2.f || r < 0.5f) {
+                st_.mispredictions++;
+                if (++missStreak_ >= 2) {
+                    onPlain_ = true;
+                    st_.fallbackIteration = st_.globalIterations;
+                }
+            } else {
+                missStreak_ = 0;
+            }
+        }
+        if (!onPlain_) {
+            predEdges_ = std::max(0.f, po.head[0]);
+            theta_ = std::min(std::max((double)po.head[1], 0.0), 2.0);
+            const double c = std::exp2(std::min(std::max((double)po.head[2], 0.0), 1.0) * 16.0);
+            roundsCap_ = (int)std::min<double>(std::max(1.0, std::round(c)), (double)o_.maxRounds);
+        }
+    }
+    if (onPlain_) {
+        theta_ = 2.0;   // never whole
+        roundsCap_ = o_.maxRounds;
+    }
+    rec.theta = (float)theta_;
+    rec.roundsCap = (uint)roundsCap_;
+    rec.onPlain = onPlain_ ? 1u : 0u;
+}
+
+// Learned schedule over static id ranges: a range whose active-edge density
+// (active edges / range edges) reaches theta_ moves whole by DMA and its
+// rounds cover every vertex of the range; consecutive whole ranges share a
+// unit up to one buffer.  The other ranges' active slices are packed into
+// compacted units of at most one buffer (DMA when they form one fully active
+// contiguous run, else gathered).  Vertices larger than a buffer stream in
+// chunks.
 template <class E, class G>
 void Engine<E, G>::buildUnitsLearned()
 {
@@ -484,10 +576,15 @@ void Engine<E, G>::buildUnitsLearned()
     const ull *np = g_->nodePointer;
     const size_t P = bounds_.size() - 1;
     uint part = 0;
+    // open compacted group
     bool open = false;
     ull gLo = 0, gHi = 0, gEdges = 0, gBase = 0;
     bool gFull = true;
     uint gFirstRange = 0;
+    // open whole group: vertex ids [wB0, wB1)
+    bool wOpen = false;
+    uint wB0 = 0, wB1 = 0;
+    ull wEdges = 0;
     auto flush = [&]() {
         if (!open) return;
         Unit u{};
@@ -503,37 +600,56 @@ void Engine<E, G>::buildUnitsLearned()
         units_.push_back(u);
         open = false;
     };
+    auto flushWhole = [&]() {
+        if (!wOpen) return;
+        Unit u{};
+        u.part = part++;
+        u.dma = true;
+        u.hostEdge = np[wB0];
+        u.count = wEdges;
+        u.whole = true;
+        u.n = wB1 - wB0;
+        u.nodes = nullptr;
+        u.ptr = g_->d_nodePointer + wB0;
+        u.fromEdge = np[wB0];
+        u.idBase = wB0;
+        units_.push_back(u);
+        wOpen = false;
+    };
     for (size_t i = 0; i < P; i++) {
         const ull lo = h_rstats_[4 * i], hi = h_rstats_[4 * i + 1], ea = h_rstats_[4 * i + 2];
         const ull base = h_rstats_[4 * i + 3];
-        if (hi == lo) {
-            flush();
-            continue;
-        }
         const uint b0 = bounds_[i], b1 = bounds_[i + 1];
         const ull re = np[b1] - np[b0];
+        if (hi == lo) {
+            // nothing active: the open groups stay open, but a compacted group
+            // no longer covers one contiguous run of edges
+            if (open && re > 0) gFull = false;
+            flushWhole();
+            continue;
+        }
         const bool hub = (b1 - b0 == 1) && re > maxPart_;
         if (hub) {
             flush();
+            flushWhole();
             pushHubUnits(part++, b0, lo, base, re);
             continue;
         }
         if (re > 0 && (double)ea >= theta_ * (double)re) {
             flush();
-            Unit u{};
-            u.part = part++;
-            u.dma = true;
-            u.hostEdge = np[b0];
-            u.count = re;
-            u.whole = true;
-            u.n = b1 - b0;
-            u.nodes = nullptr;
-            u.ptr = g_->d_nodePointer + b0;
-            u.fromEdge = np[b0];
-            u.idBase = b0;
-            units_.push_back(u);
+            if (wOpen && wB1 == b0 && wEdges + re <= maxPart_) {
+                wB1 = b1;
+                wEdges += re;
+            } else {
+                flushWhole();
+                wOpen = true;
+                wB0 = b0;
+                wB1 = b1;
+                wEdges = re;
+            }
             continue;
         }
+        flushWhole();
         const bool full = (hi - lo) == (ull)(b1 - b0);
         if (open && gEdges + ea > maxPart_) flush();
         if (!open) {
@@ -545,19 +661,20 @@ void Engine<E, G>::buildUnitsLearned()
             gFull = full;
             gFirstRange = (uint)i;
         } else {
-            // consecutive ranges: their active slices are adjacent
+            // the active slices of consecutive ranges are adjacent in the active list
             gFull = gFull && full;
             gHi = hi;
             gEdges += ea;
         }
     }
     flush();
+    flushWhole();
 }
 
 template <class E, class G>
 void Engine<E, G>::buildUnits()
 {
-    if (d_bounds_) {
+    if (d_bounds_ && !onPlain_) {
         buildUnitsLearned();
         return;
     }
@@ -885,6 +1002,14 @@ void Engine<E, G>::resetRun()
     st_ = RunStats();
     roundsCap_ = o_.maxRounds;
     theta_ = o_.theta;
+    onPlain_ = false;
+    predEdges_ = -1.f;
+    missStreak_ = 0;
+    if (d_pstate_) {
+        // updates counted from here on belong to this run
+        SUBWAY_CHECK(hipMemcpyAsync(&d_pstate_->prevUpdates, &fr_.ctl->updates, sizeof(ull),
+                                    hipMemcpyDeviceToDevice, sc_));
+    }
     pAdv_.reset();
     pFr_.reset();
     pXfer_.reset();
@@ -908,6 +1033,7 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
         rec.activeEdges = sub_->numActiveEdges;
         rec.partitions = part_->numPartitions;
         const ull h2d0 = st_.h2dBytes;
+        if (d_bounds_) applyPolicy(rec);
         buildUnits();
         const size_t nu = units_.size();
         const size_t ahead = (size_t)sub_->numBuffers - 1;
@@ -916,7 +1042,8 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
             while (issued < nu && issued <= u + ahead) issue(issued++);
             compute(u);
         }
-        st_.partitions += d_bounds_ ? (units_.empty() ? 0 : units_.back().part + 1) : part_->numPartitions;
+        st_.partitions += (d_bounds_ && !onPlain_) ? (units_.empty() ? 0 : units_.back().part + 1)
+                                                   : part_->numPartitions;
         rec.h2dBytes = st_.h2dBytes - h2d0;
         generateAndPartition();
         rec.msIteration = ms_since(ti);
@@ -1041,6 +1168,7 @@ public:
     const void *valuesPtr() override { return eng_.graph().value; }
     ull numNodes() const override { return const_cast<SessionImpl *>(this)->eng_.graph().num_nodes; }
     RunStats runBatch(const uint *sources, int k) override { return eng_.runBatch(sources, k); }
+    void setPolicy(const float *params) override { eng_.setPolicy(params); }
     void rows(uint *out) override
     {
         const uint32_t *r = eng_.batchRows();

@@ -17,6 +17,7 @@
 //    subway/sssp-async.cu:86).
 #include "kernels.hpp"
 #include "device.hpp"
+#include "mlp_device.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -73,6 +74,30 @@ __device__ __forceinline__ bool scan_tile(const Op &op, u64 n, u32 tile, u32 epo
     if (lane == WAVE - 1) {
         s_wcnt[wid] = icnt;
         s_wsum[wid] = isum;
+    }
+    if constexpr (Op::TALLY) {
+        // degree histogram of the selected items, one atomic per bucket and tile
+        __shared__ u32 s_hist[NW][4];
+        u32 h[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            if ((mask >> j) & 1u) {
+                const u64 d = w[j];
+                h[d < 4 ? 0 : d < 32 ? 1 : d < 256 ? 2 : 3]++;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const u32 t = wave_reduce(h[b]);
+            if (lane == 0) s_hist[wid][b] = t;
+        }
+        __syncthreads();
+        if (tid < 4) {
+            u32 t = 0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) t += s_hist[q][tid];
+            if (t) atomicAdd((unsigned long long *)&ctl->hist[tid], (unsigned long long)t);
+        }
     }
     __syncthreads();
     u32 wbc = 0, totc = 0;
@@ -157,6 +182,7 @@ struct Empty {};
 // activeness = label1 || label2 (shared/subgraph_generator.cu:15)
 struct GenLabelOp {
     typedef Empty Item;
+    static constexpr bool TALLY = true;
     const uint8_t *l1, *l2;
     const u32 *outDegree;
     u32 *activeNodes;
@@ -210,6 +236,7 @@ struct GenLabelOp {
 // activeness = delta > acc (shared/subgraph_generator.cu:24-41)
 struct GenPrOp {
     typedef Empty Item;
+    static constexpr bool TALLY = true;
     const float *delta;
     float acc;
     const u32 *outDegree;
@@ -309,6 +336,7 @@ __device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc
 template <bool COH>
 struct FrontierLabelOp {
     typedef FrItem Item;
+    static constexpr bool TALLY = false;
     const u32 *nodes;
     const u64 *ptr;
     u64 fromEdge;
@@ -361,6 +389,7 @@ struct FrontierLabelOp {
 template <bool COH>
 struct FrontierPrOp {
     typedef FrItem Item;
+    static constexpr bool TALLY = false;
     const u32 *nodes;
     const u64 *ptr;
     u64 fromEdge;
@@ -416,6 +445,7 @@ constexpr int BATCH_CHUNK = 256;
 // activeness = (mask1 | mask2) != 0
 struct GenMaskOp {
     typedef Empty Item;
+    static constexpr bool TALLY = true;
     const u64 *m1, *m2;
     const u32 *outDegree;
     u32 *activeNodes;
@@ -456,6 +486,7 @@ struct FrMaskItem {
 template <bool COH>
 struct FrontierMaskOp {
     typedef FrMaskItem Item;
+    static constexpr bool TALLY = false;
     const u32 *nodes;
     const u64 *ptr;
     u64 fromEdge;
@@ -1213,18 +1244,74 @@ __device__ __forceinline__ u64 lower_bound_u32(const u32 *a, u64 n, u32 key)
     return lo;
 }
 
-__global__ void range_stats_kernel(const u32 *activeNodes, const u64 *anp, const u64 *dA, const u32 *bounds,
-                                   int P, u64 *out)
+// The scheduler's per-iteration reduction and inference in one workgroup,
+// right after the generator: per id range the slice [lo, hi) of the active
+// list, its active edges and its first edge offset; the 12 features of the
+// iteration; and, when parameters are given, the network's forward pass on
+// wave 0.  Clears the generator's degree histogram for the next iteration.
+__global__ __launch_bounds__(256) void policy_kernel(PolicyArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    const u64 A = selected_count(dA);
-    const u64 lo = lower_bound_u32(activeNodes, A, bounds[i]);
-    const u64 hi = lower_bound_u32(activeNodes, A, bounds[i + 1]);
-    out[4 * i + 0] = lo;
-    out[4 * i + 1] = hi;
-    out[4 * i + 2] = anp[hi] - anp[lo];
-    out[4 * i + 3] = anp[lo];
+    __shared__ u32 s_dense;
+    __shared__ float s_x[MLP_IN];
+    const int tid = threadIdx.x;
+    if (tid == 0) s_dense = 0;
+    __syncthreads();
+    const u64 A = selected_count(a.totals);
+    const u64 EA = A ? ld_relaxed(&a.totals[1]) : 0ull;
+    u32 dense = 0;
+    for (int i = tid; i < a.P; i += blockDim.x) {
+        const u64 lo = lower_bound_u32(a.activeNodes, A, a.bounds[i]);
+        const u64 hi = lower_bound_u32(a.activeNodes, A, a.bounds[i + 1]);
+        const u64 ea = a.anp[hi] - a.anp[lo];
+        a.rstats[4 * i + 0] = lo;
+        a.rstats[4 * i + 1] = hi;
+        a.rstats[4 * i + 2] = ea;
+        a.rstats[4 * i + 3] = a.anp[lo];
+        const u64 re = a.nodePointer[a.bounds[i + 1]] - a.nodePointer[a.bounds[i]];
+        if (re && 2 * ea >= re) dense++;
+    }
+    dense = wave_reduce(dense);
+    if (lane_id() == 0 && dense) atomicAdd(&s_dense, dense);
+    __syncthreads();
+    if (tid == 0) {
+        const float V = (float)a.V, E = (float)(a.E ? a.E : 1);
+        const u64 upd = ld_relaxed(a.updatesNow);
+        const u64 lastUpd = upd - a.state->prevUpdates;
+        a.state->prevUpdates = upd;
+        float hsum = 0.f, h[4];
+        for (int b = 0; b < 4; b++) {
+            h[b] = (float)a.hist[b];
+            hsum += h[b];
+        }
+        float x[MLP_IN];
+        x[0] = log2f(1.f + (float)A) / fmaxf(log2f(V), 1.f);
+        x[1] = (float)A / V;
+        x[2] = (float)EA / E;
+        x[3] = log2f(1.f + (float)EA) / fmaxf(log2f(E), 1.f);
+        for (int b = 0; b < 4; b++) x[4 + b] = hsum > 0.f ? h[b] / hsum : 0.f;
+        x[8] = (float)a.bufEdges / E;
+        x[9] = (float)a.iteration / 16.f;
+        x[10] = (float)lastUpd / E;
+        x[11] = a.P ? (float)s_dense / (float)a.P : 0.f;
+        for (int i = 0; i < MLP_IN; i++) {
+            s_x[i] = x[i];
+            a.out->feat[i] = x[i];
+        }
+        for (int b = 0; b < 4; b++) a.hist[b] = 0;
+    }
+    __syncthreads();
+    if (tid < WAVE) {
+        float head[MLP_OUT];
+        if (a.params) {
+            Fwd f;
+            forward_wave(a.params, s_x, f);
+            for (int m = 0; m < MLP_OUT; m++) head[m] = f.out[m];
+        } else {
+            for (int m = 0; m < MLP_OUT; m++) head[m] = 0.f;
+        }
+        if (tid == 0)
+            for (int m = 0; m < MLP_OUT; m++) a.out->head[m] = head[m];
+    }
 }
 
 // ============================================================== gather
@@ -1789,12 +1876,10 @@ void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l
                                             dim3(ADV_BLOCK), params, 0, st));
 }
 
-void launch_range_stats(const uint32_t *activeNodes, const ull *anp, const ull *dA, const uint32_t *bounds, int P,
-                        ull *out, hipStream_t st)
+void launch_policy(const PolicyArgs &a, hipStream_t st)
 {
-    if (P <= 0) return;
-    hipLaunchKernelGGL(range_stats_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, activeNodes,
-                       (const u64 *)anp, (const u64 *)dA, bounds, P, (u64 *)out);
+    if (a.P <= 0) throw Error(ERR_ARG, "policy over no id ranges");
+    hipLaunchKernelGGL(policy_kernel, dim3(1), dim3(256), 0, st, a);
     SUBWAY_CHECK(hipGetLastError());
 }
 

@@ -107,8 +107,33 @@ void launch_partition(const ull *activeNodesPointer, const uint32_t *activeNodes
 
 // ---- per static id range: active-list slice [lo, hi), its active edges and
 // the active-edge offset at lo (out: 4 x u64 per range)
-void launch_range_stats(const uint32_t *activeNodes, const ull *anp, const ull *dA, const uint32_t *bounds, int P,
-                        ull *out, hipStream_t st);
+// Scheduler features and heads of one iteration (the policy kernel's output).
+struct PolicyOut {
+    float feat[12];
+    float head[3];
+    float pad;
+};
+// Device state the policy kernel carries from iteration to iteration.
+struct PolicyState {
+    ull prevUpdates;
+};
+struct PolicyArgs {
+    const uint32_t *activeNodes;
+    const ull *anp;                  // activeNodesPointer
+    const ull *totals;               // generator [count, total, error]
+    ull *hist;                       // generator degree histogram (cleared here)
+    const ull *updatesNow;           // cumulative successful updates
+    PolicyState *state;
+    const uint32_t *bounds;          // P + 1 id-range bounds
+    int P;
+    const ull *nodePointer;          // device CSR offsets
+    const float *params;             // MLP parameters or null (features only)
+    ull V, E, bufEdges;
+    uint32_t iteration;
+    PolicyOut *out;
+    ull *rstats;                     // P x {lo, hi, active edges, first offset}
+};
+void launch_policy(const PolicyArgs &a, hipStream_t st);
 
 // ---- zero-copy gather of a partition's adjacency from mapped host memory
 void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *activeNodesPointer,

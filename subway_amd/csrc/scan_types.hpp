@@ -31,6 +31,10 @@ struct ScanCtl {
     u64 entry_sum;  // selected items over all launches (statistics)
     u64 work_sum;   // selected weight over all launches (statistics)
     u64 item_sum;   // items scanned over all launches (statistics)
+    // generator only: out-degree histogram of the selected items (< 4, 4..31,
+    // 32..255, >= 256), accumulated until the scheduler's policy kernel reads
+    // and clears it
+    u64 hist[4];
 };
 static_assert(offsetof(ScanCtl, total) == offsetof(ScanCtl, count) + 8 &&
                   offsetof(ScanCtl, error) == offsetof(ScanCtl, count) + 16,

@@ -14,6 +14,7 @@ threshold; log2(inner-round cap) / 16.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -71,6 +72,15 @@ def train(params, x, y, mask, steps: int = 2000, lr: float = 3e-3):
     losses = np.empty(max(steps, 1), np.float32)
     check(lib.subway_mlp_train(_p(params), _p(x), _p(y), _p(mask), len(x), steps, lr, _p(losses)))
     return params, losses[:steps]
+
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "scheduler_mlp.f32")
+
+
+def default_params():
+    """The scheduler network trained by subway_amd.train_scheduler and shipped
+    with the package, or None when the package has none."""
+    return load(DEFAULT_PATH) if os.path.exists(DEFAULT_PATH) else None
 
 
 def save(path: str, params) -> None:

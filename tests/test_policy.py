@@ -1,0 +1,119 @@
+"""Learned scheduler on the GPU: the features the policy kernel records, the
+fused inference against the stand-alone MLP kernels and the FP64 oracle, the
+misprediction judge and its plain fallback, and bit-exact results under any
+schedule the network picks."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+
+def _rmat(sw, scale, seed, weighted=True):
+    return sw.Graph.rmat(scale, 16, seed=seed, weighted=weighted)
+
+
+def test_features_of_first_iteration(sw, gpu):
+    """Async runs activate every vertex for the first generate: the active
+    counts, edge fraction and degree histogram are those of the whole graph."""
+    g = _rmat(sw, 14, 3)
+    ptr, adj, w = g.csr()
+    deg = np.diff(ptr).astype(np.int64)
+    vals, st = sw.run(g, "sssp", mode="async", source=g.max_degree_vertex, budget=128 << 10, policy="learned",
+                      theta=2.0)
+    assert np.array_equal(vals, O.sssp(ptr, adj, w, g.max_degree_vertex))
+    f = np.array(st.trace[0]["features"], np.float64)
+    a, ea = g.n, int(deg.sum())                    # every vertex is labelled, edges or not
+    assert st.trace[0]["active_nodes"] == a and st.trace[0]["active_edges"] == ea
+    np.testing.assert_allclose(f[1], a / g.n, rtol=1e-6)
+    np.testing.assert_allclose(f[2], ea / g.m, rtol=1e-6)
+    np.testing.assert_allclose(f[0], np.log2(1 + a) / np.log2(g.n), rtol=1e-5)
+    d = deg
+    hist = np.array([(d < 4).sum(), ((d >= 4) & (d < 32)).sum(), ((d >= 32) & (d < 256)).sum(), (d >= 256).sum()])
+    np.testing.assert_allclose(f[4:8], hist / hist.sum(), rtol=1e-5)
+    np.testing.assert_allclose(f[9], 1 / 16, rtol=1e-6)
+    # later iterations see only their labelled vertices
+    for r in st.trace[1:]:
+        np.testing.assert_allclose(r["features"][2], r["active_edges"] / g.m, rtol=1e-5)
+
+
+def test_fused_inference_matches_mlp_kernel_and_oracle(sw, gpu):
+    """The heads the policy kernel computes from an iteration's features
+    equal the stand-alone forward kernel's and the FP64 oracle's."""
+    from subway_amd import policy
+    g = _rmat(sw, 14, 5)
+    rng = np.random.default_rng(2)
+    p = policy.init_params(7, rng.random(12).astype(np.float32), 0.5 + rng.random(12).astype(np.float32))
+    _, st = sw.run(g, "sssp", mode="async", source=g.max_degree_vertex, budget=128 << 10, policy="learned",
+                   params=p)
+    # iterations the network scheduled (after a fallback the plain policy runs
+    # and no inference is made)
+    used = [r for r in st.trace if not r["on_plain"]]
+    assert len(used) >= 2
+    x = np.array([r["features"] for r in used], np.float32)
+    h = np.array([r["heads"] for r in used], np.float32)
+    np.testing.assert_allclose(h, policy.forward(p, x), rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(h, M.forward(p.astype(np.float64), x.astype(np.float64))[0], rtol=1e-3, atol=1e-4)
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "sswp"])
+def test_any_network_keeps_results_exact(sw, gpu, algo):
+    """Random networks choose arbitrary thresholds and round caps; values
+    stay bit-exact, and the decisions applied follow the heads."""
+    from subway_amd import policy
+    weighted = algo != "bfs"
+    g = _rmat(sw, 15, 9, weighted)
+    ptr, adj, w = g.csr()
+    src = g.max_degree_vertex
+    exp = O.expected(algo, ptr, adj, w, src)
+    for seed in range(3):
+        p = policy.init_params(seed)
+        vals, st = sw.run(g, algo, mode="async", source=src, budget=256 << 10, policy="learned", params=p)
+        assert np.array_equal(vals, exp), (algo, seed)
+        for r in st.trace:
+            if r["on_plain"]:
+                assert r["theta"] == 2.0
+                continue
+            assert r["theta"] == pytest.approx(min(max(r["heads"][1], 0.0), 2.0), rel=1e-6)
+            cap = 2 ** (16 * min(max(r["heads"][2], 0.0), 1.0))
+            assert r["rounds_cap"] == min(max(1, round(cap)), 4096)
+
+
+def test_mispredictions_fall_back_to_plain(sw, gpu):
+    """A network that always predicts the whole graph active next is judged
+    wrong twice in a row and hands the run to the plain schedule."""
+    from subway_amd import policy
+    g = _rmat(sw, 15, 4)
+    p = policy.init_params(1)
+    p[policy.NORM:] = 0.0
+    p[-3] = 1.0          # next active-edge fraction: always 1.0
+    p[-2] = 0.5          # theta 0.5
+    p[-1] = 0.375        # cap 2^6
+    ptr, adj, w = g.csr()
+    src = g.max_degree_vertex
+    vals, st = sw.run(g, "sssp", mode="async", source=src, budget=256 << 10, policy="learned", params=p)
+    assert np.array_equal(vals, O.sssp(ptr, adj, w, src))
+    assert st.mispredictions >= 2 and st.fallback_iteration >= 2
+    on = [r["on_plain"] for r in st.trace]
+    k = on.index(1)
+    assert all(on[k:]) and not any(on[:k])
+
+
+def test_trained_network_is_shipped_and_exact(sw, gpu):
+    """The committed scheduler network loads and keeps results exact."""
+    from subway_amd import policy
+    p = policy.default_params()
+    if p is None:
+        pytest.skip("no trained scheduler network in the package")
+    g = _rmat(sw, 16, 12)
+    ptr, adj, w = g.csr()
+    src = g.max_degree_vertex
+    vals, _ = sw.run(g, "sssp", mode="async", source=src, budget=512 << 10, policy="learned", params=p)
+    assert np.array_equal(vals, O.sssp(ptr, adj, w, src))
