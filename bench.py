@@ -7,12 +7,15 @@ edge budget capped at 1 GiB so every global iteration goes through
 subgraph generation, partitioning and host-to-device transfer.
 
 A step is one full run from the initial state to the fixpoint (first
-generate .. values back on the host), with the graph resident in pinned
-host memory.  value = input edges / step time (GTEPS).  Each timed step
-records HIP events around every advance launch (on its stream) and every
-transfer, from which the roofline entry and the achieved H2D rate are
-computed.  The CPU baseline is the OpenMP build of the same update rules
-on the same CSR, timed once on this host.
+generate .. values in the session's host result array), with the graph
+resident in pinned host memory.  value = input edges / step time (GTEPS),
+wall clock over the timed steps.  The same workload runs under the plain
+Subway policy and under the learned scheduler (the network shipped in
+subway_amd/data), each in its own session; the values of the two must be
+identical.  Each timed step records HIP events around every rounds-kernel
+launch (on its stream) and every transfer, from which the roofline entry and
+the achieved H2D rate are computed.  The CPU baseline is the OpenMP build of
+the same update rules on the same CSR, timed once on this host.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -55,6 +58,106 @@ def profiled_traffic(kernel, workload_key):
     return None
 
 
+def measure(sw, np, g, args, src, kw, names, params):
+    """Warm-up and timed steps of each policy, interleaved step by step so
+    that both see the same machine state; each run is timed on its own.
+    Returns {name: (summary, stats, values of the last step)}."""
+    sessions = {}
+    for name in names:
+        t0 = time.perf_counter()
+        sessions[name] = sw.Session(g, args.algo, policy=name, params=params if name == "learned" else None, **kw)
+        log(f"{name}: session setup {1e3 * (time.perf_counter() - t0):.1f} ms")
+    for i in range(args.warmup):
+        for name in names:
+            _, st = sessions[name].run(src, copy=False)
+            log(f"{name} warmup {i}: {st.ms_total:.1f} ms, {st.global_iterations} global iterations")
+    stats = {n: [] for n in names}
+    wall = {n: 0.0 for n in names}
+    vals = {}
+    for i in range(args.steps):
+        for name in names:
+            t0 = time.perf_counter()
+            v, st = sessions[name].run(src, copy=False)
+            wall[name] += time.perf_counter() - t0
+            vals[name] = np.array(v, copy=True)
+            stats[name].append(st)
+            log(f"{name} step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
+                f"H2D {st.h2d_bytes / 1e9:.3f} GB, rounds {st.ms_rounds:.1f} ms ({st.rounds_launches} launches), "
+                f"transfer {st.ms_transfer:.1f} ms, generate {st.ms_generate:.1f} ms, "
+                f"mispredictions {st.mispredictions}, fallback at {st.fallback_iteration}")
+    out = {}
+    for name in names:
+        ss = stats[name]
+        for r in ss[-1].trace:
+            log(f"  {name} iter {r['iteration']}: active {r['active_nodes']} v / {r['active_edges']} e, "
+                f"{r['partitions']} partitions, H2D {r['h2d_bytes'] / 1e6:.1f} MB, {r['ms']:.2f} ms, "
+                f"theta {r['theta']:.2f}, cap {r['rounds_cap']}{' (plain)' if r['on_plain'] else ''}")
+        ms_step = wall[name] / args.steps * 1e3
+        h2d = float(np.mean([s.h2d_bytes for s in ss]))
+        xfer_ms = float(np.mean([s.ms_transfer for s in ss]))
+        out[name] = ({
+            "gteps": round(g.m / (ms_step / 1e3) / 1e9, 4),
+            "ms_per_step": round(ms_step, 3),
+            "e2e_ms_library": round(float(np.mean([s.ms_total for s in ss])), 3),
+            "global_iterations": ss[-1].global_iterations,
+            "partitions": ss[-1].partitions,
+            "inner_rounds": ss[-1].inner_rounds,
+            "h2d_gb_per_step": round(h2d / 1e9, 4),
+            "h2d_mb_per_iteration": [round(r["h2d_bytes"] / 1e6, 2) for r in ss[-1].trace],
+            "h2d_gbps_achieved": round(h2d / (xfer_ms * 1e-3) / 1e9, 2) if xfer_ms > 0 else None,
+            "ms_breakdown": {"rounds": round(float(np.mean([s.ms_rounds for s in ss])), 3),
+                             "transfer": round(xfer_ms, 3),
+                             "generate": round(float(np.mean([s.ms_generate for s in ss])), 3)},
+            "mispredictions": ss[-1].mispredictions,
+            "fallback_iteration": ss[-1].fallback_iteration,
+        }, ss, vals[name])
+    for s in sessions.values():
+        s.close()
+    return out
+
+
+def roofline(np, stats, args, budget):
+    """Roofline of the dominant kernel.  Persistent async rounds: one
+    cooperative rounds_kernel launch per partition visit does every inner
+    round (frontier + advance).  Algorithmic bytes, summed over the rounds of
+    a launch: 22 B per labelled active vertex (id 4 + offset 8 + degree 4 +
+    label r/w 2 + value 4), s(E) + 4 B per relaxed edge (edge as buffered +
+    destination value), 9 B per successful update (atomic RMW 8 + label 1).
+    The frontier pass over unlabelled items is not counted.  Launch-per-round
+    mode: the advance_kernel, edges and updates only."""
+    eb = stats[-1].edge_bytes
+    rounds_ms = sum(s.ms_rounds for s in stats)
+    rounds_launches = sum(s.rounds_launches for s in stats)
+    work = sum(s.work_edges for s in stats)
+    upd = sum(s.updates for s in stats)
+    ents = sum(s.work_entries for s in stats)
+    if rounds_launches and rounds_ms > 0:
+        kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
+        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
+    else:
+        kern, k_ms = "advance_kernel", sum(s.ms_advance for s in stats)
+        k_launches = sum(s.advance_launches for s in stats)
+        alg_bytes = (eb + 4) * work + 9 * upd
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    traffic = None
+    if (kern == "rounds_kernel" and args.algo == "sssp" and args.mode == "async" and args.scale == 24
+            and args.edge_factor == 16 and budget == 1 << 30 and args.buffers == 2):
+        traffic = profiled_traffic(kern, "sssp-async rmat24 budget1GiB learned")
+    return {
+        "bound": "hbm",
+        "kernel": kern,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "launches": k_launches,
+        "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
+        "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
+        "edge_bytes": eb,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +171,7 @@ def main():
     ap.add_argument("--buffers", type=int, default=2)
     ap.add_argument("--transfer", default="auto")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--policy", default="both", choices=["both", "plain", "learned"])
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="launch the plain copy kernel once on this many MiB (counter calibration)")
@@ -81,6 +185,7 @@ def main():
     import numpy as np
 
     import subway_amd as sw
+    from subway_amd import policy
 
     if sw.device_count() < 1:
         log("no GPU visible")
@@ -92,74 +197,28 @@ def main():
                       symmetrize=(args.algo == "cc"))
     log(f"R-MAT scale {args.scale}: {g.n} vertices, {g.m} edges in {time.perf_counter() - t0:.1f} s")
     src = g.max_degree_vertex
-    edge_bytes = 8 if weighted else 4
     kw = dict(mode=args.mode, budget=budget, buffers=args.buffers, transfer=args.transfer, profile=True)
     if args.algo == "pr":
         kw["threshold"] = 1e-6
-    # device state and every pipeline buffer are built once; each step is a
-    # full run from the initial state (one query)
-    t0 = time.perf_counter()
-    sess = sw.Session(g, args.algo, **kw)
-    log(f"session setup {1e3 * (time.perf_counter() - t0):.1f} ms")
 
     if args.probe_copy_mb:
         ms = sw.probe_copy(args.probe_copy_mb << 20)
         log(f"copy probe: {args.probe_copy_mb} MiB read + written in {ms:.3f} ms")
     peak = sw.h2d_peak(256 << 20, 10)
     log(f"pinned H2D peak {peak:.1f} GB/s")
-    # results stay in the session's host value array (graph.value); the
-    # timed steps read them in place
-    for i in range(args.warmup):
-        _, st = sess.run(src, copy=False)
-        log(f"warmup {i}: {st.ms_total:.1f} ms, {st.global_iterations} global iterations")
 
-    stats = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        vals, st = sess.run(src, copy=False)
-        stats.append(st)
-        log(f"step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
-            f"H2D {st.h2d_bytes / 1e9:.2f} GB, rounds {st.ms_rounds:.1f} ms ({st.rounds_launches} launches), "
-            f"advance {st.ms_advance:.1f} ms, frontier {st.ms_frontier:.1f} ms, "
-            f"transfer {st.ms_transfer:.1f} ms, generate {st.ms_generate:.1f} ms")
-    for r in stats[-1].trace:
-        log(f"  iter {r['iteration']}: active {r['active_nodes']} v / {r['active_edges']} e, "
-            f"{r['partitions']} partitions, H2D {r['h2d_bytes'] / 1e6:.1f} MB, {r['ms']:.2f} ms")
-    wall = time.perf_counter() - t_start
-    ms_step = wall / args.steps * 1e3
-    e2e_ms = float(np.mean([s.ms_total for s in stats]))
-    value = g.m / (ms_step / 1e3) / 1e9
-
-    # Roofline of the dominant kernel.  Persistent async rounds: one
-    # cooperative rounds_kernel launch per partition visit does every inner
-    # round (frontier + advance).  Algorithmic bytes, summed over the rounds
-    # of a launch: 22 B per labelled active vertex (id 4 + offset 8 + degree
-    # 4 + label r/w 2 + value 4), s(E) + 4 B per relaxed edge (edge +
-    # destination value), 9 B per successful update (atomic RMW 8 + label 1).
-    # The frontier pass over unlabelled items is not counted.  Launch-per-
-    # round mode: the advance_kernel, edges and updates only.
-    rounds_ms = sum(s.ms_rounds for s in stats)
-    rounds_launches = sum(s.rounds_launches for s in stats)
-    work = sum(s.work_edges for s in stats)
-    upd = sum(s.updates for s in stats)
-    items = sum(s.work_items for s in stats)
-    ents = sum(s.work_entries for s in stats)
-    if rounds_launches and rounds_ms > 0:
-        kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
-        alg_bytes = 22 * ents + (edge_bytes + 4) * work + 9 * upd
-    else:
-        kern, k_ms = "advance_kernel", sum(s.ms_advance for s in stats)
-        k_launches = sum(s.advance_launches for s in stats)
-        alg_bytes = (edge_bytes + 4) * work + 9 * upd
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    traffic = None
-    if (kern == "rounds_kernel" and args.algo == "sssp" and args.mode == "async" and args.scale == 24
-            and args.edge_factor == 16 and budget == 1 << 30 and args.buffers == 2):
-        traffic = profiled_traffic(kern, "sssp-async rmat24 budget1GiB")
-    adv_ms = sum(s.ms_advance for s in stats)
-    h2d = float(np.mean([s.h2d_bytes for s in stats]))
-    xfer_ms = float(np.mean([s.ms_transfer for s in stats]))
-    h2d_rate = h2d / (xfer_ms * 1e-3) / 1e9 if xfer_ms > 0 else 0.0
+    params = policy.default_params()
+    names = ["plain", "learned"] if args.policy == "both" else [args.policy]
+    results = measure(sw, np, g, args, src, kw, names, params)
+    head = "learned" if "learned" in results else "plain"
+    out, stats, vals = results[head]
+    same = None
+    if len(results) == 2:
+        a, b = results["plain"][2], results["learned"][2]
+        same = bool(np.array_equal(a, b)) if args.algo != "pr" else bool(
+            np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-30)) < 1e-4)
+        log(f"plain vs learned: {results['plain'][0]['ms_per_step']:.2f} vs {results['learned'][0]['ms_per_step']:.2f}"
+            f" ms per run, same result: {same}")
 
     cpu = None
     if args.cpu_baseline:
@@ -177,14 +236,14 @@ def main():
                "sample": f"one full {args.algo} run (OpenMP frontier rounds, same CSR and source), {ct:.2f} s"}
         log(f"cpu baseline: {ct:.2f} s on {O.omp_threads()} threads")
 
-    out = {
+    line = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": out["gteps"],
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": out["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -194,35 +253,21 @@ def main():
             "workload": f"{args.algo}-{args.mode} on weighted R-MAT scale {args.scale} ef {args.edge_factor}, "
                         f"source = max out-degree vertex, device edge budget {budget} B",
             "vertices": g.n, "edges": g.m, "source": src, "budget_bytes": budget, "buffers": args.buffers,
-            "transfer": args.transfer, "policy": "plain",
+            "transfer": args.transfer, "policy": head,
+            "scheduler_network": "subway_amd/data/scheduler_mlp.f32" if params is not None else None,
         },
-        "e2e_ms_library": round(e2e_ms, 3),
-        "global_iterations": stats[-1].global_iterations,
-        "ms_breakdown": {"rounds": round(rounds_ms / args.steps, 3), "advance": round(adv_ms / args.steps, 3),
-                         "frontier": round(sum(s.ms_frontier for s in stats) / args.steps, 3),
-                         "transfer": round(xfer_ms, 3),
-                         "generate": round(sum(s.ms_generate for s in stats) / args.steps, 3)},
-        "inner_rounds": stats[-1].inner_rounds,
-        "partitions": stats[-1].partitions,
-        "h2d_gb_per_step": round(h2d / 1e9, 4),
-        "h2d_gbps_achieved": round(h2d_rate, 2),
+        "policies": {k: v[0] for k, v in results.items()},
+        "learned_speedup_vs_plain": (round(results["plain"][0]["ms_per_step"] / results["learned"][0]["ms_per_step"], 4)
+                                     if len(results) == 2 else None),
+        "same_result_learned_vs_plain": same,
+        "h2d_gb_per_step": out["h2d_gb_per_step"],
+        "h2d_gbps_achieved": out["h2d_gbps_achieved"],
         "h2d_gbps_peak": round(peak, 2),
-        "h2d_frac": round(h2d_rate / peak, 3) if peak else None,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kern,
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "launches": k_launches,
-            "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
-            "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
-        },
+        "h2d_frac": round(out["h2d_gbps_achieved"] / peak, 3) if out["h2d_gbps_achieved"] and peak else None,
+        "roofline": roofline(np, stats, args, budget),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

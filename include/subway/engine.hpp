@@ -88,6 +88,7 @@ struct RunStats {
     ull workItems = 0;         // partition items scanned: sum over all rounds
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
+    uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)
     ull h2dBytes = 0;          // edge bytes moved host -> device
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads

@@ -98,6 +98,8 @@ typedef struct subway_stats {
     uint64_t work_items;    /* partition items scanned by frontier passes over all rounds */
     uint32_t mispredictions;       /* learned policy: iterations whose active edges were > 2x off the prediction */
     uint32_t fallback_iteration;   /* learned policy: first iteration on the plain fallback (0 = none) */
+    uint32_t edge_bytes;           /* bytes per edge moved and buffered (4 for packed weighted edges) */
+    uint32_t pad0;
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -108,9 +110,9 @@ typedef struct subway_iter_record {
     uint64_t h2d_bytes;
     double ms_iteration;
     /* learned policy: the iteration's 12 scheduler features and 3 network heads
-     * (next active-edge fraction, whole-range density threshold, log2(round
-     * cap) / 16), the threshold and cap applied, and 1 when the plain
-     * schedule ran the iteration */
+     * (log2(next active-edge fraction) / 32, whole-range density threshold,
+     * log2(round cap) / 16), the threshold and cap applied, and 1 when the
+     * plain schedule ran the iteration */
     float features[12];
     float heads[3];
     float theta;

@@ -64,6 +64,7 @@ class _Stats(ctypes.Structure):
         ("work_entries", ctypes.c_uint64),
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
         ("work_items", ctypes.c_uint64), ("mispredictions", ctypes.c_uint32), ("fallback_iteration", ctypes.c_uint32),
+        ("edge_bytes", ctypes.c_uint32), ("pad0", ctypes.c_uint32),
     ]
 
 
@@ -192,6 +193,7 @@ class RunStats:
     work_items: int = 0
     mispredictions: int = 0
     fallback_iteration: int = 0
+    edge_bytes: int = 0
     trace: list = field(default_factory=list)
 
 

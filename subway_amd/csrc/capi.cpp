@@ -282,6 +282,7 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->work_items = s.workItems;
     stats->mispredictions = s.mispredictions;
     stats->fallback_iteration = s.fallbackIteration;
+    stats->edge_bytes = s.edgeBytes;
 }
 
 extern "C" {

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
+#include <map>
 #include <iostream>
 #include <memory>
 #include <vector>
@@ -116,6 +118,29 @@ struct Unit {
     bool whole;         // whole id range resident (learned policy)
 };
 
+// One compute and one copy stream per device, shared by every engine of the
+// process.  HIP maps streams onto a few hardware queues (four by default);
+// with two streams per session, a second session would put its copy and
+// compute streams on one queue and lose the transfer/compute overlap.
+// Runs of different sessions on one device therefore serialise on the pair.
+static void shared_streams(hipStream_t &compute, hipStream_t &copy)
+{
+    static std::mutex mu;
+    static std::map<int, std::pair<hipStream_t, hipStream_t>> pairs;
+    int dev = 0;
+    SUBWAY_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = pairs.find(dev);
+    if (it == pairs.end()) {
+        hipStream_t a = nullptr, b = nullptr;
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        it = pairs.emplace(dev, std::make_pair(a, b)).first;
+    }
+    compute = it->second.first;
+    copy = it->second.second;
+}
+
 template <class E, class G>
 class Engine {
 public:
@@ -138,8 +163,7 @@ public:
             sub_.reset(new Subgraph<E>(g_->num_nodes, g_->num_edges));
         gen_.reset(new SubgraphGenerator<E>(*g_));
         part_.reset(new Partitioner<E>());
-        SUBWAY_CHECK(hipStreamCreateWithFlags(&sc_, hipStreamNonBlocking));
-        SUBWAY_CHECK(hipStreamCreateWithFlags(&sx_, hipStreamNonBlocking));
+        shared_streams(sc_, sx_);
         gen_->stream = sc_;
         part_->stream = sc_;
         for (int b = 0; b < 2; b++) {
@@ -279,8 +303,7 @@ public:
         part_.reset();
         gen_.reset();
         sub_.reset();
-        if (sc_) (void)hipStreamDestroy(sc_);
-        if (sx_) (void)hipStreamDestroy(sx_);
+        // the streams are the process's shared pair (see shared_streams)
     }
 
     RunStats run(uint source);
@@ -353,7 +376,7 @@ private:
     float *d_params_ = nullptr;                   // scheduler network; null: fixed theta / cap
     bool onPlain_ = false;                        // learned policy fell back to the plain schedule
     float predEdges_ = -1.f;                      // last prediction of the next active-edge fraction
-    int missStreak_ = 0;
+    uint judged_ = 0;
     double theta_ = 2.0;    // density at or above which a range moves whole
     void applyPolicy(IterRecord &rec);
     void setupRanges();
@@ -523,9 +546,9 @@ void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, u
 // network's heads give the next iteration's active-edge fraction, the
 // whole-range density threshold and the inner-round cap.  The plain policy is
 // the fallback and the judge: a prediction more than 2x off the active edges
-// the iteration then shows is a misprediction, and two in a row hand the rest
-// of the run to the plain schedule.  Without parameters the options' fixed
-// theta and cap apply.
+// the iteration then shows is a misprediction, and once more than half of at
+// least three judged predictions missed, the rest of the run goes to the
+// plain schedule.  Without parameters the options' fixed theta and cap apply.
 template <class E, class G>
 void Engine<E, G>::applyPolicy(IterRecord &rec)
 {
@@ -536,18 +559,17 @@ void Engine<E, G>::applyPolicy(IterRecord &rec)
         const float actual = po.feat[2];
         if (predEdges_ >= 0.f) {
             const float r = (predEdges_ + 1e-6f) / (actual + 1e-6f);
-            if (r > 2.f || r < 0.5f) {
-                st_.mispredictions++;
-                if (++missStreak_ >= 2) {
-                    onPlain_ = true;
-                    st_.fallbackIteration = st_.globalIterations;
-                }
-            } else {
-                missStreak_ = 0;
+            judged_++;
+            if (r > 2.f || r < 0.5f) st_.mispredictions++;
+            // more than half of at least three judged predictions missed
+            if (st_.mispredictions >= 3 && 2 * st_.mispredictions > judged_) {
+                onPlain_ = true;
+                st_.fallbackIteration = st_.globalIterations;
             }
         }
         if (!onPlain_) {
-            predEdges_ = std::max(0.f, po.head[0]);
+            // head 0: log2 of the next iteration's active-edge fraction, / 32
+            predEdges_ = (float)std::exp2(32.0 * std::min(std::max((double)po.head[0], -1.0), 0.0));
             theta_ = std::min(std::max((double)po.head[1], 0.0), 2.0);
             const double c = std::exp2(std::min(std::max((double)po.head[2], 0.0), 1.0) * 16.0);
             roundsCap_ = (int)std::min<double>(std::max(1.0, std::round(c)), (double)o_.maxRounds);
@@ -1004,7 +1026,7 @@ void Engine<E, G>::resetRun()
     theta_ = o_.theta;
     onPlain_ = false;
     predEdges_ = -1.f;
-    missStreak_ = 0;
+    judged_ = 0;
     if (d_pstate_) {
         // updates counted from here on belong to this run
         SUBWAY_CHECK(hipMemcpyAsync(&d_pstate_->prevUpdates, &fr_.ctl->updates, sizeof(ull),
@@ -1141,6 +1163,7 @@ void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries
     st_.workEdges = h_ctl_->work_sum - work0;
     st_.workEntries = h_ctl_->entry_sum - entries0;
     st_.workItems = h_ctl_->item_sum - items0;
+    st_.edgeBytes = (uint)eb_;
     if (o_.profile) {
         ull c = 0;
         st_.msAdvance = pAdv_.total(st_.advanceLaunches);

@@ -6,8 +6,8 @@
 //   [0, 12)        feature mean         (inputs are standardised first)
 //   [12, 24)       feature scale (1/std)
 //   W1 12x64, b1 64, W2 64x64, b2 64, W3 64x3, b3 3   (row-major, in x out)
-// Heads: 0 = next iteration's active-edge fraction, 1 = whole-range density
-// threshold, 2 = log2 of the async inner-round cap / 16.
+// Heads: 0 = log2 of the next iteration's active-edge fraction / 32, 1 =
+// whole-range density threshold, 2 = log2 of the async inner-round cap / 16.
 #ifndef SUBWAY_MLP_HPP
 #define SUBWAY_MLP_HPP
 

@@ -8,8 +8,8 @@ Features (12, computed per global iteration from state the pipeline has):
   3 log2(1 + active edges) / log2(E)         9 global iteration index / 16
   4 fraction of active vertices, degree < 4 10 last iteration's updates / E
   5 fraction of active vertices, degree 4..31 11 fraction of id ranges with active density >= 0.5
-Heads (3): next iteration's active-edge fraction; whole-range density
-threshold; log2(inner-round cap) / 16.
+Heads (3): log2(next iteration's active-edge fraction) / 32; whole-range
+density threshold; log2(inner-round cap) / 16.
 """
 from __future__ import annotations
 

@@ -8,10 +8,12 @@ For every training configuration (algorithm, R-MAT graph, device budget) the
 async run is repeated under a grid of fixed schedules -- whole-range density
 threshold theta x inner-round cap -- with the learned policy's unit builder
 and no network, so every iteration's 12 features are recorded by the policy
-kernel.  The fastest schedule of a configuration (median end-to-end time;
-the plain-like schedule wins ties within 3%) labels heads 1 and 2 of every
-iteration of its run; head 0 (the next iteration's active-edge fraction) is
-labelled from every run.  Results are bit-exact under every schedule, so the
+kernel; the plain policy runs beside them for reference.  All sessions of a
+configuration are timed round-robin in random order.  Each decision of a configuration is labelled by its own best value (median
+end-to-end time, the other decision at its best; near-ties within 2% go to
+a fixed preference order) for every iteration of that schedule's run; head 0
+(log2 of the next iteration's active-edge fraction) is labelled from every
+run.  Results are bit-exact under every schedule, so the
 sweep only measures time.
 """
 from __future__ import annotations
@@ -30,16 +32,27 @@ from subway_amd import policy
 THETAS = (2.0, 0.8, 0.5, 0.25)     # 2.0: no range moves whole (the plain transfer choice)
 CAPS = (4096, 64, 8)
 PLAIN_LIKE = (2.0, 4096)
+# near-ties (within 2%) go to the earlier entry
+THETA_PREF = (0.8, 0.5, 2.0, 0.25)
+CAP_PREF = (64, 8, 4096)
 
 
 def configs(quick: bool):
-    """(algo, scale, seed, budget divisor) of the training set."""
-    scales = (15, 16) if quick else (16, 17, 18, 19)
+    """(algo, scale, seed, budget divisor) of the training set: the device
+    budget holds 1/div of the graph's edges (two buffers)."""
     out = []
     for algo in ("sssp", "sswp", "bfs", "pr"):
+        if quick:
+            scales = (15, 16)
+        elif algo == "pr":
+            scales = (16, 17)
+        elif algo in ("sssp", "bfs"):
+            scales = (16, 17, 18, 19, 20, 21, 22, 23)
+        else:
+            scales = (16, 17, 18, 19, 20, 21)
         for scale in scales:
-            for seed in ((1,) if quick else (1, 2)):
-                for div in (2, 6):
+            for seed in ((1,) if quick or scale >= 20 else (1, 2)):
+                for div in ((2, 6) if quick else (1, 2, 6)):
                     out.append((algo, scale, seed, div))
     return out
 
@@ -48,39 +61,55 @@ def make_graph(algo, scale, seed):
     return sw.Graph.rmat(scale, 16, seed=seed, weighted=algo in ("sssp", "sswp"))
 
 
-def run_config(g, algo, div, reps, log):
-    """Median time and the trace of each fixed schedule."""
+def run_config(g, algo, div, reps, log, seed=0):
+    """Median time and the trace of each fixed schedule, and of the plain
+    policy.  All sessions exist at once and are timed round-robin in a fresh
+    random order each round, so warm-up and clock drift touch every schedule
+    alike."""
     edge_bytes = 4    # weighted R-MAT at these scales packs into one word; unweighted edges are 4 B
     budget = max(64 << 10, (g.m * edge_bytes) // div)
     src = g.max_degree_vertex if algo != "pr" else 0
-    kw = dict(mode="async", budget=budget, policy="learned")
+    kw = dict(mode="async", budget=budget)
     if algo == "pr":
         kw["threshold"] = 1e-4
-    res = {}
+    keys = [(th, cap) for th in THETAS for cap in CAPS] + ["plain"]
+    sess = {}
+    for k in keys:
+        if k == "plain":
+            sess[k] = sw.Session(g, algo, policy="plain", **kw)
+        else:
+            sess[k] = sw.Session(g, algo, policy="learned", theta=k[0], max_rounds=k[1], **kw)
     ref = None
-    for theta in THETAS:
-        for cap in CAPS:
-            s = sw.Session(g, algo, theta=theta, max_rounds=cap, **kw)
-            times, trace = [], None
-            for r in range(reps + 1):
-                vals, st = s.run(src)
-                if r == 0:
-                    # warm-up run; the values must not depend on the schedule
-                    if ref is None:
-                        ref = vals
-                    elif algo != "pr" and not np.array_equal(ref, vals):
-                        raise RuntimeError(f"{algo}: schedule {theta}/{cap} changed the result")
-                    continue
-                times.append(st.ms_total)
-                trace = st.trace
-            s.close()
-            res[(theta, cap)] = (float(np.median(times)), trace)
-    best = min(res, key=lambda k: res[k][0])
-    if res[PLAIN_LIKE][0] <= 1.03 * res[best][0]:
-        best = PLAIN_LIKE
-    log(f"  {algo} n={g.n} m={g.m} budget={budget}: plain-like {res[PLAIN_LIKE][0]:.2f} ms, "
-        f"best {best} {res[best][0]:.2f} ms")
-    return res, best
+    times = {k: [] for k in keys}
+    traces = {}
+    rng = np.random.default_rng(seed)
+    for r in range(reps + 2):
+        order = list(keys)
+        rng.shuffle(order)
+        for k in order:
+            vals, st = sess[k].run(src)
+            if r < 2:
+                # warm-up rounds; the values must not depend on the schedule
+                if ref is None:
+                    ref = np.array(vals, copy=True)
+                elif algo != "pr" and not np.array_equal(ref, vals):
+                    raise RuntimeError(f"{algo}: schedule {k} changed the result")
+                continue
+            times[k].append(st.ms_total)
+            traces[k] = st.trace
+    for s in sess.values():
+        s.close()
+    res = {k: (float(np.median(times[k])), traces[k]) for k in keys if k != "plain"}
+    plain_ms = float(np.median(times["plain"]))
+    # each decision labelled by its own best (the other decision at its best)
+    t_theta = {th: min(res[(th, c)][0] for c in CAPS) for th in THETAS}
+    t_cap = {c: min(res[(th, c)][0] for th in THETAS) for c in CAPS}
+    th_best = next(th for th in THETA_PREF if t_theta[th] <= 1.02 * min(t_theta.values()))
+    cap_best = next(c for c in CAP_PREF if t_cap[c] <= 1.02 * min(t_cap.values()))
+    best = (th_best, cap_best)
+    log(f"  {algo} n={g.n} m={g.m} budget={budget}: plain {plain_ms:.2f} ms, plain-like {res[PLAIN_LIKE][0]:.2f} ms, "
+        f"label {best} {res[best][0]:.2f} ms, fastest {min(r[0] for r in res.values()):.2f} ms")
+    return res, best, plain_ms
 
 
 def dataset(results):
@@ -89,7 +118,7 @@ def dataset(results):
         for sched, (_, trace) in res.items():
             for t, rec in enumerate(trace):
                 nxt = trace[t + 1]["features"][2] if t + 1 < len(trace) else 0.0
-                y = [nxt, best[0], np.log2(best[1]) / 16.0]
+                y = [max(np.log2(nxt + 2.0 ** -30), -30.0) / 32.0, best[0], np.log2(best[1]) / 16.0]
                 m = [1.0, 1.0, 1.0] if sched == best else [1.0, 0.0, 0.0]
                 xs.append(rec["features"])
                 ys.append(y)
@@ -105,6 +134,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--save-data", default="", help="also write the training samples to this .npz")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -114,12 +144,14 @@ def main(argv=None):
     results, summary = [], []
     for algo, scale, seed, div in configs(args.quick):
         g = make_graph(algo, scale, seed)
-        res, best = run_config(g, algo, div, args.reps, log)
+        res, best, plain_ms = run_config(g, algo, div, args.reps, log, seed=len(results))
         results.append((res, best))
-        summary.append({"algo": algo, "scale": scale, "seed": seed, "budget_divisor": div,
+        summary.append({"algo": algo, "scale": scale, "seed": seed, "budget_divisor": div, "plain_ms": plain_ms,
                         "best": list(best), "ms": {f"{k[0]}/{k[1]}": v[0] for k, v in res.items()}})
     x, y, mask = dataset(results)
     log(f"{len(x)} samples from {len(results)} configurations in {time.time() - t0:.0f} s")
+    if args.save_data:
+        np.savez(args.save_data, x=x, y=y, mask=mask)
     mean, std = x.mean(0), x.std(0)
     p0 = policy.init_params(args.seed, mean, std)
     params, losses = policy.train(p0, x, y, mask, steps=args.steps, lr=args.lr)
