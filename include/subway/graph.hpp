@@ -116,7 +116,7 @@ public:
     float *delta;
     uint *d_outDegree;
     float *d_value;
-    float *d_delta;
+    double *d_delta;   // FP64 residual: an FP32 sum at a hub rounds away shares below half its ulp
     ull *d_nodePointer;
     E *d_edgeList;
     std::string graphFormat;

@@ -194,6 +194,7 @@ public:
         SUBWAY_CHECK(hipHostMalloc((void **)&h_err_, 4 * sizeof(uint32_t), 0));
         std::memset(h_err_, 0, 4 * sizeof(uint32_t));
         fr_.spinLimit = o_.spinLimit ? o_.spinLimit : dev::SPIN_LIMIT_DEFAULT;
+        if constexpr (isPR) SUBWAY_CHECK(hipMalloc(&d_sum_, (size_t)g_->num_nodes * sizeof(double)));
         gen_->scanScratch().spinLimit = fr_.spinLimit;
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
@@ -291,6 +292,7 @@ public:
         if (chunkMap_) (void)hipFree(chunkMap_);
         if (h_ctl_) (void)hipHostFree(h_ctl_);
         if (h_err_) (void)hipHostFree(h_err_);
+        if (d_sum_) (void)hipFree(d_sum_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
             (void)hipEventDestroy(evLoaded_[b]);
@@ -323,6 +325,7 @@ private:
     uint32_t *chunkMap_ = nullptr;
     ScanCtl *h_ctl_ = nullptr;
     uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
+    double *d_sum_ = nullptr;     // PR: FP64 running sum of each vertex's absorbed residual
     bool broken_ = false;         // a device wait gave up: the device state is no longer trusted
     ull maxPart_ = 0;       // edges per device buffer (in the buffer's edge format)
     int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
@@ -408,7 +411,7 @@ void Engine<E, G>::initState()
 {
     const uint n = g_->num_nodes;
     if constexpr (isPR) {
-        launch_init_pr_state(n, g_->d_value, g_->d_delta, o_.initPR, sc_);
+        launch_init_pr_state(n, d_sum_, g_->d_delta, o_.initPR, sc_);
     } else {
         launch_init_label_state(o_.algo, o_.mode == ASYNC, n, g_->d_value, (uint8_t *)g_->d_label1,
                                 (uint8_t *)g_->d_label2, src_, sc_);
@@ -836,8 +839,8 @@ void Engine<E, G>::frontier(int first, uint8_t *labA, uint8_t *labB, const Unit 
     a.labA = labA;
     a.labB = labB;
     if constexpr (isPR) {
-        a.value = (uint32_t *)g_->d_value;
         a.delta = g_->d_delta;
+        a.sum = d_sum_;
         a.acc = o_.acc;
     } else {
         a.value = g_->d_value;
@@ -896,8 +899,8 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.l1 = l1;
         l.l2 = l2;
         if constexpr (isPR) {
-            l.value = (uint32_t *)g_->d_value;
             l.delta = g_->d_delta;
+            l.sum = d_sum_;
             l.acc = o_.acc;
         } else {
             l.value = g_->d_value;
@@ -1104,9 +1107,10 @@ RunStats Engine<E, G>::run(uint source)
         }
     }
     loop(t0);
-    if constexpr (isPR)
+    if constexpr (isPR) {
+        launch_finish_pr(n, g_->d_value, d_sum_, sc_);
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, sc_));
-    else
+    } else
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);

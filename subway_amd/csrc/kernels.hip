@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <stdexcept>
 
 #include "subway/globals.hpp"
@@ -74,30 +75,6 @@ __device__ __forceinline__ bool scan_tile(const Op &op, u64 n, u32 tile, u32 epo
     if (lane == WAVE - 1) {
         s_wcnt[wid] = icnt;
         s_wsum[wid] = isum;
-    }
-    if constexpr (Op::TALLY) {
-        // degree histogram of the selected items, one atomic per bucket and tile
-        __shared__ u32 s_hist[NW][4];
-        u32 h[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int j = 0; j < IPT; j++) {
-            if ((mask >> j) & 1u) {
-                const u64 d = w[j];
-                h[d < 4 ? 0 : d < 32 ? 1 : d < 256 ? 2 : 3]++;
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const u32 t = wave_reduce(h[b]);
-            if (lane == 0) s_hist[wid][b] = t;
-        }
-        __syncthreads();
-        if (tid < 4) {
-            u32 t = 0;
-#pragma unroll
-            for (int q = 0; q < NW; q++) t += s_hist[q][tid];
-            if (t) atomicAdd((unsigned long long *)&ctl->hist[tid], (unsigned long long)t);
-        }
     }
     __syncthreads();
     u32 wbc = 0, totc = 0;
@@ -176,6 +153,177 @@ __global__ __launch_bounds__(BLOCK) void select_scan_kernel(Op op, u64 n, ScanCt
     }
 }
 
+// ================================================ generator: reduce-then-scan
+// The generator's selection runs in three wait-free launches: count (each
+// workgroup reduces its 16384-vertex chunk to a selected count and degree
+// sum, and tallies the degree histogram), scan (one workgroup turns the
+// chunk totals into exclusive prefixes and publishes the totals through
+// op.finish), emit (each workgroup re-reads its chunk, scans it and
+// compacts from its chunk's prefix).  No workgroup waits for another, so the
+// time does not grow with a chain of inter-tile waits; the second read of
+// the labels and degrees mostly hits L2.
+
+// Exclusive workgroup scan of (count, sum) over SCAN_BLOCK threads; also
+// returns the workgroup totals.  Ends with a barrier, so it may be called
+// again right away.
+__device__ __forceinline__ void block_excl_scan2(u32 cnt, u64 sum, u32 &xc, u64 &xs, u32 &tc, u64 &ts)
+{
+    constexpr int NW = SCAN_BLOCK / WAVE;
+    __shared__ u32 s_c[NW];
+    __shared__ u64 s_s[NW];
+    const int wid = threadIdx.x / WAVE, lane = lane_id();
+    const u32 ic = wave_incl_scan(cnt);
+    const u64 is = wave_incl_scan(sum);
+    if (lane == WAVE - 1) {
+        s_c[wid] = ic;
+        s_s[wid] = is;
+    }
+    __syncthreads();
+    u32 bc = 0, totc = 0;
+    u64 bs = 0, tots = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) {
+        if (q < wid) {
+            bc += s_c[q];
+            bs += s_s[q];
+        }
+        totc += s_c[q];
+        tots += s_s[q];
+    }
+    xc = bc + ic - cnt;
+    xs = bs + is - sum;
+    tc = totc;
+    ts = tots;
+    __syncthreads();
+}
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void select_count_kernel(Op op, u64 n, u64 *aggCnt, u64 *aggSum,
+                                                                  ScanCtl *ctl)
+{
+    static_assert(Op::TALLY, "the count pass re-reads its items; it is for the generator's side-effect-free ops");
+    const u64 base = (u64)blockIdx.x * GEN_CHUNK;
+    u32 cnt = 0;
+    u64 sum = 0;
+    u32 h[4] = {0u, 0u, 0u, 0u};
+#pragma unroll 1
+    for (int r = 0; r < GEN_SUB; r++) {
+        const u64 i0 = base + (u64)r * GEN_TILE + (u64)threadIdx.x * GEN_IPT;
+        if (i0 >= n) break;
+        typename Op::Item it[GEN_IPT];
+        u64 w[GEN_IPT];
+        u32 mask = 0;
+        op.load(i0, n, it, w, mask);
+        cnt += __popc(mask);
+#pragma unroll
+        for (int j = 0; j < GEN_IPT; j++) {
+            if ((mask >> j) & 1u) {
+                const u64 d = w[j];
+                sum += d;
+                h[d < 4 ? 0 : d < 32 ? 1 : d < 256 ? 2 : 3]++;
+            }
+        }
+    }
+    u32 xc, tc;
+    u64 xs, ts;
+    block_excl_scan2(cnt, sum, xc, xs, tc, ts);
+    __shared__ u32 s_h[4];
+    if (threadIdx.x < 4) s_h[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const u32 t = wave_reduce(h[b]);
+        if (lane_id() == 0 && t) atomicAdd(&s_h[b], t);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        aggCnt[blockIdx.x] = tc;
+        aggSum[blockIdx.x] = ts;
+    }
+    if (threadIdx.x < 4 && s_h[threadIdx.x])
+        atomicAdd((unsigned long long *)&ctl->hist[threadIdx.x], (unsigned long long)s_h[threadIdx.x]);
+}
+
+// One workgroup of 1024 threads: exclusive prefixes of the G chunk totals in
+// place, then op.finish with the grand totals.
+template <class Op>
+__global__ __launch_bounds__(1024) void select_scan_chunks_kernel(Op op, u64 G, u64 *aggCnt, u64 *aggSum,
+                                                                  ScanCtl *ctl)
+{
+    constexpr int T = 1024, NW = T / WAVE;
+    __shared__ u64 s_c[NW], s_s[NW];
+    const int tid = threadIdx.x, wid = tid / WAVE, lane = lane_id();
+    const u64 per = (G + T - 1) / T;
+    const u64 b0 = (u64)tid * per < G ? (u64)tid * per : G;
+    const u64 b1 = b0 + per < G ? b0 + per : G;
+    u64 c = 0, s = 0;
+    for (u64 b = b0; b < b1; b++) {
+        c += aggCnt[b];
+        s += aggSum[b];
+    }
+    const u64 ic = wave_incl_scan(c), is = wave_incl_scan(s);
+    if (lane == WAVE - 1) {
+        s_c[wid] = ic;
+        s_s[wid] = is;
+    }
+    __syncthreads();
+    u64 bc = 0, bs = 0, tc = 0, ts = 0;
+    for (int q = 0; q < NW; q++) {
+        if (q < wid) {
+            bc += s_c[q];
+            bs += s_s[q];
+        }
+        tc += s_c[q];
+        ts += s_s[q];
+    }
+    u64 xc = bc + ic - c, xs = bs + is - s;
+    for (u64 b = b0; b < b1; b++) {
+        const u64 cb = aggCnt[b], sb = aggSum[b];
+        aggCnt[b] = xc;
+        aggSum[b] = xs;
+        xc += cb;
+        xs += sb;
+    }
+    if (tid == 0) op.finish(tc, ts, ctl);
+}
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void select_emit_kernel(Op op, u64 n, const u64 *aggCnt,
+                                                                 const u64 *aggSum)
+{
+    const u64 base = (u64)blockIdx.x * GEN_CHUNK;
+    u64 k0 = aggCnt[blockIdx.x], s0 = aggSum[blockIdx.x];
+#pragma unroll 1
+    for (int r = 0; r < GEN_SUB; r++) {
+        const u64 t0 = base + (u64)r * GEN_TILE;
+        if (t0 >= n) break;   // uniform across the workgroup
+        const u64 i0 = t0 + (u64)threadIdx.x * GEN_IPT;
+        typename Op::Item it[GEN_IPT];
+        u64 w[GEN_IPT];
+        u32 mask = 0;
+        op.load(i0, n, it, w, mask);
+        const u32 cnt = __popc(mask);
+        u64 sum = 0;
+#pragma unroll
+        for (int j = 0; j < GEN_IPT; j++)
+            if ((mask >> j) & 1u) sum += w[j];
+        u32 xc, tc;
+        u64 xs, ts;
+        block_excl_scan2(cnt, sum, xc, xs, tc, ts);
+        u64 k = k0 + xc, pre = s0 + xs;
+#pragma unroll
+        for (int j = 0; j < GEN_IPT; j++) {
+            if ((mask >> j) & 1u) {
+                op.emit(k, pre, i0 + j, it[j], w[j]);
+                k++;
+                pre += w[j];
+            }
+        }
+        k0 += tc;
+        s0 += ts;
+    }
+}
+
 // ------------------------------------------------------- generator ops
 struct Empty {};
 
@@ -233,12 +381,15 @@ struct GenLabelOp {
     }
 };
 
-// activeness = delta > acc (shared/subgraph_generator.cu:24-41)
+// activeness = delta > acc (shared/subgraph_generator.cu:24-41); the
+// residual is FP32 (the stand-alone generator over a caller's array) or FP64
+// (the pipeline's own residual)
+template <class T>
 struct GenPrOp {
     typedef Empty Item;
     static constexpr bool TALLY = true;
-    const float *delta;
-    float acc;
+    const T *delta;
+    T acc;
     const u32 *outDegree;
     u32 *activeNodes;
     u64 *activeNodesPointer;
@@ -246,13 +397,16 @@ struct GenPrOp {
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *, u64 *w, u32 &mask) const
     {
         if (i0 + GEN_IPT <= n) {
+            // 16-byte loads: 4 floats or 2 doubles each
+            constexpr int PER = 16 / sizeof(T);
+            typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
 #pragma unroll
-            for (int q = 0; q < GEN_IPT / 4; q++) {
-                float4 d = *reinterpret_cast<const float4 *>(delta + i0 + 4 * q);
-                if (d.x > acc) mask |= 1u << (4 * q + 0);
-                if (d.y > acc) mask |= 1u << (4 * q + 1);
-                if (d.z > acc) mask |= 1u << (4 * q + 2);
-                if (d.w > acc) mask |= 1u << (4 * q + 3);
+            for (int q = 0; q < GEN_IPT / PER; q++) {
+                const V d = *reinterpret_cast<const V *>(delta + i0 + PER * q);
+                const T *e = reinterpret_cast<const T *>(&d);
+#pragma unroll
+                for (int r = 0; r < PER; r++)
+                    if (e[r] > acc) mask |= 1u << (PER * q + r);
             }
             if (mask) {
 #pragma unroll
@@ -393,9 +547,9 @@ struct FrontierPrOp {
     const u32 *nodes;
     const u64 *ptr;
     u64 fromEdge;
-    float *value;
-    float *delta;
-    float acc;
+    double *sum;     // running value sum (FP64)
+    double *delta;   // residual (FP64)
+    double acc;
     Entry *entries;
     u32 *chunkMap;
     u32 idBase;
@@ -407,13 +561,13 @@ struct FrontierPrOp {
             u64 i = i0 + j;
             if (i < n) {
                 u32 id = nodes ? nodes[i] : idBase + (u32)i;
-                float d = mld<COH>(&delta[id]);
+                const double d = mld<COH>(&delta[id]);
                 if (d > acc) {
-                    mst<COH>(&value[id], mld<COH>(&value[id]) + d);
-                    mst<COH>(&delta[id], 0.0f);
+                    mst<COH>(&sum[id], mld<COH>(&sum[id]) + d);
+                    mst<COH>(&delta[id], 0.0);
                     u64 p0 = ptr[i], p1 = ptr[i + 1];
                     if (p1 > p0) {
-                        float share = (float)((double)(d / (float)(p1 - p0)) * 0.85);
+                        const float share = (float)(d / (double)(p1 - p0) * 0.85);
                         mask |= 1u << j;
                         w[j] = p1 - p0;
                         it[j].loc = (u32)(p0 - fromEdge);
@@ -544,7 +698,7 @@ struct Relax;
 template <>
 struct Relax<ALGO_BFS> {
     template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, float *)
+    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, double *)
     {
         u32 cand = val + 1;
         u32 v = e.end;
@@ -562,7 +716,7 @@ struct Relax<ALGO_BFS> {
 template <>
 struct Relax<ALGO_CC> {
     template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, float *)
+    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, double *)
     {
         u32 v = e.end;
         if (val < value[v]) {
@@ -580,7 +734,7 @@ template <>
 struct Relax<ALGO_SSSP> {
     template <bool COH>
     __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
-                                                 uint8_t *lab, float *)
+                                                 uint8_t *lab, double *)
     {
         u32 cand = val + e.w8;
         u32 v = e.end;
@@ -599,7 +753,7 @@ template <>
 struct Relax<ALGO_SSWP> {
     template <bool COH>
     __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
-                                                 uint8_t *lab, float *)
+                                                 uint8_t *lab, double *)
     {
         u32 cand = val < e.w8 ? val : e.w8;
         u32 v = e.end;
@@ -617,9 +771,11 @@ struct Relax<ALGO_SSWP> {
 template <>
 struct Relax<ALGO_PR> {
     template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *, uint8_t *, float *delta)
+    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *, uint8_t *, double *delta)
     {
-        unsafeAtomicAdd(&delta[e.end], __uint_as_float(val));
+        // FP64 residual: an FP32 sum at a hub rounds away shares below half
+        // its ulp, a systematic loss of rank mass
+        unsafeAtomicAdd(&delta[e.end], (double)__uint_as_float(val));
         return true;
     }
 };
@@ -656,7 +812,7 @@ struct Codec<PackedEdge> {
 // calling thread.
 template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
-                                             u32 *value, uint8_t *lab, float *delta, u64 count, u64 W, u64 w0,
+                                             u32 *value, uint8_t *lab, double *delta, u64 count, u64 W, u64 w0,
                                              u64 wend, u64 shift, u32 psh, u32 *err)
 {
     __shared__ u32 s_pre[ADV_CHUNK + 1];
@@ -698,7 +854,7 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
 template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries, const u32 *chunkMap,
                                                             const E *edges, u32 *value, uint8_t *lab,
-                                                            float *delta, u64 w0, u64 w1, u64 shift,
+                                                            double *delta, u64 w0, u64 w1, u64 shift,
                                                             u32 psh, ScanCtl *ctl)
 {
     __shared__ u32 s_chunk;
@@ -798,9 +954,10 @@ struct RoundsArgs {
     const ull *ptr;        // activeNodesPointer + fromNode
     ull fromEdge;
     uint8_t *l1, *l2;      // label arrays (label algorithms)
-    u32 *value;            // u32 values or PR float values
-    float *delta;          // PR
-    float acc;             // PR
+    u32 *value;            // u32 values (label algorithms)
+    double *delta;         // PR residual
+    double *sum;           // PR running value sum
+    double acc;            // PR
     Entry *entries;
     u32 *chunkMap;
     const void *edges;     // device partition buffer
@@ -848,8 +1005,8 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             __syncthreads();
             if (t >= numTiles) break;
             if constexpr (ALGO == ALGO_PR) {
-                FrontierPrOp<true> op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
-                                a.idBase};
+                FrontierPrOp<true> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc,
+                                      a.entries, a.chunkMap, a.idBase};
                 if (!scan_tile<FrontierPrOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                  wt))
                     break;
@@ -1413,13 +1570,21 @@ __global__ void init_label_state_kernel(int algo, int async, u64 n, u32 *value, 
     }
 }
 
-// PageRank: value 0, delta = initPR (subway/pr-sync.cu:33-40)
-__global__ void init_pr_state_kernel(u64 n, float *value, float *delta, float init)
+// PageRank: value 0, delta = initPR (subway/pr-sync.cu:33-40); the running
+// sum and the residual are FP64
+__global__ void init_pr_state_kernel(u64 n, double *sum, double *delta, double init)
 {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
-        value[i] = 0.0f;
+        sum[i] = 0.0;
         delta[i] = init;
     }
+}
+
+// FP32 result values from the FP64 running sums
+__global__ void finish_pr_kernel(u64 n, float *value, const double *sum)
+{
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+        value[i] = (float)sum[i];
 }
 
 // Plain streaming copy (16 B per lane and iteration); used to calibrate
@@ -1514,30 +1679,45 @@ static void check_tiles(ScanScratch &s, u64 tiles, hipStream_t st)
     if (tiles > 0x7FFFFFFFull) throw Error(ERR_ARG, "scan launch too large");
 }
 
+// Generator selection: count, scan of the chunk totals, emit (see above).
+template <class Op>
+static void run_generator(const Op &op, ull n, ScanScratch &s, hipStream_t st)
+{
+    if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
+    const u64 G = (n + GEN_CHUNK - 1) / GEN_CHUNK;
+    if (G > s.maxTiles) throw Error(ERR_INTERNAL, "generator scratch too small");
+    if (G > 0x7FFFFFFFull) throw Error(ERR_ARG, "generate too large");
+    u64 *aggCnt = s.td.flag, *aggSum = s.td.inc;
+    hipLaunchKernelGGL((select_count_kernel<Op>), dim3((unsigned)G), dim3(SCAN_BLOCK), 0, st, op, (u64)n, aggCnt,
+                       aggSum, s.ctl);
+    hipLaunchKernelGGL((select_scan_chunks_kernel<Op>), dim3(1), dim3(1024), 0, st, op, G, aggCnt, aggSum, s.ctl);
+    hipLaunchKernelGGL((select_emit_kernel<Op>), dim3((unsigned)G), dim3(SCAN_BLOCK), 0, st, op, (u64)n, aggCnt,
+                       aggSum);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
 void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
                             const uint32_t *outDegree, uint32_t *activeNodes,
                             ull *activeNodesPointer, ScanScratch &s, hipStream_t st)
 {
-    if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
-    u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles, st);
     GenLabelOp op{l1, l2, outDegree, activeNodes, activeNodesPointer};
-    hipLaunchKernelGGL((select_scan_kernel<GenLabelOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
-    SUBWAY_CHECK(hipGetLastError());
+    run_generator(op, n, s, st);
 }
 
 void launch_generate_pr(ull n, const float *delta, float acc, const uint32_t *outDegree,
                         uint32_t *activeNodes, ull *activeNodesPointer, ScanScratch &s,
                         hipStream_t st)
 {
-    if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
-    u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles, st);
-    GenPrOp op{delta, acc, outDegree, activeNodes, activeNodesPointer};
-    hipLaunchKernelGGL((select_scan_kernel<GenPrOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
-    SUBWAY_CHECK(hipGetLastError());
+    GenPrOp<float> op{delta, acc, outDegree, activeNodes, activeNodesPointer};
+    run_generator(op, n, s, st);
+}
+
+void launch_generate_pr(ull n, const double *delta, double acc, const uint32_t *outDegree,
+                        uint32_t *activeNodes, ull *activeNodesPointer, ScanScratch &s,
+                        hipStream_t st)
+{
+    GenPrOp<double> op{delta, acc, outDegree, activeNodes, activeNodesPointer};
+    run_generator(op, n, s, st);
 }
 
 void launch_partition(const ull *activeNodesPointer, const uint32_t *activeNodes,
@@ -1578,8 +1758,8 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
     u64 tiles = (a.n + FR_TILE - 1) / FR_TILE;
     check_tiles(s, tiles, st);
     if (a.algo == ALGO_PR) {
-        FrontierPrOp<false> op{a.nodes, a.ptr, a.fromEdge, (float *)a.value, a.delta, a.acc, a.entries, a.chunkMap,
-                        a.idBase};
+        FrontierPrOp<false> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc, a.entries,
+                               a.chunkMap, a.idBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                            dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first,
                            s.spinLimit);
@@ -1638,10 +1818,16 @@ void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_
     SUBWAY_CHECK(hipGetLastError());
 }
 
-void launch_init_pr_state(ull n, float *value, float *delta, float init, hipStream_t st)
+void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStream_t st)
 {
-    hipLaunchKernelGGL(init_pr_state_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value,
+    hipLaunchKernelGGL(init_pr_state_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, sum,
                        delta, init);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_finish_pr(ull n, float *value, const double *sum, hipStream_t st)
+{
+    hipLaunchKernelGGL(finish_pr_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value, sum);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1711,6 +1897,7 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.l2 = l.l2;
     a.value = l.value;
     a.delta = l.delta;
+    a.sum = l.sum;
     a.acc = l.acc;
     a.entries = l.entries;
     a.chunkMap = l.chunkMap;
@@ -1732,13 +1919,8 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
 void launch_generate_masks(ull n, const ull *m1, const ull *m2, const uint32_t *outDegree, uint32_t *activeNodes,
                            ull *activeNodesPointer, ScanScratch &s, hipStream_t st)
 {
-    if (n == 0) throw Error(ERR_ARG, "generate on an empty graph");
-    u64 tiles = (n + GEN_TILE - 1) / GEN_TILE;
-    check_tiles(s, tiles, st);
     GenMaskOp op{m1, m2, outDegree, activeNodes, activeNodesPointer};
-    hipLaunchKernelGGL((select_scan_kernel<GenMaskOp, SCAN_BLOCK, GEN_IPT>), dim3((unsigned)tiles),
-                       dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
-    SUBWAY_CHECK(hipGetLastError());
+    run_generator(op, n, s, st);
 }
 
 void launch_init_batch_state(int algo, ull n, uint32_t *dist, ull *m1, ull *m2, int allActive, hipStream_t st)

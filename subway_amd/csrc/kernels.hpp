@@ -22,6 +22,8 @@ enum Algo { ALGO_BFS = 0, ALGO_CC = 1, ALGO_SSSP = 2, ALGO_SSWP = 3, ALGO_PR = 4
 constexpr int SCAN_BLOCK = 256;
 constexpr int GEN_IPT = 16;                            // vertices per thread
 constexpr int GEN_TILE = SCAN_BLOCK * GEN_IPT;         // 4096 vertices per tile
+constexpr int GEN_SUB = 4;                              // tiles per generator chunk
+constexpr int GEN_CHUNK = GEN_TILE * GEN_SUB;           // 16384 vertices per workgroup of the generator
 constexpr int FR_IPT = 8;
 constexpr int FR_TILE = SCAN_BLOCK * FR_IPT;           // 2048 active-list items per tile
 // Advance kernel: work space is cut into chunks of ADV_CHUNK edges.
@@ -93,6 +95,9 @@ void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
 void launch_generate_pr(ull n, const float *delta, float acc,
                         const uint32_t *outDegree, uint32_t *activeNodes,
                         ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
+void launch_generate_pr(ull n, const double *delta, double acc,
+                        const uint32_t *outDegree, uint32_t *activeNodes,
+                        ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
 
 // ---- partitioner: greedy contiguous ranges of the active list
 // Writes at most maxRows rows starting at active index `start`; out[0] =
@@ -143,7 +148,7 @@ void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *active
 
 // ---- per-round frontier (filter) kernel
 // Label algorithms: an item is selected when labA (or labB) is set; both are
-// cleared.  PR: selected when delta > acc; value absorbs delta.
+// cleared.  PR: selected when delta > acc; the running sum absorbs delta.
 struct FrontierArgs {
     int algo;
     ull n;                      // items in the partition
@@ -152,9 +157,10 @@ struct FrontierArgs {
     ull fromEdge;               // subtracted from ptr to get buffer offsets
     uint8_t *labA;
     uint8_t *labB;                   // may be null
-    uint32_t *value;                 // u32 value (label algos) or float bits (PR)
-    float *delta;                    // PR only
-    float acc;                       // PR only
+    uint32_t *value;                 // u32 value (label algos)
+    double *delta;                   // PR only: residual
+    double *sum;                     // PR only: running value sum
+    double acc;                      // PR only
     Entry *entries;
     uint32_t *chunkMap;
     int first;                       // 1: ignore the stop flag of the previous launch
@@ -172,7 +178,7 @@ struct AdvanceArgs {
     const void *edges;               // device edge buffer
     uint32_t *value;                 // u32 values; PR: unused
     uint8_t *labelOut;               // label algos
-    float *delta;                    // PR
+    double *delta;                   // PR residual
     ull w0, w1;                 // work window [w0, w1); w1 = ~0 for all
     ull shift;                  // subtracted from edge indices
     int grid;                        // persistent workgroups
@@ -187,8 +193,9 @@ struct RoundsLaunch {
     ull fromEdge;
     uint8_t *l1, *l2;
     uint32_t *value;
-    float *delta;
-    float acc;
+    double *delta;         // PR residual
+    double *sum;           // PR running value sum
+    double acc;
     Entry *entries;
     uint32_t *chunkMap;
     const void *edges;
@@ -242,7 +249,11 @@ void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l
 // Initial vertex state (values and both label arrays / PR value and delta).
 void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
                              uint32_t source, hipStream_t st);
-void launch_init_pr_state(ull n, float *value, float *delta, float init, hipStream_t st);
+// PR: the running sum and the residual are FP64 (an FP32 residual at a hub
+// rounds away shares below half its ulp).
+void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStream_t st);
+// PR result: FP32 values from the running sums
+void launch_finish_pr(ull n, float *value, const double *sum, hipStream_t st);
 
 // Streaming copy of `bytes` (multiple of 16) for counter calibration.
 void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st);

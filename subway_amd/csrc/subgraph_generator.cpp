@@ -50,7 +50,7 @@ void SubgraphGenerator<E>::enqueue(Graph<E> &graph, Subgraph<E> &subgraph)
 template <class E>
 void SubgraphGenerator<E>::enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc)
 {
-    launch_generate_pr(graph.num_nodes, graph.d_delta, acc, graph.d_outDegree, subgraph.d_activeNodes,
+    launch_generate_pr(graph.num_nodes, graph.d_delta, (double)acc, graph.d_outDegree, subgraph.d_activeNodes,
                        subgraph.d_activeNodesPointer, *scratch, stream);
 }
 

@@ -250,6 +250,28 @@ def test_pr_rmat_tolerance(sw, gpu):
     check_app(sw, g, "pr", "sync", threshold=1e-6)
 
 
+@pytest.mark.parametrize("max_rounds", [4096, 8])
+def test_pr_async_keeps_hub_mass(sw, gpu, max_rounds):
+    """Async PR with partitions run to local convergence (thousands of tiny
+    pushes into a hub that is not resident) keeps the rank mass: the error
+    against the converged FP64 ranks and the missing mass stay those of the
+    FP64 oracle at the same threshold (about 3e-6), far inside the 1e-4
+    bound.  An FP32 residual rounds away shares below half the hub's ulp
+    here and misses by 8e-5 at the hub."""
+    g = sw.Graph.rmat(15, 16, seed=21)
+    ptr, adj, _ = g.csr()
+    exact, _ = O.pr(ptr, adj, 1e-10)
+    at_tau, _ = O.pr(ptr, adj, 1e-6)
+    v, _ = sw.run(g, "pr", mode="async", budget=g.m * 4, threshold=1e-6, max_rounds=max_rounds)
+    v = v.astype(np.float64)
+    own = np.abs(at_tau - exact) / exact
+    err = np.abs(v - exact) / exact
+    assert err.max() < 2 * own.max() + 1e-6, (err.max(), own.max())
+    deficit = (exact.sum() - v.sum()) / exact.sum()
+    oracle_deficit = (exact.sum() - at_tau.sum()) / exact.sum()
+    assert abs(deficit) < 2 * abs(oracle_deficit) + 1e-6, (deficit, oracle_deficit)
+
+
 @pytest.mark.parametrize("algo", ["sssp", "sswp", "bfs", "cc"])
 @pytest.mark.parametrize("mode", ["sync", "async"])
 def test_packed_edges_match_unpacked(sw, gpu, algo, mode):
