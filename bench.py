@@ -17,6 +17,14 @@ launch (on its stream) and every transfer, from which the roofline entry and
 the achieved H2D rate are computed.  The CPU baseline is the OpenMP build of
 the same update rules on the same CSR, timed once on this host.
 
+With --batch K (K <= 64) the workload is K-source batched SSSP instead: K
+sources drawn uniformly (seeded) from the vertices with out-degree >= 1
+advance together, one cooperative kernel per transferred partition, after
+the autotuner has picked the kernel's unroll; a step is one batched run and
+value = K * input edges / step time.  The same K sources then run one after
+another as single-source SSSP-async sessions (plain policy, same budget);
+the rows must be identical and the line reports the speedup.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -116,6 +124,114 @@ def measure(sw, np, g, args, src, kw, names, params):
     return out
 
 
+def batch_bench(sw, np, g, args, budget, peak):
+    """K-source batched SSSP against K sequential single-source runs."""
+    k = args.batch
+    deg = g.out_degree()
+    cand = np.flatnonzero(deg >= 1)
+    rng = np.random.default_rng(args.seed)
+    sources = np.sort(rng.choice(cand, size=k, replace=False)).astype(np.uint32)
+    bs = sw.BatchSession(g, args.algo, budget=budget, buffers=args.buffers, transfer=args.transfer, profile=True)
+    t0 = time.perf_counter()
+    unroll, tune_ms = bs.tune(sources)
+    log(f"autotuner: unroll {unroll} (runs {', '.join(f'{t:.0f}' for t in tune_ms)} ms) in "
+        f"{time.perf_counter() - t0:.1f} s")
+    for i in range(args.warmup):
+        _, st = bs.run(sources, rows_out=False)
+        log(f"batch warmup {i}: {st.ms_total:.1f} ms")
+    stats, wall = [], 0.0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        _, st = bs.run(sources, rows_out=False)
+        wall += time.perf_counter() - t0
+        stats.append(st)
+        log(f"batch step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, H2D {st.h2d_bytes / 1e9:.3f} GB, "
+            f"rounds {st.ms_rounds:.1f} ms, transfer {st.ms_transfer:.1f} ms")
+    for r in stats[-1].trace:
+        log(f"  batch iter {r['iteration']}: active {r['active_nodes']} v / {r['active_edges']} e, "
+            f"{r['partitions']} partitions, H2D {r['h2d_bytes'] / 1e6:.1f} MB, {r['ms']:.2f} ms")
+    # one more (untimed) run hands the rows to Python for the comparison
+    rows, _ = bs.run(sources, rows_out=True)
+    bs.close()
+    ms_batch = wall / args.steps * 1e3
+    # k sequential single-source runs of the same sources
+    sess = sw.Session(g, args.algo, mode="async", budget=budget, buffers=args.buffers, transfer=args.transfer)
+    sess.run(int(sources[0]), copy=False)
+    same = True
+    t_seq = 0.0
+    h2d_seq = 0
+    for i, s0 in enumerate(sources):
+        t0 = time.perf_counter()
+        v, st = sess.run(int(s0), copy=False)
+        t_seq += time.perf_counter() - t0
+        h2d_seq += st.h2d_bytes
+        if not np.array_equal(v, rows[i]):
+            same = False
+            log(f"source {s0}: batched row differs from the single-source run")
+        if i % 8 == 7:
+            log(f"sequential: {i + 1} of {k} runs, {t_seq:.2f} s")
+    sess.close()
+    ms_seq = t_seq * 1e3
+    h2d = float(np.mean([s.h2d_bytes for s in stats]))
+    xfer_ms = float(np.mean([s.ms_transfer for s in stats]))
+    line = {
+        "metric": METRIC,
+        "value": round(k * g.m / (ms_batch / 1e3) / 1e9, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT generated in-process (no datasets)",
+        "config": {
+            "workload": f"{k}-source batched {args.algo} on weighted R-MAT scale {args.scale} ef {args.edge_factor}, "
+                        f"sources uniform over out-degree >= 1 (seed {args.seed}), device edge budget {budget} B",
+            "vertices": g.n, "edges": g.m, "sources": k, "budget_bytes": budget, "buffers": args.buffers,
+            "transfer": args.transfer, "unroll": unroll,
+        },
+        "batched": {"ms_per_run": round(ms_batch, 3), "global_iterations": stats[-1].global_iterations,
+                    "inner_rounds": stats[-1].inner_rounds, "rounds_launches": stats[-1].rounds_launches,
+                    "work_edges": stats[-1].work_edges, "work_entries": stats[-1].work_entries,
+                    "items_scanned": stats[-1].work_items, "updates": stats[-1].updates,
+                    "h2d_gb_per_run": round(h2d / 1e9, 4),
+                    "h2d_gbps_achieved": round(h2d / (xfer_ms * 1e-3) / 1e9, 2) if xfer_ms > 0 else None,
+                    "ms_breakdown": {"rounds": round(float(np.mean([s.ms_rounds for s in stats])), 3),
+                                     "transfer": round(xfer_ms, 3),
+                                     "generate": round(float(np.mean([s.ms_generate for s in stats])), 3)}},
+        "sequential": {"ms_total": round(ms_seq, 3), "ms_per_source": round(ms_seq / k, 3),
+                       "h2d_gb_total": round(h2d_seq / 1e9, 3), "gteps": round(k * g.m / t_seq / 1e9, 4)},
+        "batched_speedup_vs_sequential": round(ms_seq / ms_batch, 3),
+        "same_result_batched_vs_sequential": same,
+        "h2d_gbps_peak": round(peak, 2),
+        "roofline": batch_roofline(stats),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def batch_roofline(stats):
+    """Batched rounds kernel: per relaxed edge s(E) once plus 8 B per source
+    active at its vertex is the figure the work counters allow; they give the
+    union work (edges x active vertices), so the per-source term is priced
+    with the item count (sources active summed over items)."""
+    eb = stats[-1].edge_bytes
+    ms = sum(s.ms_rounds for s in stats)
+    launches = sum(s.rounds_launches for s in stats)
+    work = sum(s.work_edges for s in stats)
+    items = sum(s.work_items for s in stats)
+    ents = sum(s.work_entries for s in stats)
+    alg = eb * work + 8 * items + 12 * ents
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": "rounds_batch_kernel", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,
+            "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
+            "algorithmic_bytes_per_launch": int(alg / max(launches, 1)), "edge_bytes": eb}
+
+
 def roofline(np, stats, args, budget):
     """Roofline of the dominant kernel.  Persistent async rounds: one
     cooperative rounds_kernel launch per partition visit does every inner
@@ -173,6 +289,7 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--policy", default="both", choices=["both", "plain", "learned"])
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0, help="K-source batched SSSP against K sequential runs")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="launch the plain copy kernel once on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -206,6 +323,13 @@ def main():
         log(f"copy probe: {args.probe_copy_mb} MiB read + written in {ms:.3f} ms")
     peak = sw.h2d_peak(256 << 20, 10)
     log(f"pinned H2D peak {peak:.1f} GB/s")
+
+    if args.batch:
+        if not 1 <= args.batch <= 64 or args.algo not in ("sssp", "sswp", "bfs"):
+            log("--batch takes 1..64 sources of sssp, sswp or bfs")
+            sys.exit(2)
+        batch_bench(sw, np, g, args, budget, peak)
+        return
 
     params = policy.default_params()
     names = ["plain", "learned"] if args.policy == "both" else [args.policy]

@@ -48,9 +48,10 @@ struct RunOptions {
     int roundsGrid = 0;     // workgroups of the persistent kernel; 0 (and the cap) = one below full occupancy per CU
     int maxRounds = 4096;     // async rounds per partition visit (labels left set resume next iteration)
     int batch = 0;          // 1: batched k-source queries (runBatch), BFS/SSSP/SSWP async
-    int batchUnroll = 0;    // batched advance: edges in flight per wave (1, 2, 4, 8); 0 = 4
+    int batchUnroll = 0;    // batched advance: edges in flight per wave (1 .. 32, powers of 2); 0 = 8
     float theta = 0.5f;     // learned policy without a model: whole-range density threshold
     uint32_t spinLimit = 0; // polls before a device wait gives up (the run then fails); 0 = default
+    uint32_t joinWindow = 0;   // persistent launches: membership window in 100 MHz ticks; 0 = 2000 (20 us)
     int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
 };
 
@@ -89,6 +90,7 @@ struct RunStats {
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)
+    uint minMembers = 0;       // fewest member workgroups of a persistent launch (0: none ran)
     ull h2dBytes = 0;          // edge bytes moved host -> device
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads

@@ -36,12 +36,14 @@ struct HostCsr {
     uint maxDegree = 0;
     bool large = false;     // read from / destined for the 64-bit file variant
     bool pinned = false;    // edges live in pinned, device-mapped memory
-    // Weighted graphs: every edge as one 32-bit word, destination | weight <<
-    // packShift (pinned, device-mapped), built on first use when the vertex
-    // ids fit packShift bits and every weight the remaining 32 - packShift.
+    // Weighted graphs: every edge as one 32-bit word, destination |
+    // (weight - packBase) << packShift (pinned, device-mapped), built on
+    // first use when the vertex ids fit packShift bits and every weight minus
+    // the smallest one (packBase, below 2^27) the remaining 32 - packShift.
     uint32_t *packed = nullptr;
     uint32_t *d_packed = nullptr;   // device address of `packed`
-    uint32_t packShift = 0;         // 0: no packed copy (unweighted, or ids + weight exceed 32 bits)
+    uint32_t packShift = 0;         // 0: no packed copy (unweighted, or ids + weight span exceed 32 bits)
+    uint32_t packBase = 0;          // smallest weight, added back when an edge is decoded
     uint32_t ensurePacked();        // builds the packed copy once; returns packShift
     HostCsr() = default;
     HostCsr(const HostCsr &) = delete;

@@ -63,7 +63,7 @@ typedef struct subway_options {
                                the co-resident maximum per CU */
     int32_t max_rounds;     /* async rounds per partition visit; 0 = 4096 */
     int32_t batch;          /* 1: batched session (BFS/SSSP/SSWP async, <= 64 sources per run) */
-    int32_t batch_unroll;   /* batched advance: edges in flight per wave, 1/2/4/8; 0 = 4 */
+    int32_t batch_unroll;   /* batched advance: edges in flight per wave, 1/2/4/8/16/32; 0 = 8 */
     float theta;            /* learned policy without a model: id ranges whose active-edge density is at
                                least theta move whole (default 0.5; > 1 never) */
     uint32_t spin_limit;    /* polls before a device-side wait (scan lookback, grid barrier) gives up and
@@ -71,7 +71,9 @@ typedef struct subway_options {
     int32_t unpacked_edges; /* weighted graphs: 0 (default) moves and buffers each edge as one 32-bit word,
                                destination | weight << ceil(log2 V), when every weight fits the remaining
                                bits; 1 keeps 8-byte {end, w8} edges */
-    int32_t reserved[5];
+    uint32_t join_window;   /* persistent launches: how long (ticks of the 100 MHz constant clock) the first
+                               workgroup waits for the rest of the grid before closing membership; 0 = 2000 */
+    int32_t reserved[4];
 } subway_options;
 
 typedef struct subway_stats {
@@ -99,7 +101,7 @@ typedef struct subway_stats {
     uint32_t mispredictions;       /* learned policy: iterations whose active edges were > 2x off the prediction */
     uint32_t fallback_iteration;   /* learned policy: first iteration on the plain fallback (0 = none) */
     uint32_t edge_bytes;           /* bytes per edge moved and buffered (4 for packed weighted edges) */
-    uint32_t pad0;
+    uint32_t min_members;          /* fewest member workgroups of a persistent launch in the run (0: none) */
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -177,8 +179,8 @@ void subway_session_free(subway_session *s);
  * runs k separate processes. */
 int32_t subway_session_run_batch(subway_session *s, const uint32_t *sources, int32_t k, uint32_t *rows,
                                  subway_stats *stats, subway_iter_record *trace, uint32_t trace_cap);
-/* Autotuner: runs the batch once per candidate wave assignment (1, 2, 4, 8
- * edges in flight per wave), keeps the fastest; times_ms[4] may be NULL. */
+/* Autotuner: runs the batch once per candidate wave assignment (4, 8, 16,
+ * 32 edges in flight per wave), keeps the fastest; times_ms[4] may be NULL. */
 int32_t subway_session_tune_batch(subway_session *s, const uint32_t *sources, int32_t k, int32_t *best_unroll,
                                   double *times_ms);
 

@@ -48,7 +48,8 @@ class _Options(ctypes.Structure):
         ("verbose", ctypes.c_int32), ("advance_grid", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
-        ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
+        ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -64,7 +65,7 @@ class _Stats(ctypes.Structure):
         ("work_entries", ctypes.c_uint64),
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
         ("work_items", ctypes.c_uint64), ("mispredictions", ctypes.c_uint32), ("fallback_iteration", ctypes.c_uint32),
-        ("edge_bytes", ctypes.c_uint32), ("pad0", ctypes.c_uint32),
+        ("edge_bytes", ctypes.c_uint32), ("min_members", ctypes.c_uint32),
     ]
 
 
@@ -194,6 +195,7 @@ class RunStats:
     mispredictions: int = 0
     fallback_iteration: int = 0
     edge_bytes: int = 0
+    min_members: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -211,7 +213,8 @@ def _set_policy(h, params):
 
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True):
+             profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
+             join_window=0):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -232,6 +235,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.theta = theta
     o.spin_limit = spin_limit
     o.unpacked_edges = 0 if packed else 1
+    o.join_window = join_window
     return o
 
 
@@ -256,10 +260,10 @@ class Session:
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
-                 packed: bool = True, params=None):
+                 packed: bool = True, params=None, join_window: int = 0):
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
+                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -307,10 +311,10 @@ class BatchSession:
 
     def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",
                  unroll: int = 0, rounds_grid: int = 0, profile: bool = False, policy: str = "plain",
-                 theta: float = 0.5, packed: bool = True, params=None):
+                 theta: float = 0.5, packed: bool = True, params=None, join_window: int = 0):
         self.graph, self.algo = graph, algo
         o = _options(algo, "async", 0, budget, buffers, 0.01, 0.15, transfer, policy, False, 0, profile,
-                     "persistent", rounds_grid, 0, theta, 0, packed)
+                     "persistent", rounds_grid, 0, theta, 0, packed, join_window)
         o.batch = 1
         o.batch_unroll = unroll
         h = ctypes.c_void_p()
@@ -350,25 +354,26 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
         rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True,
-        params=None):
+        params=None, join_window: int = 0):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
     float32 (PR).  ``budget`` is the device edge budget in bytes (0 = 90% of
     device memory, the reference's sizing).  ``params``: scheduler network
-    for policy="learned".
+    for policy="learned".  ``join_window``: how long (100 MHz ticks) a
+    persistent launch waits for its workgroups before closing membership.
     """
     if params is not None:
         s = Session(graph, algo, mode=mode, budget=budget, buffers=buffers, threshold=threshold, init_pr=init_pr,
                     transfer=transfer, policy=policy, verbose=verbose, advance_grid=advance_grid, profile=profile,
                     rounds=rounds, rounds_grid=rounds_grid, max_rounds=max_rounds, theta=theta,
-                    spin_limit=spin_limit, packed=packed, params=params)
+                    spin_limit=spin_limit, packed=packed, params=params, join_window=join_window)
         try:
             return s.run(source)
         finally:
             s.close()
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed)
+                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

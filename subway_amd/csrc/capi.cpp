@@ -224,6 +224,7 @@ static RunOptions to_options(const subway_options *o)
     r.roundsGrid = o->rounds_grid;
     if (o->max_rounds > 0) r.maxRounds = o->max_rounds;
     r.spinLimit = o->spin_limit;
+    r.joinWindow = o->join_window;
     r.unpackedEdges = o->unpacked_edges;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
@@ -283,6 +284,7 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->mispredictions = s.mispredictions;
     stats->fallback_iteration = s.fallbackIteration;
     stats->edge_bytes = s.edgeBytes;
+    stats->min_members = s.minMembers;
 }
 
 extern "C" {

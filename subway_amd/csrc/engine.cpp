@@ -178,7 +178,8 @@ public:
             // weighted edges travel and sit in the buffers as one word each
             // when ids and weights fit: twice the edges per buffer, half the
             // bytes per transferred edge
-            if (!o_.unpackedEdges && (packShift_ = g_->csr->ensurePacked()) != 0) {
+            if (!o_.unpackedEdges && g_->csr->ensurePacked() != 0) {
+                packCode_ = g_->csr->packShift | (g_->csr->packBase << 5);
                 fmt_ = EDGE_PACKED;
                 eb_ = sizeof(uint32_t);
                 maxPart_ = std::min<ull>(maxPart_ * (sizeof(E) / sizeof(uint32_t)), 0x7FFFF800ull);
@@ -234,7 +235,7 @@ public:
             SUBWAY_CHECK(hipMalloc(&d_entryMask_, maxEntries * sizeof(ull)));
             SUBWAY_CHECK(hipMalloc(&d_sources_, BATCH_LANES * sizeof(uint32_t)));
             SUBWAY_CHECK(hipHostMalloc((void **)&h_sources_, BATCH_LANES * sizeof(uint32_t), 0));
-            setBatchUnroll(o_.batchUnroll > 0 ? o_.batchUnroll : 4);
+            setBatchUnroll(o_.batchUnroll > 0 ? o_.batchUnroll : 8);
             SUBWAY_CHECK(hipMalloc(&pc_, sizeof(dev::PersistCtl)));
             SUBWAY_CHECK(hipMemset(pc_, 0, sizeof(dev::PersistCtl)));
             SUBWAY_CHECK(hipDeviceSynchronize());
@@ -330,7 +331,7 @@ private:
     ull maxPart_ = 0;       // edges per device buffer (in the buffer's edge format)
     int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
     int eb_ = (int)sizeof(E);           // bytes per transferred / buffered edge
-    uint32_t packShift_ = 0;            // EDGE_PACKED: destination bits
+    uint32_t packCode_ = 0;            // EDGE_PACKED: shift | weight base << 5
     int advGrid_ = 0;
     RunStats st_;
     std::vector<Unit> units_;
@@ -338,7 +339,7 @@ private:
     dev::PersistCtl *pc_ = nullptr;
     int roundsGrid_ = 0;    // 0: persistent rounds kernel unavailable
     // batched state
-    int batchK_ = 0, batchUnroll_ = 4, batchGrid_ = 0;
+    int batchK_ = 0, batchUnroll_ = 8, batchGrid_ = 0;
     uint32_t *d_dist_ = nullptr, *d_sources_ = nullptr, *h_sources_ = nullptr, *h_dist_ = nullptr;
     ull *d_m1_ = nullptr, *d_m2_ = nullptr, *d_entryMask_ = nullptr;
 
@@ -485,8 +486,8 @@ void Engine<E, G>::checkWaits()
                 dev::PersistCtl pc;
                 SUBWAY_CHECK(hipMemcpy(&pc, pc_, sizeof(pc), hipMemcpyDeviceToHost));
                 diag = " [round " + std::to_string(pc.diag[0] >> 1) + (pc.diag[0] & 1 ? " advance" : " frontier") +
-                       " barrier: " + std::to_string(pc.diag[1]) + " of " + std::to_string(pc.diag[4]) +
-                       " workgroups started, group counter " + std::to_string(pc.diag[2]) + ", top counter " +
+                       " barrier: " + std::to_string(pc.diag[1]) + " members of " + std::to_string(pc.diag[4]) +
+                       " workgroups, group counter " + std::to_string(pc.diag[2]) + ", top counter " +
                        std::to_string(pc.diag[3]) + "]";
             }
             throw Error(ERR_INTERNAL, what + diag + " at global iteration " + std::to_string(st_.globalIterations) +
@@ -859,7 +860,7 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf)
     AdvanceArgs a{};
     a.algo = o_.algo;
     a.edgeFmt = fmt_;
-    a.packShift = packShift_;
+    a.packCode = packCode_;
     a.entries = entries_;
     a.chunkMap = chunkMap_;
     a.edges = sub_->d_buffers[buf];
@@ -913,7 +914,8 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.maxRounds = (uint32_t)std::max(1, roundsCap_);
         l.roundBase = 0;
         if (o_.profile) pRounds_.begin(sc_);
-        l.packShift = packShift_;
+        l.packCode = packCode_;
+        l.joinTicks = o_.joinWindow;
         launch_rounds(o_.algo, fmt_, l, roundsGrid_, sc_);
         if (o_.profile) pRounds_.end(sc_);
         return;
@@ -986,7 +988,7 @@ void Engine<E, G>::computeBatch(const Unit &u, int b)
             launch_frontier_masks(u.n, u.nodes, u.ptr, u.fromEdge, d_m1_, d_m2_, entries_, d_entryMask_, chunkMap_,
                                   fr_, sc_, u.idBase);
         const ull w0 = u.chunk * maxPart_;
-        launch_advance_batch_window(o_.algo, fmt_, packShift_, entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
+        launch_advance_batch_window(o_.algo, fmt_, packCode_, entries_, d_entryMask_, chunkMap_, sub_->d_buffers[b],
                                     d_dist_, d_m2_, w0, w0 + u.count, w0, fr_.ctl, advGrid_, sc_);
         return;
     }
@@ -1007,7 +1009,8 @@ void Engine<E, G>::computeBatch(const Unit &u, int b)
     l.pc = pc_;
     l.maxRounds = (uint32_t)std::max(1, roundsCap_);
     if (o_.profile) pRounds_.begin(sc_);
-    l.packShift = packShift_;
+    l.packCode = packCode_;
+    l.joinTicks = o_.joinWindow;
     launch_rounds_batch(o_.algo, fmt_, batchUnroll_, l, batchGrid_, sc_);
     if (o_.profile) pRounds_.end(sc_);
 }
@@ -1035,6 +1038,7 @@ void Engine<E, G>::resetRun()
         SUBWAY_CHECK(hipMemcpyAsync(&d_pstate_->prevUpdates, &fr_.ctl->updates, sizeof(ull),
                                     hipMemcpyDeviceToDevice, sc_));
     }
+    if (pc_) launch_fill_u32(&pc_->min_members, 0xFFFFFFFFu, 1, sc_);
     pAdv_.reset();
     pFr_.reset();
     pXfer_.reset();
@@ -1168,6 +1172,11 @@ void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries
     st_.workEntries = h_ctl_->entry_sum - entries0;
     st_.workItems = h_ctl_->item_sum - items0;
     st_.edgeBytes = (uint)eb_;
+    if (pc_) {
+        uint32_t mm = 0;
+        SUBWAY_CHECK(hipMemcpy(&mm, &pc_->min_members, sizeof(mm), hipMemcpyDeviceToHost));
+        st_.minMembers = mm == 0xFFFFFFFFu ? 0u : mm;
+    }
     if (o_.profile) {
         ull c = 0;
         st_.msAdvance = pAdv_.total(st_.advanceLaunches);
@@ -1204,7 +1213,7 @@ public:
     }
     int tuneBatch(const uint *sources, int k, double *timesMs) override
     {
-        static const int cand[4] = {1, 2, 4, 8};
+        static const int cand[4] = {4, 8, 16, 32};
         int best = eng_.batchUnroll();
         double bestMs = 1e300;
         for (int i = 0; i < 4; i++) {

@@ -65,14 +65,20 @@ uint32_t HostCsr<E>::ensurePacked()
         while (db < 32 && (1ull << db) < n) db++;   // ids 0 .. n-1 fit db bits
         if (db >= 32) return 0;
         const int T = host_threads();
-        std::vector<uint32_t> mx((size_t)T, 0);
+        std::vector<uint32_t> mx((size_t)T, 0), mn((size_t)T, 0xFFFFFFFFu);
         parallel_for(m, [&](uint64_t b, uint64_t e, int t) {
-            uint32_t x = 0;
-            for (uint64_t i = b; i < e; i++) x = std::max(x, edges[i].w8);
+            uint32_t x = 0, y = 0xFFFFFFFFu;
+            for (uint64_t i = b; i < e; i++) {
+                x = std::max(x, edges[i].w8);
+                y = std::min(y, edges[i].w8);
+            }
             mx[(size_t)t] = std::max(mx[(size_t)t], x);
+            mn[(size_t)t] = std::min(mn[(size_t)t], y);
         }, T);
         const uint32_t maxW = *std::max_element(mx.begin(), mx.end());
-        if ((uint64_t)maxW >> (32 - db)) return 0;   // a weight needs more than 32 - db bits
+        uint32_t base = *std::min_element(mn.begin(), mn.end());
+        if (base >> 27) base = 0;                         // the decode word holds a base below 2^27
+        if ((uint64_t)(maxW - base) >> (32 - db)) return 0;   // the weight span needs more than 32 - db bits
         uint32_t *p = nullptr;
         if (hipHostMalloc((void **)&p, (size_t)m * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocPortable) !=
             hipSuccess) {
@@ -80,10 +86,11 @@ uint32_t HostCsr<E>::ensurePacked()
             return 0;   // no room for the packed copy: transfers keep the 8-byte edges
         }
         parallel_for(m, [&](uint64_t b, uint64_t e, int) {
-            for (uint64_t i = b; i < e; i++) p[i] = edges[i].end | (edges[i].w8 << db);
+            for (uint64_t i = b; i < e; i++) p[i] = edges[i].end | ((edges[i].w8 - base) << db);
         }, T);
         packed = p;
         SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_packed, packed, 0));
+        packBase = base;
         packShift = db;
         return db;
     }

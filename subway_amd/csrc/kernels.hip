@@ -691,94 +691,46 @@ struct FrontierMaskOp {
 };
 
 // ============================================================ advance
+// Update rules of the label algorithms (shared/gpu_kernels.cu): the candidate
+// a pushing vertex with value `du` offers across an edge of weight w, the
+// improvement test, and the atomic that applies it.
 template <int ALGO>
-struct Relax;
-
+struct Rule;
 // cand = value[u] + 1, keep min   (shared/gpu_kernels.cu:42-56)
 template <>
-struct Relax<ALGO_BFS> {
-    template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, double *)
-    {
-        u32 cand = val + 1;
-        u32 v = e.end;
-        if (cand < value[v]) {
-            u32 prev = atomicMin(&value[v], cand);
-            if (cand < prev) {
-                mst<COH>(&lab[v], (uint8_t)1);
-                return true;
-            }
-        }
-        return false;
-    }
+struct Rule<ALGO_BFS> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32) { return du + 1; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
 };
 // cand = value[u], keep min       (shared/gpu_kernels.cu:90-106)
 template <>
-struct Relax<ALGO_CC> {
-    template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *value, uint8_t *lab, double *)
-    {
-        u32 v = e.end;
-        if (val < value[v]) {
-            u32 prev = atomicMin(&value[v], val);
-            if (val < prev) {
-                mst<COH>(&lab[v], (uint8_t)1);
-                return true;
-            }
-        }
-        return false;
-    }
+struct Rule<ALGO_CC> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32) { return du; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
 };
 // cand = value[u] + w, keep min   (shared/gpu_kernels.cu:143-157)
 template <>
-struct Relax<ALGO_SSSP> {
-    template <bool COH>
-    __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
-                                                 uint8_t *lab, double *)
-    {
-        u32 cand = val + e.w8;
-        u32 v = e.end;
-        if (cand < value[v]) {
-            u32 prev = atomicMin(&value[v], cand);
-            if (cand < prev) {
-                mst<COH>(&lab[v], (uint8_t)1);
-                return true;
-            }
-        }
-        return false;
-    }
+struct Rule<ALGO_SSSP> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du + w; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
 };
 // cand = min(value[u], w), keep max (shared/gpu_kernels.cu:193-207)
 template <>
-struct Relax<ALGO_SSWP> {
-    template <bool COH>
-    __device__ __forceinline__ static bool apply(u32 val, const OutEdgeWeighted &e, u32 *value,
-                                                 uint8_t *lab, double *)
-    {
-        u32 cand = val < e.w8 ? val : e.w8;
-        u32 v = e.end;
-        if (cand > value[v]) {
-            u32 prev = atomicMax(&value[v], cand);
-            if (cand > prev) {
-                mst<COH>(&lab[v], (uint8_t)1);
-                return true;
-            }
-        }
-        return false;
-    }
+struct Rule<ALGO_SSWP> {
+    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du < w ? du : w; }
+    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c > old; }
+    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMax(p, c); }
 };
-// delta[v] += share             (shared/gpu_kernels.cu:241-244)
-template <>
-struct Relax<ALGO_PR> {
-    template <bool COH, class E>
-    __device__ __forceinline__ static bool apply(u32 val, const E &e, u32 *, uint8_t *, double *delta)
-    {
-        // FP64 residual: an FP32 sum at a hub rounds away shares below half
-        // its ulp, a systematic loss of rank mass
-        unsafeAtomicAdd(&delta[e.end], (double)__uint_as_float(val));
-        return true;
-    }
-};
+
+template <class E>
+__device__ __forceinline__ u32 edge_weight(const E &e)
+{
+    if constexpr (sizeof(E) == 8) return reinterpret_cast<const u32 *>(&e)[1];
+    else return 1u;
+}
 
 // Storage format -> logical edge of a partition buffer.
 template <class S>
@@ -796,12 +748,14 @@ struct Codec<OutEdgeWeighted> {
 template <>
 struct Codec<PackedEdge> {
     typedef OutEdgeWeighted T;
-    __device__ __forceinline__ static T get(const PackedEdge *p, u64 i, u32 sh)
+    // code = destination bits | weight base << 5
+    __device__ __forceinline__ static T get(const PackedEdge *p, u64 i, u32 code)
     {
         const u32 x = p[i].bits;
+        const u32 sh = code & 31u;
         T e;
         e.end = x & ((1u << sh) - 1u);
-        e.w8 = x >> sh;
+        e.w8 = (x >> sh) + (code >> 5);
         return e;
     }
 };
@@ -834,18 +788,65 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
         if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
         return 0;
     }
-    u32 updates = 0;
-    for (u64 p = pbeg + tid; p < pend; p += ADV_BLOCK) {
-        const u32 pp = (u32)p;
-        int lo = 0, hi = nk - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (s_pre[mid] <= pp) lo = mid;
-            else hi = mid - 1;
+    // every thread owns positions pbeg + tid + u * ADV_BLOCK, u < PPT; the
+    // phases below keep PPT edge loads, then PPT value loads, then PPT
+    // atomics in flight per thread
+    constexpr int PPT = ADV_CHUNK / ADV_BLOCK;
+    static_assert(ADV_CHUNK % ADV_BLOCK == 0, "a chunk is a whole number of positions per thread");
+    u32 dst[PPT], cv[PPT];
+    bool ok[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; u++) {
+        const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
+        ok[u] = p < pend;
+        dst[u] = 0;
+        cv[u] = 0;
+        if (ok[u]) {
+            const u32 pp = (u32)p;
+            int lo = 0, hi = nk - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= pp) lo = mid;
+                else hi = mid - 1;
+            }
+            const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
+            const auto e = Codec<E>::get(edges, eidx, psh);
+            dst[u] = e.end;
+            if constexpr (ALGO == ALGO_PR) cv[u] = s_val[lo];
+            else cv[u] = Rule<ALGO>::cand(s_val[lo], edge_weight(e));
         }
-        const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-        const auto e = Codec<E>::get(edges, eidx, psh);
-        if (Relax<ALGO>::template apply<COH>(s_val[lo], e, value, lab, delta)) updates++;
+    }
+    u32 updates = 0;
+    if constexpr (ALGO == ALGO_PR) {
+        // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
+        // FP64: an FP32 sum at a hub rounds away shares below half its ulp
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (ok[u]) {
+                unsafeAtomicAdd(&delta[dst[u]], (double)__uint_as_float(cv[u]));
+                updates++;
+            }
+        (void)value;
+        (void)lab;
+    } else {
+        u32 cur[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; u++) cur[u] = ok[u] ? value[dst[u]] : 0u;
+        u32 old[PPT];
+        bool tried[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; u++) {
+            tried[u] = ok[u] && Rule<ALGO>::better(cv[u], cur[u]);
+            old[u] = tried[u] ? Rule<ALGO>::atomic(&value[dst[u]], cv[u]) : 0u;
+        }
+        // a label is set only where the atomic's old value shows the improvement
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (tried[u] && Rule<ALGO>::better(cv[u], old[u])) {
+                mst<COH>(&lab[dst[u]], (uint8_t)1);
+                updates++;
+            }
+        (void)delta;
     }
     __syncthreads();
     return updates;
@@ -890,27 +891,88 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 }
 
 // ================================================== persistent rounds
-// Two-level grid barrier for a cooperative launch (every workgroup resident).
-// Everything the workgroups of a persistent launch exchange is read and
-// written with agent-scope atomics (the COH memory policy), so the barrier
-// orders only their completion: every wave waits for its outstanding memory
-// operations, then thread 0 arrives at its group counter, the last of a
-// group arrives at the top counter, and the last overall bumps the
-// generation the others poll.  No L2 write-back or invalidate is needed
-// (with them a barrier of 768 workgroups costs about 60 us instead of 4).
-// The wait is bounded (`limit` polls, WAIT_BARRIER recorded on expiry).
-// Returns false on every workgroup once any wait of the launch has given up
-// (checked after the barrier, so all workgroups that pass one barrier agree).
-__device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u32 where)
+// ------------------------------------------------ persistent launches
+// A persistent launch does not rely on every workgroup of its grid being
+// resident at once (kernels of other streams may hold slots).  The first
+// workgroup to start waits, at most a window of the constant 100 MHz clock
+// (JOIN_WINDOW_TICKS by default), for the rest of the grid, then closes
+// membership; workgroups that
+// started before the close are the members (ids 0 .. members-1, in start
+// order) and take part in every barrier, the others leave at once.  Work is
+// claimed through counters, so any member count covers it.
+constexpr u32 JOIN_CLOSED = 0x80000000u;
+constexpr u64 JOIN_WINDOW_TICKS = 2000;   // 20 us
+
+// Returns false for a workgroup that is not a member (or whose wait for the
+// member count gave up; the error word then holds WAIT_BARRIER).
+__device__ __forceinline__ bool join_launch(PersistCtl *pc, u32 limit, u32 window, u32 &me, u32 &members)
+{
+    __shared__ u32 s_me, s_members;
+    if (threadIdx.x == 0) {
+        u32 old = fetch_add_relaxed(&pc->join, 1u);
+        u32 m = 0;
+        if (old & JOIN_CLOSED) {
+            old = 0;                       // started after the close
+        } else if (old == 0) {
+            const u64 t0 = wall_clock64();
+            while ((ld_relaxed(&pc->join) & ~JOIN_CLOSED) < gridDim.x && wall_clock64() - t0 < window)
+                __builtin_amdgcn_s_sleep(1);
+            m = __hip_atomic_fetch_or(&pc->join, JOIN_CLOSED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                ~JOIN_CLOSED;
+            st_relaxed(&pc->members, m);
+            (void)atomicMin(&pc->min_members, m);
+        } else {
+            u32 polls = 0;
+            while ((m = ld_relaxed(&pc->members)) == 0) {
+                if (++polls > limit) {
+                    (void)atomicCAS(&pc->error, 0u, WAIT_BARRIER);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        s_me = old;
+        s_members = m;
+    }
+    __syncthreads();
+    me = s_me;
+    members = s_members;
+    return members != 0;
+}
+
+// Every workgroup of the grid, member or not, passes here exactly once; the
+// last one clears the membership words for the next launch.
+__device__ __forceinline__ void leave_launch(PersistCtl *pc)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 && fetch_add_relaxed(&pc->left, 1u) == gridDim.x - 1) {
+        st_relaxed(&pc->join, 0u);
+        st_relaxed(&pc->members, 0u);
+        st_relaxed(&pc->left, 0u);
+    }
+}
+
+// Two-level barrier over the members.  Everything the workgroups of a
+// persistent launch exchange is read and written with agent-scope atomics
+// (the COH memory policy), so the barrier orders only their completion: every
+// wave waits for its outstanding memory operations, then thread 0 arrives at
+// its group counter (up to 8 groups), the last of a group arrives at the top
+// counter, and the last overall bumps the generation the others poll.  No L2
+// write-back or invalidate is needed (with them a barrier of 768 workgroups
+// costs about 60 us instead of 4).  The wait is bounded (`limit` polls,
+// WAIT_BARRIER recorded on expiry).  Returns false on every member once any
+// wait of the launch has given up (checked after the barrier, so all members
+// that pass one barrier agree).
+__device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u32 where, u32 me, u32 members)
 {
     __shared__ u32 s_fail;
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const u32 g = gen;
-        const u32 ngroups = gridDim.x < 8u ? gridDim.x : 8u;
-        const u32 grp = blockIdx.x % ngroups;
-        const u32 grpSize = (gridDim.x - grp + ngroups - 1) / ngroups;
+        const u32 ngroups = members < 8u ? members : 8u;
+        const u32 grp = me % ngroups;
+        const u32 grpSize = (members - grp + ngroups - 1) / ngroups;
         bool releaser = false;
         u32 a = fetch_add_relaxed(&pc->bar_sub[grp * 16], 1u) + 1;
         if (a == grpSize) {
@@ -937,7 +999,7 @@ __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u
         gen = g + 1;
         if (gaveUp && atomicCAS(&pc->error, 0u, WAIT_BARRIER) == 0u) {
             pc->diag[0] = where;
-            pc->diag[1] = ld_relaxed(&pc->started);
+            pc->diag[1] = members;
             pc->diag[2] = ld_relaxed(&pc->bar_sub[grp * 16]);
             pc->diag[3] = ld_relaxed(&pc->bar_top[0]);
             pc->diag[4] = gridDim.x;
@@ -968,7 +1030,8 @@ struct RoundsArgs {
     u32 roundBase;         // round index (within the partition) of this launch's first round
     u32 idBase;            // nodes == null: items are the id range idBase ..
     u32 spinLimit;         // poll limit of every wait of the launch
-    u32 packShift;         // EDGE_PACKED: destination bits
+    u32 joinTicks;         // membership window
+    u32 packCode;         // EDGE_PACKED: destination bits
 };
 
 // All async inner rounds of one resident partition in one cooperative
@@ -977,14 +1040,13 @@ struct RoundsArgs {
 // counter), grid barrier; the loop ends, uniformly for every workgroup, on
 // the first round that selects nothing or after maxRounds rounds.
 template <int ALGO, class E>
-__global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
+__device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 members)
 {
     __shared__ u32 s_claim, s_gen, s_epoch;
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_gen = ld_relaxed(&a.pc->bar_gen[0]);
         s_epoch = ld_relaxed(&a.sc->epoch);
-        atomicAdd(&a.pc->started, 1u);
     }
     __syncthreads();
     u32 gen = s_gen;
@@ -1020,14 +1082,14 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
         }
         // a wait that gave up leaves the launch's results unusable; every
         // workgroup leaves here and the host reports the error word
-        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u, me, members)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
             conv = 1;
             break;
         }
-        if (blockIdx.x == 0 && tid == 0) {
+        if (me == 0 && tid == 0) {
             st_relaxed(&a.pc->tile_ctr[(r + 1) & 1], 0u);
             st_relaxed(&a.pc->chunk_ctr[(r + 1) & 1], 0u);
         }
@@ -1040,15 +1102,14 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
-                                              a.delta, count, W, 0, W, 0, a.packShift, wt.err);
+                                              a.delta, count, W, 0, W, 0, a.packCode, wt.err);
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
-    if (blockIdx.x == 0 && tid == 0) {
+    if (me == 0 && tid == 0) {
         st_relaxed(&a.sc->epoch, epoch0 + r + conv);
-        st_relaxed(&a.pc->started, 0u);   // every workgroup has passed the last barrier
         a.pc->rounds_done = r;
         a.pc->converged = conv;
         st_relaxed(&a.pc->tile_ctr[0], 0u);
@@ -1058,36 +1119,15 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
     }
 }
 
-// ======================================================= batched rounds
-// Per-lane relaxation of one source's value (lane = source).
-template <int ALGO>
-struct BatchRelax;
-template <>
-struct BatchRelax<ALGO_SSSP> {
-    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du + w; }
-    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
-    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
-};
-template <>
-struct BatchRelax<ALGO_BFS> {
-    __device__ __forceinline__ static u32 cand(u32 du, u32) { return du + 1; }
-    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c < old; }
-    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMin(p, c); }
-};
-template <>
-struct BatchRelax<ALGO_SSWP> {
-    __device__ __forceinline__ static u32 cand(u32 du, u32 w) { return du < w ? du : w; }
-    __device__ __forceinline__ static bool better(u32 c, u32 old) { return c > old; }
-    __device__ __forceinline__ static u32 atomic(u32 *p, u32 c) { return atomicMax(p, c); }
-};
-
-template <class E>
-__device__ __forceinline__ u32 edge_weight(const E &e)
+template <int ALGO, class E>
+__global__ __launch_bounds__(ADV_BLOCK) void rounds_kernel(RoundsArgs a)
 {
-    if constexpr (sizeof(E) == 8) return reinterpret_cast<const u32 *>(&e)[1];
-    else return 1u;
+    u32 me, members;
+    if (join_launch(a.pc, a.spinLimit, a.joinTicks, me, members)) rounds_body<ALGO, E>(a, me, members);
+    leave_launch(a.pc);
 }
 
+// ======================================================= batched rounds
 // One chunk of BATCH_CHUNK positions: each wave walks a contiguous quarter of
 // it, U positions at a time (U edge loads and U destination rows in flight);
 // lane s relaxes source s when bit s of the pushing vertex's mask is set.
@@ -1162,16 +1202,22 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
                 dv[u] = 0;
                 if (ok[u] && ((mm[u] >> lane) & 1ull)) dv[u] = dist[(u64)vv[u] * WAVE + lane];
             }
+            // all U atomics are issued before any result is used
+            u32 cc[U], ol[U];
+            bool tr[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                bool imp = false;
+                tr[u] = false;
+                cc[u] = 0;
                 if (ok[u] && ((mm[u] >> lane) & 1ull)) {
-                    const u32 c = BatchRelax<ALGO>::cand(dd[u], ww[u]);
-                    if (BatchRelax<ALGO>::better(c, dv[u])) {
-                        const u32 old = BatchRelax<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], c);
-                        imp = BatchRelax<ALGO>::better(c, old);
-                    }
+                    cc[u] = Rule<ALGO>::cand(dd[u], ww[u]);
+                    tr[u] = Rule<ALGO>::better(cc[u], dv[u]);
                 }
+                ol[u] = tr[u] ? Rule<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], cc[u]) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool imp = tr[u] && Rule<ALGO>::better(cc[u], ol[u]);
                 const unsigned long long bal = __ballot(imp);
                 if (bal) {
                     if (lane == 0) atomicOr((unsigned long long *)&maskOut[vv[u]], bal);
@@ -1201,20 +1247,20 @@ struct BatchArgs {
     u32 maxRounds;
     u32 idBase;
     u32 spinLimit;
-    u32 packShift;
+    u32 joinTicks;         // membership window
+    u32 packCode;
 };
 
 // All async rounds of one resident partition for up to 64 sources at once,
 // in one cooperative launch (same phase / barrier structure as rounds_kernel).
 template <int ALGO, class E, int U>
-__global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
+__device__ __forceinline__ void rounds_batch_body(const BatchArgs &a, u32 me, u32 members)
 {
     __shared__ u32 s_claim, s_gen, s_epoch;
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_gen = ld_relaxed(&a.pc->bar_gen[0]);
         s_epoch = ld_relaxed(&a.sc->epoch);
-        atomicAdd(&a.pc->started, 1u);
     }
     __syncthreads();
     u32 gen = s_gen;
@@ -1238,14 +1284,14 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
                                                                wt))
                 break;
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 0u, me, members)) return;
         const u64 count = ld_relaxed(&a.sc->count);
         const u64 W = ld_relaxed(&a.sc->total);
         if (count == 0) {
             conv = 1;
             break;
         }
-        if (blockIdx.x == 0 && tid == 0) {
+        if (me == 0 && tid == 0) {
             st_relaxed(&a.pc->tile_ctr[(r + 1) & 1], 0u);
             st_relaxed(&a.pc->chunk_ctr[(r + 1) & 1], 0u);
         }
@@ -1257,15 +1303,14 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
             __syncthreads();
             if (c >= nchunks) break;
             updates += advance_chunk_batch<ALGO, E, U, true>(c, a.entries, a.entryMask, a.chunkMap, (const E *)a.edges,
-                                                       a.dist, labOut, count, W, 0, W, 0, a.packShift, wt.err);
+                                                       a.dist, labOut, count, W, 0, W, 0, a.packCode, wt.err);
         }
-        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u)) return;
+        if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
-    if (blockIdx.x == 0 && tid == 0) {
+    if (me == 0 && tid == 0) {
         st_relaxed(&a.sc->epoch, epoch0 + r + conv);
-        st_relaxed(&a.pc->started, 0u);   // every workgroup has passed the last barrier
         a.pc->rounds_done = r;
         a.pc->converged = conv;
         st_relaxed(&a.pc->tile_ctr[0], 0u);
@@ -1273,6 +1318,17 @@ __global__ __launch_bounds__(ADV_BLOCK) void rounds_batch_kernel(BatchArgs a)
         st_relaxed(&a.pc->chunk_ctr[0], 0u);
         st_relaxed(&a.pc->chunk_ctr[1], 0u);
     }
+}
+
+// At most 128 VGPRs (four waves per SIMD): a few registers of the frontier
+// scan spill, but the advance, bound by the latency of its row accesses,
+// gains more from the extra waves (22% at 64 sources on R-MAT scale 26).
+template <int ALGO, class E, int U>
+__global__ __launch_bounds__(ADV_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void rounds_batch_kernel(BatchArgs a)
+{
+    u32 me, members;
+    if (join_launch(a.pc, a.spinLimit, a.joinTicks, me, members)) rounds_batch_body<ALGO, E, U>(a, me, members);
+    leave_launch(a.pc);
 }
 
 // Standalone batched advance over the work window [w0, w1) (a vertex larger
@@ -1777,7 +1833,7 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 {
     hipLaunchKernelGGL((advance_kernel<ALGO, E>), dim3((unsigned)a.grid), dim3(ADV_BLOCK), 0, st,
                        a.entries, a.chunkMap, (const E *)a.edges, a.value, a.labelOut, a.delta,
-                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packShift, ctl);
+                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packCode, ctl);
 }
 
 // Launch `L<ALGO, storage>` for the buffer's edge format; SSSP and SSWP need
@@ -1910,7 +1966,8 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.maxRounds = l.maxRounds;
     a.roundBase = l.roundBase;
     a.idBase = l.idBase;
-    a.packShift = l.packShift;
+    a.packCode = l.packCode;
+    a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
     SUBWAY_CHECK(hipLaunchCooperativeKernel(rounds_fn_for(algo, edgeFmt), dim3((unsigned)grid), dim3(ADV_BLOCK),
                                             params, 0, st));
@@ -1958,7 +2015,7 @@ struct BatchWindow {
     uint32_t *dist;
     ull *maskOut;
     ull w0, w1, shift;
-    uint32_t packShift;
+    uint32_t packCode;
     ScanCtl *ctl;
     int grid;
 };
@@ -1968,15 +2025,15 @@ static void adv_batch(const BatchWindow &w, hipStream_t st)
 {
     hipLaunchKernelGGL((advance_batch_kernel<ALGO, E>), dim3((unsigned)w.grid), dim3(ADV_BLOCK), 0, st, w.entries,
                        w.entryMask, w.chunkMap, (const E *)w.edges, w.dist, w.maskOut, (u64)w.w0, (u64)w.w1,
-                       (u64)w.shift, (u32)w.packShift, w.ctl);
+                       (u64)w.shift, (u32)w.packCode, w.ctl);
 }
 
-void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packShift, const Entry *entries,
+void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packCode, const Entry *entries,
                                  const ull *entryMask, const uint32_t *chunkMap, const void *edges, uint32_t *dist,
                                  ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st)
 {
     if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
-    const BatchWindow w{entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, packShift, ctl, grid};
+    const BatchWindow w{entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, packCode, ctl, grid};
     switch (algo) {
         case ALGO_BFS: SUBWAY_BY_FORMAT(adv_batch, ALGO_BFS, edgeFmt, w, st); break;
         case ALGO_SSSP: SUBWAY_BY_FORMAT(adv_batch, ALGO_SSSP, edgeFmt, w, st); break;
@@ -2000,7 +2057,9 @@ static void batch_fn_u(const void **f, int unroll)
         case 2: *f = batch_fn<ALGO, E, 2>(); break;
         case 4: *f = batch_fn<ALGO, E, 4>(); break;
         case 8: *f = batch_fn<ALGO, E, 8>(); break;
-        default: throw Error(ERR_ARG, "batched unroll must be 1, 2, 4 or 8");
+        case 16: *f = batch_fn<ALGO, E, 16>(); break;
+        case 32: *f = batch_fn<ALGO, E, 32>(); break;
+        default: throw Error(ERR_ARG, "batched unroll must be 1, 2, 4, 8, 16 or 32");
     }
 }
 
@@ -2052,7 +2111,8 @@ void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l
     a.spinLimit = l.scan->spinLimit;
     a.maxRounds = l.maxRounds;
     a.idBase = l.idBase;
-    a.packShift = l.packShift;
+    a.packCode = l.packCode;
+    a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
     SUBWAY_CHECK(hipLaunchCooperativeKernel(batch_fn_for(algo, edgeFmt, unroll), dim3((unsigned)grid),
                                             dim3(ADV_BLOCK), params, 0, st));

@@ -38,8 +38,9 @@ constexpr int GATHER_CHUNK = 4096;
 // (uint32 value, or float bits of the PageRank share).
 // Device formats of the edges in a partition buffer.  The unpacked formats'
 // values are their sizes in bytes.  EDGE_PACKED stores a weighted edge in one
-// word, destination | weight << packShift, for graphs whose vertex ids fit
-// packShift bits and whose weights fit the remaining 32 - packShift bits.
+// word, destination | (weight - base) << shift, for graphs whose vertex ids
+// fit `shift` bits and whose weight span fits the remaining 32 - shift bits;
+// kernels take the decode word packCode = shift | base << 5.
 enum EdgeFormat : int {
     EDGE_U32 = 4,      // OutEdge
     EDGE_W64 = 8,      // OutEdgeWeighted
@@ -172,7 +173,7 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
 struct AdvanceArgs {
     int algo;
     int edgeFmt;                     // EdgeFormat of the buffer
-    uint32_t packShift;              // EDGE_PACKED: destination bits
+    uint32_t packCode;              // EDGE_PACKED: shift | weight base << 5
     const Entry *entries;
     const uint32_t *chunkMap;
     const void *edges;               // device edge buffer
@@ -204,7 +205,8 @@ struct RoundsLaunch {
     uint32_t maxRounds;
     uint32_t roundBase;
     uint32_t idBase;       // nodes == null: a whole id range starting here
-    uint32_t packShift;    // EDGE_PACKED: destination bits
+    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 20 us
+    uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
 // below full occupancy (0: no cooperative launch).
@@ -233,7 +235,8 @@ struct BatchLaunch {
     dev::PersistCtl *pc;
     uint32_t maxRounds;
     uint32_t idBase;
-    uint32_t packShift;
+    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 20 us
+    uint32_t packCode;
 };
 int batch_max_grid(int algo, int edgeFmt, int unroll);
 void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fromEdge, ull *labA, ull *labB,
@@ -241,7 +244,7 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
                            uint32_t idBase = 0);
 // Batched advance over the work window [w0, w1) of the last frontier (a
 // vertex larger than a buffer, streamed chunk by chunk).
-void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packShift, const Entry *entries,
+void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packCode, const Entry *entries,
                                  const ull *entryMask, const uint32_t *chunkMap, const void *edges, uint32_t *dist,
                                  ull *maskOut, ull w0, ull w1, ull shift, ScanCtl *ctl, int grid, hipStream_t st);
 void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l, int grid, hipStream_t st);

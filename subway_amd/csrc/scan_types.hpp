@@ -54,10 +54,16 @@ struct PersistCtl {
     u32 rounds_done;   // rounds that selected work in the last launch
     u32 converged;     // 1: the last launch ended on a round that selected nothing
     u32 error;         // nonzero: a bounded device wait gave up (WAIT_* code)
-    u32 started;       // workgroups of the running launch that have started
+    // membership of the running launch: arrivals (bit 31 set once the first
+    // workgroup has closed membership), the member count it published, and
+    // workgroups that have left (the last one clears all three)
+    u32 join;
+    u32 members;
+    u32 left;
+    u32 min_members;   // fewest members of any launch since the host last reset it
     // first barrier wait that gave up: round << 1 | phase (0 frontier, 1
-    // advance), workgroups started, arrivals counted at its group and top
-    // counters, grid size
+    // advance), members, arrivals counted at its group and top counters,
+    // grid size
     u32 diag[5];
 };
 
