@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <map>
@@ -553,6 +554,28 @@ void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, u
 // the iteration then shows is a misprediction, and once more than half of at
 // least three judged predictions missed, the rest of the run goes to the
 // plain schedule.  Without parameters the options' fixed theta and cap apply.
+// The schedules of the training sweep: whole-range density thresholds and
+// inner-round caps.  Head 1 is theta; head 2 is log2(cap) / 16.
+static const double kThetas[4] = {0.25, 0.5, 0.8, 2.0};
+static const int kCaps[4] = {1, 8, 64, 4096};
+
+static double snap_theta(float h)
+{
+    double best = kThetas[0];
+    for (double t : kThetas)
+        if (std::fabs((double)h - t) < std::fabs((double)h - best)) best = t;
+    return best;
+}
+
+static int snap_cap(float h)
+{
+    const double l = 16.0 * std::min(std::max((double)h, 0.0), 1.0);
+    int best = kCaps[0];
+    for (int c : kCaps)
+        if (std::fabs(l - std::log2((double)c)) < std::fabs(l - std::log2((double)best))) best = c;
+    return best;
+}
+
 template <class E, class G>
 void Engine<E, G>::applyPolicy(IterRecord &rec)
 {
@@ -561,7 +584,9 @@ void Engine<E, G>::applyPolicy(IterRecord &rec)
     std::memcpy(rec.heads, po.head, sizeof(rec.heads));
     if (d_params_ && !onPlain_) {
         const float actual = po.feat[2];
-        if (predEdges_ >= 0.f) {
+        // iterations with under 1e-4 of the edges active (predicted and
+        // actual) are not judged: their schedule hardly matters
+        if (predEdges_ >= 0.f && (predEdges_ >= 1e-4f || actual >= 1e-4f)) {
             const float r = (predEdges_ + 1e-6f) / (actual + 1e-6f);
             judged_++;
             if (r > 2.f || r < 0.5f) st_.mispredictions++;
@@ -574,9 +599,9 @@ void Engine<E, G>::applyPolicy(IterRecord &rec)
         if (!onPlain_) {
             // head 0: log2 of the next iteration's active-edge fraction, / 32
             predEdges_ = (float)std::exp2(32.0 * std::min(std::max((double)po.head[0], -1.0), 0.0));
-            theta_ = std::min(std::max((double)po.head[1], 0.0), 2.0);
-            const double c = std::exp2(std::min(std::max((double)po.head[2], 0.0), 1.0) * 16.0);
-            roundsCap_ = (int)std::min<double>(std::max(1.0, std::round(c)), (double)o_.maxRounds);
+            // decisions snap to the schedules the network was trained on
+            theta_ = snap_theta(po.head[1]);
+            roundsCap_ = std::min(snap_cap(po.head[2]), o_.maxRounds);
         }
     }
     if (onPlain_) {
@@ -886,6 +911,14 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
     if constexpr (!isPR) {
         l1 = (uint8_t *)g_->d_label1;
         l2 = (uint8_t *)g_->d_label2;
+    }
+    if (roundsCap_ == 1) {
+        // a single round: one frontier + advance pair (the persistent launch
+        // would add its membership and barrier costs for nothing); the labels
+        // it sets are picked up by the next generate
+        frontier(1, l1, l2, un);
+        advance(l2, 0, ~0ull, 0, buf);
+        return;
     }
     if (roundsGrid_ > 0) {
         // every round of the partition in one cooperative launch; a partition

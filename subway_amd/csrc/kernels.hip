@@ -1504,7 +1504,7 @@ __global__ __launch_bounds__(256) void policy_kernel(PolicyArgs a)
         for (int b = 0; b < 4; b++) x[4 + b] = hsum > 0.f ? h[b] / hsum : 0.f;
         x[8] = (float)a.bufEdges / E;
         x[9] = (float)a.iteration / 16.f;
-        x[10] = (float)lastUpd / E;
+        x[10] = log2f(1.f + (float)lastUpd / E) / 8.f;
         x[11] = a.P ? (float)s_dense / (float)a.P : 0.f;
         for (int i = 0; i < MLP_IN; i++) {
             s_x[i] = x[i];

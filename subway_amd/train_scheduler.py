@@ -30,11 +30,11 @@ import subway_amd as sw
 from subway_amd import policy
 
 THETAS = (2.0, 0.8, 0.5, 0.25)     # 2.0: no range moves whole (the plain transfer choice)
-CAPS = (4096, 64, 8)
+CAPS = (4096, 64, 8, 1)
 PLAIN_LIKE = (2.0, 4096)
 # near-ties (within 2%) go to the earlier entry
 THETA_PREF = (0.8, 0.5, 2.0, 0.25)
-CAP_PREF = (64, 8, 4096)
+CAP_PREF = (64, 8, 1, 4096)
 
 
 def configs(quick: bool):
@@ -45,7 +45,7 @@ def configs(quick: bool):
         if quick:
             scales = (15, 16)
         elif algo == "pr":
-            scales = (16, 17)
+            scales = (16, 18, 20)
         elif algo in ("sssp", "bfs"):
             scales = (16, 17, 18, 19, 20, 21, 22, 23)
         else:
@@ -71,7 +71,7 @@ def run_config(g, algo, div, reps, log, seed=0):
     src = g.max_degree_vertex if algo != "pr" else 0
     kw = dict(mode="async", budget=budget)
     if algo == "pr":
-        kw["threshold"] = 1e-4
+        kw["threshold"] = 1e-6
     keys = [(th, cap) for th in THETAS for cap in CAPS] + ["plain"]
     sess = {}
     for k in keys:

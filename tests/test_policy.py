@@ -66,7 +66,9 @@ def test_fused_inference_matches_mlp_kernel_and_oracle(sw, gpu):
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "sswp"])
 def test_any_network_keeps_results_exact(sw, gpu, algo):
     """Random networks choose arbitrary thresholds and round caps; values
-    stay bit-exact, and the decisions applied follow the heads."""
+    stay bit-exact, and the decisions applied are the trained schedules
+    nearest to the heads (theta in {0.25, 0.5, 0.8, 2}, cap in {1, 8, 64,
+    4096}, nearest in log2)."""
     from subway_amd import policy
     weighted = algo != "bfs"
     g = _rmat(sw, 15, 9, weighted)
@@ -81,9 +83,11 @@ def test_any_network_keeps_results_exact(sw, gpu, algo):
             if r["on_plain"]:
                 assert r["theta"] == 2.0
                 continue
-            assert r["theta"] == pytest.approx(min(max(r["heads"][1], 0.0), 2.0), rel=1e-6)
-            cap = 2 ** (16 * min(max(r["heads"][2], 0.0), 1.0))
-            assert r["rounds_cap"] == min(max(1, round(cap)), 4096)
+            thetas = np.array([0.25, 0.5, 0.8, 2.0])
+            assert r["theta"] == pytest.approx(thetas[np.argmin(np.abs(thetas - r["heads"][1]))], rel=1e-6)
+            caps = np.array([1, 8, 64, 4096])
+            l2 = 16 * min(max(r["heads"][2], 0.0), 1.0)
+            assert r["rounds_cap"] == caps[np.argmin(np.abs(np.log2(caps) - l2))]
 
 
 def test_mispredictions_fall_back_to_plain(sw, gpu):
