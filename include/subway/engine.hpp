@@ -51,7 +51,7 @@ struct RunOptions {
     int batchUnroll = 0;    // batched advance: edges in flight per wave (1 .. 32, powers of 2); 0 = 8
     float theta = 0.5f;     // learned policy without a model: whole-range density threshold
     uint32_t spinLimit = 0; // polls before a device wait gives up (the run then fails); 0 = default
-    uint32_t joinWindow = 0;   // persistent launches: membership window in 100 MHz ticks; 0 = 2000 (20 us)
+    uint32_t joinWindow = 0;   // persistent launches: membership window in 100 MHz ticks; 0 = 50000 (500 us)
     int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
 };
 

@@ -72,7 +72,7 @@ typedef struct subway_options {
                                destination | weight << ceil(log2 V), when every weight fits the remaining
                                bits; 1 keeps 8-byte {end, w8} edges */
     uint32_t join_window;   /* persistent launches: how long (ticks of the 100 MHz constant clock) the first
-                               workgroup waits for the rest of the grid before closing membership; 0 = 2000 */
+                               workgroup waits for the rest of the grid before closing membership; 0 = 50000 */
     int32_t reserved[4];
 } subway_options;
 

@@ -901,7 +901,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
 // order) and take part in every barrier, the others leave at once.  Work is
 // claimed through counters, so any member count covers it.
 constexpr u32 JOIN_CLOSED = 0x80000000u;
-constexpr u64 JOIN_WINDOW_TICKS = 2000;   // 20 us
+constexpr u64 JOIN_WINDOW_TICKS = 50000;   // 500 us
 
 // Returns false for a workgroup that is not a member (or whose wait for the
 // member count gave up; the error word then holds WAIT_BARRIER).
@@ -1543,8 +1543,9 @@ __global__ __launch_bounds__(GATHER_BLOCK) void gather_kernel(const u32 *nodes, 
     __shared__ u64 s_src[GATHER_SEGS];
     __shared__ u64 s_k[2];
     const int tid = threadIdx.x;
-    const u64 q0 = (u64)blockIdx.x * GATHER_CHUNK;
-    if (q0 >= edgeCount) return;
+    const u64 nchunks = (edgeCount + GATHER_CHUNK - 1) / GATHER_CHUNK;
+    for (u64 chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const u64 q0 = chunk * GATHER_CHUNK;
     const u64 q1 = (q0 + GATHER_CHUNK) < edgeCount ? (q0 + GATHER_CHUNK) : edgeCount;
     if (tid < WAVE) {
         u64 ka = wave_search_le(ptr, kbeg, kend - 1, edgeBase + q0);
@@ -1591,6 +1592,7 @@ __global__ __launch_bounds__(GATHER_BLOCK) void gather_kernel(const u32 *nodes, 
             if (at[u] != ~0ull) dst[at[u]] = buf[u];
         __syncthreads();
         k0 = kk1 + 1;
+    }
     }
 }
 
@@ -1794,8 +1796,17 @@ void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *active
 {
     if (edgeCount == 0) return;
     if (kend <= kbeg) throw Error(ERR_ARG, "gather over an empty vertex range");
-    u64 blocks = (edgeCount + GATHER_CHUNK - 1) / GATHER_CHUNK;
-    if (blocks > 0x7FFFFFFFull) throw Error(ERR_ARG, "gather too large");
+    // a few workgroups loop over the chunks: enough host reads in flight to
+    // keep the link busy, while the compute stream's persistent kernels,
+    // sized one slot per CU below full occupancy, find their slots free
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        SUBWAY_CHECK(hipGetDevice(&dev));
+        SUBWAY_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const u64 chunks = (edgeCount + GATHER_CHUNK - 1) / GATHER_CHUNK;
+    const u64 blocks = std::min<u64>(chunks, (u64)std::max(1, cus / 2));
     if (edgeBytes == 8)
         hipLaunchKernelGGL((gather_kernel<OutEdgeWeighted>), dim3((unsigned)blocks), dim3(GATHER_BLOCK), 0,
                            st, activeNodes, activeNodesPointer, (u64)kbeg, (u64)kend, nodePointer,

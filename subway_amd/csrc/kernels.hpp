@@ -205,7 +205,7 @@ struct RoundsLaunch {
     uint32_t maxRounds;
     uint32_t roundBase;
     uint32_t idBase;       // nodes == null: a whole id range starting here
-    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 20 us
+    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 500 us
     uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
@@ -235,7 +235,7 @@ struct BatchLaunch {
     dev::PersistCtl *pc;
     uint32_t maxRounds;
     uint32_t idBase;
-    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 20 us
+    uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 500 us
     uint32_t packCode;
 };
 int batch_max_grid(int algo, int edgeFmt, int unroll);

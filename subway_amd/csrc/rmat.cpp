@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -73,6 +75,20 @@ static inline uint32_t rmat_weight(const RmatParams &p, uint64_t e)
     return p.wmin + (uint32_t)(h % p.wspan);
 }
 
+// Graphs of 2^31 nominal edges and more take minutes to build: each pass
+// reports its end on stderr.
+struct RmatLog {
+    bool on;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    void pass(const char *what) const
+    {
+        if (!on) return;
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "[rmat] %s done at %.1f s\n", what, s);
+        std::fflush(stderr);
+    }
+};
+
 template <class E>
 std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
                                      bool symmetrize, uint32_t wmin, uint32_t wmax)
@@ -101,6 +117,7 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
     const uint64_t NB = 1ull << bbits;
     const int bshift = scale - bbits;
     const int copies = symmetrize ? 2 : 1;
+    const RmatLog log{p.nominal >= (1ull << 31)};
 
     // pass 1: per-thread bucket counts
     std::vector<uint64_t> cnt((size_t)T * NB, 0);
@@ -117,6 +134,7 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
         }
         kept[t] = k;
     }, T);
+    log.pass("pass 1 of 3 (bucket counts)");
     // bucket offsets: bucket-major, then thread order (keeps generation order)
     std::vector<uint64_t> off((size_t)T * NB);
     std::vector<uint64_t> bstart(NB + 1);
@@ -145,6 +163,7 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
             if (symmetrize) tmp[o[d >> bshift]++] = Rec{d, s, w};
         }
     }, T);
+    log.pass("pass 2 of 3 (edges into buckets)");
 
     auto csr = std::make_shared<HostCsr<E>>();
     csr->allocate(n, m);
@@ -178,6 +197,7 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
         });
     }
     for (auto &th : pool) th.join();
+    log.pass("pass 3 of 3 (CSR)");
     tmp.reset();
     csr->finalize();
     return csr;
