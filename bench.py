@@ -381,6 +381,10 @@ def main():
             "scheduler_network": "subway_amd/data/scheduler_mlp.f32" if params is not None else None,
         },
         "policies": {k: v[0] for k, v in results.items()},
+        "learned_trace": ([{"iteration": r["iteration"], "features": [round(f, 5) for f in r["features"]],
+                            "heads": [round(h, 5) for h in r["heads"]], "theta": r["theta"],
+                            "rounds_cap": r["rounds_cap"], "on_plain": r["on_plain"]}
+                           for r in stats[-1].trace] if head == "learned" else None),
         "learned_speedup_vs_plain": (round(results["plain"][0]["ms_per_step"] / results["learned"][0]["ms_per_step"], 4)
                                      if len(results) == 2 else None),
         "same_result_learned_vs_plain": same,

@@ -450,6 +450,7 @@ void Engine<E, G>::generateAndPartition()
         pa.E = g_->num_edges;
         pa.bufEdges = maxPart_;
         pa.iteration = st_.globalIterations + 1;
+        pa.pageRank = isPR ? 1u : 0u;
         pa.out = reinterpret_cast<PolicyOut *>(d_pol_);
         pa.rstats = reinterpret_cast<ull *>(d_pol_ + sizeof(PolicyOut));
         launch_policy(pa, sc_);
@@ -590,8 +591,8 @@ void Engine<E, G>::applyPolicy(IterRecord &rec)
             const float r = (predEdges_ + 1e-6f) / (actual + 1e-6f);
             judged_++;
             if (r > 2.f || r < 0.5f) st_.mispredictions++;
-            // more than half of at least three judged predictions missed
-            if (st_.mispredictions >= 3 && 2 * st_.mispredictions > judged_) {
+            // more than half of at least eight judged predictions missed
+            if (judged_ >= 8 && 2 * st_.mispredictions > judged_) {
                 onPlain_ = true;
                 st_.fallbackIteration = st_.globalIterations;
             }

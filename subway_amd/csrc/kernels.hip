@@ -976,10 +976,17 @@ __device__ __forceinline__ bool grid_sync(PersistCtl *pc, u32 &gen, u32 limit, u
         bool releaser = false;
         u32 a = fetch_add_relaxed(&pc->bar_sub[grp * 16], 1u) + 1;
         if (a == grpSize) {
+            // each reset must be performed before the signal that follows
+            // it: relaxed operations on different addresses may complete
+            // out of order, and a reset landing after the release would
+            // erase an arrival at the next barrier (which then never
+            // completes)
             st_relaxed(&pc->bar_sub[grp * 16], 0u);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             u32 t = fetch_add_relaxed(&pc->bar_top[0], 1u) + 1;
             if (t == ngroups) {
                 st_relaxed(&pc->bar_top[0], 0u);
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 st_relaxed(&pc->bar_gen[0], g + 1);
                 releaser = true;
             }
@@ -1498,7 +1505,7 @@ __global__ __launch_bounds__(256) void policy_kernel(PolicyArgs a)
         }
         float x[MLP_IN];
         x[0] = log2f(1.f + (float)A) / fmaxf(log2f(V), 1.f);
-        x[1] = (float)A / V;
+        x[1] = a.pageRank ? 1.f : 0.f;   // PR pushes every edge of every selected vertex
         x[2] = (float)EA / E;
         x[3] = log2f(1.f + (float)EA) / fmaxf(log2f(E), 1.f);
         for (int b = 0; b < 4; b++) x[4 + b] = hsum > 0.f ? h[b] / hsum : 0.f;

@@ -136,6 +136,7 @@ struct PolicyArgs {
     const float *params;             // MLP parameters or null (features only)
     ull V, E, bufEdges;
     uint32_t iteration;
+    uint32_t pageRank;               // 1: the run is PageRank (a feature of its own)
     PolicyOut *out;
     ull *rstats;                     // P x {lo, hi, active edges, first offset}
 };

@@ -32,7 +32,7 @@ def test_features_of_first_iteration(sw, gpu):
     f = np.array(st.trace[0]["features"], np.float64)
     a, ea = g.n, int(deg.sum())                    # every vertex is labelled, edges or not
     assert st.trace[0]["active_nodes"] == a and st.trace[0]["active_edges"] == ea
-    np.testing.assert_allclose(f[1], a / g.n, rtol=1e-6)
+    assert f[1] == 0.0                             # not PageRank
     np.testing.assert_allclose(f[2], ea / g.m, rtol=1e-6)
     np.testing.assert_allclose(f[0], np.log2(1 + a) / np.log2(g.n), rtol=1e-5)
     d = deg
@@ -91,25 +91,28 @@ def test_any_network_keeps_results_exact(sw, gpu, algo):
 
 
 def test_mispredictions_fall_back_to_plain(sw, gpu):
-    """A network that always predicts the whole graph active next misses
-    every prediction after the first iteration; after three misses (more than
-    half of those judged) the run goes to the plain schedule."""
+    """A network that always predicts an empty next frontier misses every
+    prediction while PR-async (1e-6 threshold) keeps much of the graph
+    active; once at least eight predictions have been judged and more than
+    half of them missed, the run goes to the plain schedule for good."""
     from subway_amd import policy
-    g = _rmat(sw, 15, 4)
+    g = _rmat(sw, 14, 4, weighted=False)
     p = policy.init_params(1)
     p[policy.NORM:] = 0.0
-    p[-3] = 0.0          # log2(next active-edge fraction) / 32: always the whole graph
+    p[-3] = -1.0         # log2(next active-edge fraction) / 32: always 2^-32, nothing active
     p[-2] = 0.5          # theta 0.5
     p[-1] = 0.375        # cap 2^6
-    ptr, adj, w = g.csr()
-    src = g.max_degree_vertex
-    vals, st = sw.run(g, "sssp", mode="async", source=src, budget=256 << 10, policy="learned", params=p)
-    assert np.array_equal(vals, O.sssp(ptr, adj, w, src))
-    assert len(st.trace) >= 5
-    assert st.mispredictions >= 3 and st.fallback_iteration == 4
+    ptr, adj, _ = g.csr()
+    vals, st = sw.run(g, "pr", mode="async", budget=128 << 10, threshold=1e-6, policy="learned", params=p)
+    exp, _ = O.pr(ptr, adj, 1e-6)
+    assert (np.abs(vals.astype(np.float64) - exp) / exp).max() < 1e-4
     on = [r["on_plain"] for r in st.trace]
+    assert 1 in on, (st.mispredictions, len(on))
     k = on.index(1)
     assert all(on[k:]) and not any(on[:k])
+    assert st.fallback_iteration == k + 1 and k >= 8
+    # the k predictions judged before the fallback: more than half missed
+    assert 2 * st.mispredictions > k
 
 
 def test_trained_network_is_shipped_and_exact(sw, gpu):
