@@ -45,11 +45,18 @@ struct HostCsr {
     uint32_t packShift = 0;         // 0: no packed copy (unweighted, or ids + weight span exceed 32 bits)
     uint32_t packBase = 0;          // smallest weight, added back when an edge is decoded
     uint32_t ensurePacked();        // builds the packed copy once; returns packShift
+    // Weighted graphs that do not pack but whose weights fit a byte: the
+    // destinations and the weights as two pinned, device-mapped arrays (5
+    // bytes per edge instead of 8), built on first use.
+    uint32_t *splitDst = nullptr, *d_splitDst = nullptr;
+    uint8_t *splitW = nullptr, *d_splitW = nullptr;
+    bool ensureSplit();             // builds the split copy once; false when it does not apply
     HostCsr() = default;
     HostCsr(const HostCsr &) = delete;
 private:
     std::mutex packMu_;
     bool packTried_ = false;
+    bool splitTried_ = false;
 public:
     HostCsr &operator=(const HostCsr &) = delete;
     ~HostCsr();

@@ -184,6 +184,17 @@ public:
                 fmt_ = EDGE_PACKED;
                 eb_ = sizeof(uint32_t);
                 maxPart_ = std::min<ull>(maxPart_ * (sizeof(E) / sizeof(uint32_t)), 0x7FFFF800ull);
+            } else if (!o_.unpackedEdges && g_->csr->ensureSplit()) {
+                // ids too wide to pack, weights in a byte: a u32 destination
+                // array and a u8 weight array per buffer, 5 bytes per edge
+                ull cap = maxPart_ * sizeof(E) / 5;
+                cap = std::min<ull>(cap - cap % ADV_CHUNK, 0x7FFFF800ull);
+                if (cap > maxPart_) {
+                    fmt_ = EDGE_SPLIT;
+                    eb_ = 5;
+                    maxPart_ = cap;
+                    packCode_ = (uint32_t)cap;
+                }
             }
         }
         alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
@@ -354,7 +365,9 @@ private:
     void frontier(int first, uint8_t *labA, uint8_t *labB, const Unit &un);
     void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf);
     void asyncRounds(const Unit &un, int buf);
-    void hostGather(const Unit &u, int b);
+    // gathers the unit's active adjacency, elements of esz bytes from src,
+    // into buffer b's staging area at byte stageOffset
+    void hostGather(const Unit &u, int b, const char *src, size_t esz, size_t stageOffset);
     // host edge array in the transfer format (pinned)
     const char *hostEdges() const
     {
@@ -784,21 +797,47 @@ void Engine<E, G>::issue(size_t idx)
     SUBWAY_CHECK(hipStreamWaitEvent(sx_, evFree_[b], 0));
     if (u.count) {
         if (o_.profile) pXfer_.begin(sx_);
-        void *dst = sub_->d_buffers[b];
+        char *dst = (char *)sub_->d_buffers[b];
         const size_t bytes = (size_t)u.count * (size_t)eb_;
+        const bool split = fmt_ == EDGE_SPLIT;
+        // split format: destinations at the buffer's start, weights at byte 4 * capacity
+        char *dstW = split ? dst + 4ull * maxPart_ : nullptr;
         if (u.dma) {
-            SUBWAY_CHECK(hipMemcpyAsync(dst, hostEdges() + u.hostEdge * (ull)eb_, bytes, hipMemcpyHostToDevice,
-                                        sx_));
+            if (split) {
+                SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->splitDst + u.hostEdge, 4ull * u.count,
+                                            hipMemcpyHostToDevice, sx_));
+                SUBWAY_CHECK(hipMemcpyAsync(dstW, g_->csr->splitW + u.hostEdge, u.count, hipMemcpyHostToDevice,
+                                            sx_));
+            } else {
+                SUBWAY_CHECK(hipMemcpyAsync(dst, hostEdges() + u.hostEdge * (ull)eb_, bytes,
+                                            hipMemcpyHostToDevice, sx_));
+            }
             st_.h2dDmaBytes += bytes;
         } else if (o_.transfer == TRANSFER_HOST) {
-            hostGather(u, b);
-            SUBWAY_CHECK(hipMemcpyAsync(dst, sub_->stagingBuffer(b), bytes, hipMemcpyHostToDevice, sx_));
+            const char *stage = (const char *)sub_->stagingBuffer(b);
+            if (split) {
+                hostGather(u, b, (const char *)g_->csr->splitDst, 4, 0);
+                hostGather(u, b, (const char *)g_->csr->splitW, 1, 4ull * u.count);
+                SUBWAY_CHECK(hipMemcpyAsync(dst, stage, 4ull * u.count, hipMemcpyHostToDevice, sx_));
+                SUBWAY_CHECK(hipMemcpyAsync(dstW, stage + 4ull * u.count, u.count, hipMemcpyHostToDevice, sx_));
+            } else {
+                hostGather(u, b, hostEdges(), (size_t)eb_, 0);
+                SUBWAY_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, sx_));
+            }
             SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
             st_.h2dDmaBytes += bytes;
         } else {
-            const void *mapped = fmt_ == EDGE_PACKED ? (const void *)g_->csr->d_packed : (const void *)g_->d_edgeList;
-            launch_gather(eb_, sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend, g_->d_nodePointer,
-                          mapped, dst, u.edgeBase, u.count, sx_);
+            if (split) {
+                launch_gather(4, sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend, g_->d_nodePointer,
+                              g_->csr->d_splitDst, dst, u.edgeBase, u.count, sx_);
+                launch_gather(1, sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend, g_->d_nodePointer,
+                              g_->csr->d_splitW, dstW, u.edgeBase, u.count, sx_);
+            } else {
+                const void *mapped =
+                    fmt_ == EDGE_PACKED ? (const void *)g_->csr->d_packed : (const void *)g_->d_edgeList;
+                launch_gather(eb_, sub_->d_activeNodes, sub_->d_activeNodesPointer, u.kbeg, u.kend,
+                              g_->d_nodePointer, mapped, dst, u.edgeBase, u.count, sx_);
+            }
             st_.h2dGatherBytes += bytes;
         }
         st_.h2dBytes += bytes;
@@ -813,7 +852,7 @@ void Engine<E, G>::issue(size_t idx)
 // whose destination offsets come from a parallel prefix of chunk edge sums;
 // the result lands in the pinned staging buffer of device buffer b.
 template <class E, class G>
-void Engine<E, G>::hostGather(const Unit &u, int b)
+void Engine<E, G>::hostGather(const Unit &u, int b, const char *src, size_t esz, size_t stageOffset)
 {
     if (!hostIdsValid_) {
         const ull A = sub_->numActiveNodes;
@@ -823,12 +862,10 @@ void Engine<E, G>::hostGather(const Unit &u, int b)
     }
     // the staging buffer is reused only after the copy out of it finished
     SUBWAY_CHECK(hipEventSynchronize(evStaged_[b]));
-    char *stage = (char *)sub_->stagingBuffer(b);
+    char *stage = (char *)sub_->stagingBuffer(b) + stageOffset;
     const uint *ids = h_ids_ + u.kbeg;
     const ull cnt = u.kend - u.kbeg;
     const ull *np = g_->nodePointer;
-    const char *src = hostEdges();
-    const size_t esz = (size_t)eb_;
     const int T = std::max(1, std::min(host_threads(), 64));
     const ull nch = std::min<ull>(cnt, (ull)T * 8);
     std::vector<ull> chunkEdges(nch + 1, 0);

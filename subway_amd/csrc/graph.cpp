@@ -49,6 +49,8 @@ HostCsr<E>::~HostCsr()
         else std::free(edges);
     }
     if (packed) (void)hipHostFree(packed);
+    if (splitDst) (void)hipHostFree(splitDst);
+    if (splitW) (void)hipHostFree(splitW);
 }
 
 template <class E>
@@ -93,6 +95,51 @@ uint32_t HostCsr<E>::ensurePacked()
         packBase = base;
         packShift = db;
         return db;
+    }
+}
+
+template <class E>
+bool HostCsr<E>::ensureSplit()
+{
+    if constexpr (!std::is_same<E, OutEdgeWeighted>::value) {
+        return false;
+    } else {
+        std::lock_guard<std::mutex> lock(packMu_);
+        if (splitTried_) return splitDst != nullptr;
+        splitTried_ = true;
+        if (!pinned || m == 0) return false;
+        const int T = host_threads();
+        std::vector<uint32_t> mx((size_t)T, 0);
+        parallel_for(m, [&](uint64_t b, uint64_t e, int t) {
+            uint32_t x = 0;
+            for (uint64_t i = b; i < e; i++) x = std::max(x, edges[i].w8);
+            mx[(size_t)t] = std::max(mx[(size_t)t], x);
+        }, T);
+        if (*std::max_element(mx.begin(), mx.end()) > 0xFFu) return false;
+        uint32_t *d = nullptr;
+        uint8_t *w = nullptr;
+        const unsigned flags = hipHostMallocMapped | hipHostMallocPortable;
+        hipError_t e = hipHostMalloc((void **)&d, (size_t)m * sizeof(uint32_t), flags);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&w, (size_t)m, flags);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (d) (void)hipHostFree(d);
+            // no room for the split copy: transfers keep the 8-byte edges
+            std::fprintf(stderr, "subway: pinned split copy of %llu edges not allocated (%s); 8-byte transfers\n",
+                         (unsigned long long)m, hipGetErrorString(e));
+            return false;
+        }
+        parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+            for (uint64_t i = b; i < e; i++) {
+                d[i] = edges[i].end;
+                w[i] = (uint8_t)edges[i].w8;
+            }
+        }, T);
+        splitDst = d;
+        splitW = w;
+        SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_splitDst, splitDst, 0));
+        SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_splitW, splitW, 0));
+        return true;
     }
 }
 

@@ -760,6 +760,20 @@ struct Codec<PackedEdge> {
     }
 };
 
+template <>
+struct Codec<SplitEdge> {
+    typedef OutEdgeWeighted T;
+    // cap = the buffer's capacity in edges: the weight of edge i is byte i of
+    // the array that starts at byte 4 * cap
+    __device__ __forceinline__ static T get(const SplitEdge *p, u64 i, u32 cap)
+    {
+        T e;
+        e.end = p[i].dst;
+        e.w8 = reinterpret_cast<const uint8_t *>(p)[4ull * cap + i];
+        return e;
+    }
+};
+
 // Relaxes the edges of one work chunk [cbeg, cbeg + ADV_CHUNK) clipped to
 // [w0, wend): the chunk's entries are staged in LDS and every position finds
 // its entry by binary search.  Returns the successful relaxations of the
@@ -1539,6 +1553,9 @@ __global__ __launch_bounds__(256) void policy_kernel(PolicyArgs a)
 // vertex that owns them through LDS-staged segment starts; sources are read
 // straight from mapped pinned host memory.
 constexpr int GATHER_SEGS = 1024;
+struct EdgeByte {
+    uint8_t v;   // one weight of the split format
+};
 
 template <class E>
 __global__ __launch_bounds__(GATHER_BLOCK) void gather_kernel(const u32 *nodes, const u64 *ptr,
@@ -1821,10 +1838,16 @@ void launch_gather(int edgeBytes, const uint32_t *activeNodes, const ull *active
                            st, activeNodes, activeNodesPointer, (u64)kbeg, (u64)kend, nodePointer,
                            (const OutEdgeWeighted *)hostEdges, (OutEdgeWeighted *)dst, (u64)edgeBase,
                            (u64)edgeCount);
-    else
+    else if (edgeBytes == 4)
         hipLaunchKernelGGL((gather_kernel<OutEdge>), dim3((unsigned)blocks), dim3(GATHER_BLOCK), 0, st,
                            activeNodes, activeNodesPointer, (u64)kbeg, (u64)kend, nodePointer,
                            (const OutEdge *)hostEdges, (OutEdge *)dst, (u64)edgeBase, (u64)edgeCount);
+    else if (edgeBytes == 1)
+        hipLaunchKernelGGL((gather_kernel<EdgeByte>), dim3((unsigned)blocks), dim3(GATHER_BLOCK), 0, st,
+                           activeNodes, activeNodesPointer, (u64)kbeg, (u64)kend, nodePointer,
+                           (const EdgeByte *)hostEdges, (EdgeByte *)dst, (u64)edgeBase, (u64)edgeCount);
+    else
+        throw Error(ERR_ARG, "gather element size must be 1, 4 or 8 bytes");
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -1862,6 +1885,8 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
     do {                                                                                               \
         if ((fmt) == EDGE_PACKED) {                                                                    \
             L<ALGO, PackedEdge>(__VA_ARGS__);                                                          \
+        } else if ((fmt) == EDGE_SPLIT) {                                                              \
+            L<ALGO, SplitEdge>(__VA_ARGS__);                                                           \
         } else if ((fmt) == EDGE_W64) {                                                                \
             L<ALGO, OutEdgeWeighted>(__VA_ARGS__);                                                     \
         } else if constexpr (ALGO != ALGO_SSSP && ALGO != ALGO_SSWP) {                                 \

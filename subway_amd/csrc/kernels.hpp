@@ -40,16 +40,21 @@ constexpr int GATHER_CHUNK = 4096;
 // values are their sizes in bytes.  EDGE_PACKED stores a weighted edge in one
 // word, destination | (weight - base) << shift, for graphs whose vertex ids
 // fit `shift` bits and whose weight span fits the remaining 32 - shift bits;
-// kernels take the decode word packCode = shift | base << 5.
+// kernels take the decode word packCode = shift | base << 5.  EDGE_SPLIT
+// keeps a buffer of capacity C edges as C u32 destinations followed, at byte
+// 4 C, by C u8 weights (5 bytes per edge); kernels take packCode = C.
 enum EdgeFormat : int {
     EDGE_U32 = 4,      // OutEdge
     EDGE_W64 = 8,      // OutEdgeWeighted
     EDGE_PACKED = 5,   // PackedEdge
+    EDGE_SPLIT = 6,    // SplitEdge
 };
 struct PackedEdge {
     uint32_t bits;
 };
-inline int edge_format_bytes(int fmt) { return fmt == EDGE_W64 ? 8 : 4; }
+struct SplitEdge {
+    uint32_t dst;      // element of the destination array; the weights follow the array
+};
 
 struct Entry {
     uint32_t edge_local;

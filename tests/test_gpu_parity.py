@@ -306,8 +306,34 @@ def test_packed_weight_span(sw, gpu, algo):
     assert np.array_equal(vp, vu)
 
 
+@pytest.mark.parametrize("transfer", ["auto", "zerocopy", "host"])
+@pytest.mark.parametrize("algo,mode", [("sssp", "async"), ("sswp", "async"), ("sssp", "sync")])
+def test_split_edges(sw, gpu, algo, mode, transfer):
+    """Ids of 25 bits leave 7 for the weights and the weights 1 .. 255 need
+    8: the edges do not pack, and move as a u32 destination array plus a u8
+    weight array (5 bytes per edge), by DMA, GPU gather and host gather,
+    including a vertex larger than a buffer; values equal the oracle's and
+    the 8-byte format's."""
+    rng = np.random.default_rng(31)
+    n = 1 << 25
+    m = 60_000
+    src = np.concatenate([rng.integers(0, 4096, m), np.full(10_000, 7)]).astype(np.uint32)
+    dst = np.concatenate([rng.integers(0, 8192, m), rng.integers(0, n, 10_000)]).astype(np.uint32)
+    w = rng.integers(1, 256, len(src)).astype(np.uint32)
+    w[:2] = (1, 255)
+    g = sw.Graph.from_edges(n, src, dst, w)
+    kw = dict(budget=64 << 10, transfer=transfer)
+    vs, st = check_app(sw, g, algo, mode, src=7, **kw)
+    assert st.edge_bytes == 5 and st.partitions > st.global_iterations
+    if mode == "async":
+        assert st.trace[0]["h2d_bytes"] == 5 * g.m
+    vu, su = check_app(sw, g, algo, mode, src=7, packed=False, **kw)
+    assert su.edge_bytes == 8 and np.array_equal(vs, vu)
+
+
 def test_unpackable_weights_fall_back(sw, gpu):
-    """Weights that do not fit beside the ids keep the 8-byte format."""
+    """Weights that fit neither beside the ids nor in a byte keep the 8-byte
+    format."""
     rng = np.random.default_rng(8)
     n = 1 << 20                               # ids need 20 bits, 12 left for weights
     src = rng.integers(0, 4096, 30_000).astype(np.uint32)
