@@ -60,7 +60,10 @@ private:
 public:
     HostCsr &operator=(const HostCsr &) = delete;
     ~HostCsr();
-    void allocate(ull n, ull m);   // allocates ptr, edges, degree
+    // allocates ptr, edges, degree; pinEdges = false keeps the edge list in
+    // ordinary memory (a graph that moves only in the split format)
+    void allocate(ull n, ull m, bool pinEdges = true);
+    bool allocateSplit();          // pinned split arrays, filled by the caller; false if not allocated
     void finalize();               // degrees, maxDegree, device mapping
 };
 

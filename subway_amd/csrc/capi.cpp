@@ -472,6 +472,7 @@ static void gather_on(const HostCsr<E> &csr, uint64_t A, const uint32_t *nodes, 
     ptr[A] = run;
     if (num_edges) *num_edges = run;
     if (!run || !out) return;
+    if (!csr.pinned || !csr.d_edges) throw Error(ERR_ARG, "the graph's edge list is not pinned (split-format graph)");
     DevBuf<ull> dp(A + 1), np(csr.n + 1);
     DevBuf<uint32_t> dn(A);
     DevBuf<E> dst(run);

@@ -197,6 +197,9 @@ public:
                 }
             }
         }
+        if (!g_->csr->pinned && fmt_ != EDGE_SPLIT)
+            throw Error(ERR_ARG, "this graph's edge list is not pinned: it moves only in the split format "
+                                 "(unpacked_edges must be 0)");
         alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
         ull maxEntries = std::min<ull>((ull)g_->num_nodes, maxPart_) + 1;
         SUBWAY_CHECK(hipMalloc(&entries_, maxEntries * sizeof(Entry)));

@@ -144,7 +144,29 @@ bool HostCsr<E>::ensureSplit()
 }
 
 template <class E>
-void HostCsr<E>::allocate(ull n_, ull m_)
+bool HostCsr<E>::allocateSplit()
+{
+    std::lock_guard<std::mutex> lock(packMu_);
+    splitTried_ = true;
+    const unsigned flags = hipHostMallocMapped | hipHostMallocPortable;
+    uint32_t *d = nullptr;
+    uint8_t *w = nullptr;
+    hipError_t e = hipHostMalloc((void **)&d, (size_t)(m ? m : 1) * sizeof(uint32_t), flags);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&w, (size_t)(m ? m : 1), flags);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (d) (void)hipHostFree(d);
+        return false;
+    }
+    splitDst = d;
+    splitW = w;
+    SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_splitDst, splitDst, 0));
+    SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_splitW, splitW, 0));
+    return true;
+}
+
+template <class E>
+void HostCsr<E>::allocate(ull n_, ull m_, bool pinEdges)
 {
     if (n_ == 0) throw Error(ERR_FORMAT, "graph has no vertices");
     if (n_ > 0xFFFFFFFFull) throw Error(ERR_FORMAT, "vertex ids are 32-bit; graph has too many vertices");
@@ -157,6 +179,12 @@ void HostCsr<E>::allocate(ull n_, ull m_)
         throw Error(ERR_NOMEM, "host allocation of the CSR offsets failed");
     }
     size_t bytes = (size_t)(m ? m : 1) * sizeof(E);
+    if (!pinEdges) {
+        edges = static_cast<E *>(std::malloc(bytes));
+        if (!edges) throw Error(ERR_NOMEM, "host allocation of the edge list failed");
+        pinned = false;
+        return;
+    }
     hipError_t e = hipHostMalloc((void **)&edges, bytes, hipHostMallocMapped | hipHostMallocPortable);
     if (e != hipSuccess) {
         edges = nullptr;
@@ -452,7 +480,8 @@ void Graph<E>::ReadGraph()
 template <class E>
 void Graph<E>::attach()
 {
-    if (!csr->pinned) throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
+    if (!csr->pinned && !csr->splitDst)
+        throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
     num_nodes = (uint)csr->n;
     num_edges = csr->m;
     isLarge = csr->large;
@@ -530,7 +559,8 @@ void GraphPR<E>::ReadGraph()
 template <class E>
 void GraphPR<E>::attach()
 {
-    if (!csr->pinned) throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
+    if (!csr->pinned && !csr->splitDst)
+        throw subway::Error(subway::ERR_ARG, "the graph's edge list is not in pinned memory (no GPU)");
     num_nodes = (uint)csr->n;
     num_edges = csr->m;
     isLarge = csr->large;

@@ -166,7 +166,20 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
     log.pass("pass 2 of 3 (edges into buckets)");
 
     auto csr = std::make_shared<HostCsr<E>>();
-    csr->allocate(n, m);
+    // A large weighted graph whose ids and weight span do not fit one word
+    // but whose weights fit a byte moves in the split format only: its
+    // pinned memory is the split arrays (5 bytes per edge), and the 8-byte
+    // list stays in ordinary memory for host-side use.  Pinning both does
+    // not fit a host that pins about 70 GB per process (scale 28: 68.7 GB
+    // of 8-byte edges).
+    bool split = false;
+    if constexpr (sizeof(E) == 8) {
+        const bool packs = (uint64_t)(wmax - wmin) < (1ull << (32 - std::min(scale, 31)));
+        split = !packs && wmax <= 0xFFu && (uint64_t)m * sizeof(E) >= (1ull << 34);
+    }
+    csr->allocate(n, m, !split);
+    if (split && !csr->allocateSplit())
+        throw Error(ERR_NOMEM, "pinned allocation of the split edge arrays failed");
     csr->large = m > 0xFFFFFFFFull;
     // pass 3: per bucket, stable counting sort by source
     std::atomic<uint64_t> nextBucket{0};
@@ -192,6 +205,10 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
                     uint64_t at = deg[rc.s - v0]++;
                     csr->edges[at].end = rc.d;
                     if constexpr (sizeof(E) == 8) reinterpret_cast<uint32_t *>(&csr->edges[at])[1] = rc.w;
+                    if (split) {
+                        csr->splitDst[at] = rc.d;
+                        csr->splitW[at] = (uint8_t)rc.w;
+                    }
                 }
             }
         });
