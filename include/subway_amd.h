@@ -136,7 +136,11 @@ int32_t subway_load(const char *path, int32_t weighted, subway_graph **out);
 /* Build from caller arrays: node_pointer[n+1], ends[m], weights[m] or NULL. */
 int32_t subway_graph_from_csr(uint64_t n, uint64_t m, const uint64_t *node_pointer, const uint32_t *ends,
                               const uint32_t *weights, subway_graph **out);
-/* Reproducible R-MAT (a, b, c; d = 1-a-b-c), self-loops dropped. */
+/* Reproducible R-MAT (a, b, c; d = 1-a-b-c), self-loops dropped.  weighted:
+ * 0 no, 1 yes, 2 yes and laid out as a graph that moves only in the split
+ * format (pinned u32 destinations + u8 weights, the 8-byte list in ordinary
+ * memory), which large weighted graphs that do not pack get by themselves;
+ * needs weights that fit a byte. */
 int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
                           int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax,
                           subway_graph **out);

@@ -147,9 +147,14 @@ class Graph:
     @classmethod
     def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
              seed: int = 42, weighted: bool = False, symmetrize: bool = False, wmin: int = 1,
-             wmax: int = 64) -> "Graph":
+             wmax: int = 64, split_layout: bool = False) -> "Graph":
+        """split_layout: lay a weighted graph out as one that moves only in the
+        split format (pinned destination and weight arrays, the 8-byte list in
+        ordinary memory), as large weighted graphs that do not pack are by
+        themselves."""
         h = ctypes.c_void_p()
-        check(lib.subway_graph_rmat(scale, edge_factor, a, b, c, seed, int(weighted), int(symmetrize),
+        w = (2 if split_layout else 1) if weighted else 0
+        check(lib.subway_graph_rmat(scale, edge_factor, a, b, c, seed, w, int(symmetrize),
                                     wmin, wmax, ctypes.byref(h)))
         return cls(h)
 

@@ -18,7 +18,7 @@
 namespace subway {
 template <class E>
 std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
-                                     bool symmetrize, uint32_t wmin, uint32_t wmax);
+                                     bool symmetrize, uint32_t wmin, uint32_t wmax, bool splitLayout);
 }
 
 using namespace subway;
@@ -142,9 +142,10 @@ int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b
         std::unique_ptr<subway_graph> g(new subway_graph);
         g->weighted = weighted != 0;
         if (g->weighted)
-            g->w = rmat_csr<OutEdgeWeighted>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax);
+            g->w = rmat_csr<OutEdgeWeighted>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax,
+                                             weighted == 2);
         else
-            g->u = rmat_csr<OutEdge>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax);
+            g->u = rmat_csr<OutEdge>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax, false);
         *out = g.release();
     });
 }

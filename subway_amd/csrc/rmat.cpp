@@ -91,7 +91,7 @@ struct RmatLog {
 
 template <class E>
 std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
-                                     bool symmetrize, uint32_t wmin, uint32_t wmax)
+                                     bool symmetrize, uint32_t wmin, uint32_t wmax, bool splitLayout)
 {
     if (scale < 1 || scale > 31) throw Error(ERR_ARG, "R-MAT scale must be in [1, 31]");
     if (edgeFactor < 1) throw Error(ERR_ARG, "R-MAT edge factor must be positive");
@@ -176,6 +176,10 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
     if constexpr (sizeof(E) == 8) {
         const bool packs = (uint64_t)(wmax - wmin) < (1ull << (32 - std::min(scale, 31)));
         split = !packs && wmax <= 0xFFu && (uint64_t)m * sizeof(E) >= (1ull << 34);
+        if (splitLayout) {
+            if (wmax > 0xFFu) throw Error(ERR_ARG, "the split layout needs weights that fit a byte");
+            split = true;
+        }
     }
     csr->allocate(n, m, !split);
     if (split && !csr->allocateSplit())
@@ -221,8 +225,9 @@ std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double
 }
 
 template std::shared_ptr<HostCsr<OutEdge>> rmat_csr<OutEdge>(int, int, double, double, double, uint64_t, bool,
-                                                             uint32_t, uint32_t);
+                                                             uint32_t, uint32_t, bool);
 template std::shared_ptr<HostCsr<OutEdgeWeighted>> rmat_csr<OutEdgeWeighted>(int, int, double, double, double,
-                                                                             uint64_t, bool, uint32_t, uint32_t);
+                                                                             uint64_t, bool, uint32_t, uint32_t,
+                                                                             bool);
 
 }  // namespace subway

@@ -331,6 +331,20 @@ def test_split_edges(sw, gpu, algo, mode, transfer):
     assert su.edge_bytes == 8 and np.array_equal(vs, vu)
 
 
+@pytest.mark.parametrize("algo,mode", [("sssp", "async"), ("sswp", "async"), ("sssp", "sync"), ("bfs", "async")])
+def test_split_layout_graph(sw, gpu, algo, mode):
+    """A graph laid out for the split format only (pinned destination and
+    weight arrays, the 8-byte list in ordinary memory — what large weighted
+    graphs that do not pack get from the generator) gives the oracle's
+    values in the split format and refuses 8-byte transfers."""
+    g = sw.Graph.rmat(15, 16, seed=5, weighted=True, split_layout=True)
+    src = g.max_degree_vertex
+    _, st = check_app(sw, g, algo, mode, src=src, budget=96 << 10)
+    assert st.edge_bytes == 5 and st.partitions > st.global_iterations
+    with pytest.raises(sw.SubwayError, match="split format"):
+        sw.run(g, algo, mode=mode, source=src, budget=96 << 10, packed=False)
+
+
 def test_unpackable_weights_fall_back(sw, gpu):
     """Weights that fit neither beside the ids nor in a byte keep the 8-byte
     format."""
