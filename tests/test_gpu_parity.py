@@ -469,3 +469,42 @@ def test_range_schedule_batched(sw, gpu, theta):
     vals, _ = bs.run(sources)
     for i in (0, 9, 63):
         assert np.array_equal(vals[i], O.sssp(ptr, adj, w, int(sources[i])))
+
+
+# ------------------------------------------------------ application binaries
+APP_BIN = os.path.join(os.path.dirname(HERE), "subway_amd", "bin")
+
+
+@pytest.mark.parametrize("app", [f"{a}-{m}" for a in ("bfs", "cc", "sssp", "sswp", "pr") for m in ("sync", "async")])
+def test_app_binaries(sw, gpu, tmp_path, app):
+    """The ten applications as the reference's users run them: --input a
+    converted binary graph, --source, --output; the output file holds one
+    "<vertex> <value>" line per vertex (ostream formatting, 6 significant
+    digits for PR) with the oracle's values."""
+    import subprocess
+    algo, mode = app.split("-")
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(12, 8, seed=3, weighted=weighted, symmetrize=(algo == "cc"))
+    path = tmp_path / ("g.bwcsr" if weighted else "g.bcsr")
+    g.save(str(path))
+    out = tmp_path / "out.txt"
+    src = g.max_degree_vertex if algo != "cc" else 0
+    cmd = [os.path.join(APP_BIN, app), "--input", str(path), "--output", str(out), "--budget", "64K"]
+    if not (algo == "cc" and mode == "sync"):
+        cmd += ["--source", str(src)]
+    if algo == "pr":
+        cmd += ["--threshold", "1e-6"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Processing finished in" in r.stdout
+    rows = [ln.split() for ln in out.read_text().splitlines()]
+    assert len(rows) == g.n and all(int(v) == i for i, (v, _) in enumerate(rows))
+    ptr, adj, w = g.csr()
+    if algo == "pr":
+        vals = np.array([float(x) for _, x in rows])
+        exp, _ = O.pr(ptr, adj, 1e-6)
+        assert (np.abs(vals - exp) / exp).max() < PR_RTOL + 1e-5     # plus the 6-digit printing
+    else:
+        vals = np.array([int(x) for _, x in rows], dtype=np.uint64)
+        assert np.array_equal(vals, O.expected(algo, ptr, adj, w, src).astype(np.uint64))
