@@ -1,14 +1,21 @@
 // kernels.hip -- gfx950 kernels of the out-of-memory graph pipeline.
 //
-//  * select_scan_kernel: one pass of flag -> workgroup scan -> decoupled
-//    lookback -> compaction.  It implements the subgraph generator
-//    (reference: prePrefix + thrust::reduce + two thrust::exclusive_scan +
-//    makeQueue + makeActiveNodesPointer, shared/subgraph_generator.cu:10-60,
-//    128-162) and the per-round frontier filter of every algorithm (the label
-//    test/clear at the top of each kernel in shared/gpu_kernels.cu, e.g.
-//    :23-28, together with mixLabels/moveUpLabels :515-534).
+//  * select_count / select_scan_chunks / select_emit kernels: the subgraph
+//    generator as reduce-then-scan (reference: prePrefix + thrust::reduce +
+//    two thrust::exclusive_scan + makeQueue + makeActiveNodesPointer,
+//    shared/subgraph_generator.cu:10-60, 128-162).
+//  * select_scan_kernel (and scan_tile inside the persistent kernels): one
+//    pass of flag -> workgroup scan -> decoupled lookback -> compaction, the
+//    per-round frontier filter of every algorithm (the label test/clear at
+//    the top of each kernel in shared/gpu_kernels.cu, e.g. :23-28, together
+//    with mixLabels/moveUpLabels :515-534).
 //  * advance_kernel: edge-balanced relaxation over the frontier's work space
 //    (the edge loops of shared/gpu_kernels.cu:42-56 and siblings).
+//  * rounds_kernel / rounds_batch_kernel: every async inner round of one
+//    resident partition in one persistent launch (shared/gpu_kernels.cu:257-502
+//    with the round loop of subway/sssp-async.cu:92-115), single-source and
+//    up to 64 sources.
+//  * policy_kernel: the learned scheduler's features and fused inference.
 //  * partition_kernel: greedy contiguous partitioning of the active list
 //    (shared/partitioner.cu:12-91).
 //  * gather_kernel: zero-copy gather of active adjacency lists from mapped
