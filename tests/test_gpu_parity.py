@@ -148,6 +148,31 @@ def test_apps_golden(sw, gpu, name, algo, mode):
     assert vals.tolist() == gd[algo]
 
 
+EDGE_CASES = {
+    # name: (n, edges (src, dst, w), source)
+    "single_vertex": (1, [], 0),
+    "no_edges": (5, [], 3),
+    "isolated_source": (6, [(0, 1, 3), (1, 2, 4), (2, 0, 1), (4, 5, 9)], 3),
+    "self_loops_only": (3, [(0, 0, 5), (1, 1, 2)], 0),
+    "sink_source": (4, [(0, 1, 1), (1, 2, 1), (2, 3, 1)], 3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+@pytest.mark.parametrize("algo,mode", APPS + [("pr", "sync"), ("pr", "async")])
+def test_apps_edge_cases(sw, gpu, case, algo, mode):
+    """Degenerate inputs: one vertex, no edges at all, a source without
+    edges, only self-loops, a source that is a sink; every application
+    matches the oracle (PR to its tolerance)."""
+    n, edges, source = EDGE_CASES[case]
+    src = np.array([e[0] for e in edges], np.uint32)
+    dst = np.array([e[1] for e in edges], np.uint32)
+    w = np.array([e[2] for e in edges], np.uint32) if algo in ("sssp", "sswp") else None
+    g = sw.Graph.from_edges(n, src, dst, w)
+    kw = {"threshold": 1e-6} if algo == "pr" else {}
+    check_app(sw, g, algo, mode, src=source if algo != "cc" else 0, **kw)
+
+
 @pytest.mark.parametrize("mode", ["sync", "async"])
 def test_pr_golden_readme(sw, gpu, mode):
     gd = hand()["readme"]
