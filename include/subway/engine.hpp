@@ -44,7 +44,7 @@ struct RunOptions {
     int advanceGrid = 0;    // persistent workgroups of the advance kernel; 0 = auto
     int maxRoundBatch = 32; // async: rounds issued between convergence checks
     int profile = 0;        // 1: time every kernel launch and transfer with HIP events
-    int persistent = 1;     // async: all rounds of a partition in one cooperative launch
+    int persistent = 1;     // async: all rounds of a partition in one persistent launch
     int roundsGrid = 0;     // workgroups of the persistent kernel; 0 (and the cap) = one below full occupancy per CU
     int maxRounds = 4096;     // async rounds per partition visit (labels left set resume next iteration)
     int batch = 0;          // 1: batched k-source queries (runBatch), BFS/SSSP/SSWP async

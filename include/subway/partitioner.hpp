@@ -40,6 +40,13 @@ public:
     // Variant reading the active count from device memory (no extra sync).
     void partition(Subgraph<E> &subgraph, const unsigned long long *d_numActive,
                    unsigned long long budgetEdges);
+    // The same in two halves, so the first can be recorded in a HIP graph:
+    // enqueue() issues the partition kernel for the first rows and the copies
+    // of the table to the host on `stream` (no synchronisation); collect()
+    // synchronises the stream, reads the table and finishes a table longer
+    // than one launch holds.
+    void enqueue(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long budgetEdges);
+    void collect(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long budgetEdges);
     void reset();
 
 private:
@@ -49,6 +56,11 @@ private:
     unsigned long long *h_out;
     void run(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long hostCount,
              unsigned long long budgetEdges);
+    void enqueueRows(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long hostCount,
+                     unsigned long long budgetEdges, unsigned long long start);
+    bool collectRows();
+    void finish(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long hostCount,
+                unsigned long long budgetEdges);
 };
 
 #endif  // SUBWAY_PARTITIONER_HPP

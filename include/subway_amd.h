@@ -57,9 +57,9 @@ typedef struct subway_options {
     int32_t verbose;        /* print per-partition progress lines */
     int32_t advance_grid;   /* persistent workgroups of the advance kernel; 0 = auto */
     int32_t profile;        /* 1: time every kernel launch and transfer with HIP events */
-    int32_t rounds_mode;    /* async: 0 = all rounds of a partition in one cooperative launch,
+    int32_t rounds_mode;    /* async: 0 = all rounds of a partition in one persistent launch,
                                1 = one frontier + advance launch pair per round */
-    int32_t rounds_grid;    /* workgroups of the cooperative rounds kernel; 0 (and the cap) = one below
+    int32_t rounds_grid;    /* workgroups of the persistent rounds kernel; 0 (and the cap) = one below
                                the co-resident maximum per CU */
     int32_t max_rounds;     /* async rounds per partition visit; 0 = 4096 */
     int32_t batch;          /* 1: batched session (BFS/SSSP/SSWP async, <= 64 sources per run) */
@@ -82,7 +82,7 @@ typedef struct subway_stats {
     double ms_advance;      /* advance kernel time (profile = 1) */
     double ms_frontier;     /* frontier kernel time (profile = 1) */
     double ms_generate;     /* generator + partition kernels (profile = 1) */
-    double ms_rounds;       /* cooperative rounds kernel (profile = 1) */
+    double ms_rounds;       /* persistent rounds kernel (profile = 1) */
     uint64_t h2d_bytes;     /* edge bytes moved host -> device */
     uint64_t h2d_dma_bytes;
     uint64_t h2d_gather_bytes;
@@ -177,7 +177,7 @@ const void *subway_session_values(const subway_session *s);
 int32_t subway_session_set_policy(subway_session *s, const float *params);
 void subway_session_free(subway_session *s);
 /* Batched session (options.batch = 1): k <= 64 sources advance together in
- * one cooperative kernel per transferred partition.  rows: n x 64 uint32,
+ * one persistent kernel per transferred partition.  rows: n x 64 uint32,
  * vertex-major (rows[v * 64 + i] is the value of v for sources[i]; lanes
  * i >= k hold the initial value).  No reference counterpart: the reference
  * runs k separate processes. */

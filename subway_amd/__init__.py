@@ -311,7 +311,7 @@ class Session:
 
 class BatchSession:
     """k <= 64 sources of BFS/SSSP/SSWP advancing together (async), one
-    cooperative kernel per transferred partition; :meth:`run` returns a
+    persistent kernel per transferred partition; :meth:`run` returns a
     (k, n) array (row i = values for sources[i])."""
 
     def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",

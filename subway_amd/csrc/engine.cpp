@@ -234,7 +234,7 @@ public:
     }
 
     // Batched k-source state: dist rows [V][64], two 64-bit mask arrays,
-    // per-entry masks, and a cooperative grid for the batched rounds.
+    // per-entry masks, and a persistent grid for the batched rounds.
     void setupBatch(const hipDeviceProp_t &prop, ull maxEntries)
     {
         if constexpr (isPR) {
@@ -266,7 +266,7 @@ public:
         hipDeviceProp_t prop;
         SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
         const int mx = batch_max_grid(o_.algo, fmt_, u);
-        if (mx <= 0) throw Error(ERR_ARG, "this device cannot run the cooperative batched rounds kernel");
+        if (mx <= 0) throw Error(ERR_ARG, "this device cannot run the persistent batched rounds kernel");
         batchUnroll_ = u;
         batchGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
     }
@@ -309,6 +309,7 @@ public:
         if (h_ctl_) (void)hipHostFree(h_ctl_);
         if (h_err_) (void)hipHostFree(h_err_);
         if (d_sum_) (void)hipFree(d_sum_);
+        if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
             (void)hipEventDestroy(evLoaded_[b]);
@@ -362,6 +363,8 @@ private:
 
     void initState();
     void generateAndPartition();
+    void errorCopies();
+    hipGraphExec_t iterExec_ = nullptr;   // batched runs: the recorded start of an iteration
     void buildUnits();
     void issue(size_t u);
     void compute(size_t u);
@@ -437,9 +440,49 @@ void Engine<E, G>::initState()
 }
 
 template <class E, class G>
+void Engine<E, G>::errorCopies()
+{
+    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[0], &fr_.ctl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[1], &gen_->scanScratch().ctl->error, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, sc_));
+    if (pc_) SUBWAY_CHECK(hipMemcpyAsync(&h_err_[2], &pc_->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+}
+
+template <class E, class G>
 void Engine<E, G>::generateAndPartition()
 {
     hostIdsValid_ = false;
+    if (batchK_ && !o_.profile && !d_bounds_) {
+        // A batched iteration starts with the same launches and copies every
+        // time (union generate, error words, partition table): recorded once
+        // as a HIP graph (no memset nodes) and replayed.  Not with the
+        // learned policy (its kernel takes the iteration index) or when
+        // profiling (events around the launches).
+        if (!iterExec_) {
+            hipGraph_t gr = nullptr;
+            SUBWAY_CHECK(hipStreamBeginCapture(sc_, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_generate_masks(g_->num_nodes, d_m1_, d_m2_, g_->d_outDegree, sub_->d_activeNodes,
+                                      sub_->d_activeNodesPointer, gen_->scanScratch(), sc_);
+                errorCopies();
+                part_->enqueue(*sub_, gen_->deviceTotals(), maxPart_);
+            } catch (...) {
+                (void)hipStreamEndCapture(sc_, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            SUBWAY_CHECK(hipStreamEndCapture(sc_, &gr));
+            const hipError_t e = hipGraphInstantiate(&iterExec_, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            SUBWAY_CHECK(e);
+        }
+        SUBWAY_CHECK(hipGraphLaunch(iterExec_, sc_));
+        part_->collect(*sub_, gen_->deviceTotals(), maxPart_);
+        checkWaits();
+        sub_->numActiveNodes = (uint)part_->numActiveNodes;
+        sub_->numActiveEdges = part_->numActiveEdges;
+        return;
+    }
     if (o_.profile) pGen_.begin(sc_);
     if (batchK_)
         launch_generate_masks(g_->num_nodes, d_m1_, d_m2_, g_->d_outDegree, sub_->d_activeNodes,
@@ -474,10 +517,7 @@ void Engine<E, G>::generateAndPartition()
     }
     // error words of every bounded wait so far; the partitioner synchronises
     // the stream before they are read
-    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[0], &fr_.ctl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
-    SUBWAY_CHECK(hipMemcpyAsync(&h_err_[1], &gen_->scanScratch().ctl->error, sizeof(uint32_t),
-                                hipMemcpyDeviceToHost, sc_));
-    if (pc_) SUBWAY_CHECK(hipMemcpyAsync(&h_err_[2], &pc_->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+    errorCopies();
     part_->partition(*sub_, gen_->deviceTotals(), maxPart_);
     checkWaits();
     sub_->numActiveNodes = (uint)part_->numActiveNodes;
@@ -962,7 +1002,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         return;
     }
     if (roundsGrid_ > 0) {
-        // every round of the partition in one cooperative launch; a partition
+        // every round of the partition in one persistent launch; a partition
         // that has not converged after maxRounds keeps its labels and is
         // picked up by the next generate, so no host check is needed
         RoundsLaunch l{};
@@ -1052,7 +1092,7 @@ void Engine<E, G>::compute(size_t idx)
     }
 }
 
-// Batched partition: all rounds for every source in one cooperative launch;
+// Batched partition: all rounds for every source in one persistent launch;
 // an oversized vertex gets one masked frontier and windowed advances.
 template <class E, class G>
 void Engine<E, G>::computeBatch(const Unit &u, int b)

@@ -1062,7 +1062,7 @@ struct RoundsArgs {
     u32 packCode;         // EDGE_PACKED: destination bits
 };
 
-// All async inner rounds of one resident partition in one cooperative
+// All async inner rounds of one resident partition in one persistent
 // launch: phase A selects the labelled vertices (scan tiles claimed by
 // counter), grid barrier, phase B relaxes their edges (work chunks claimed by
 // counter), grid barrier; the loop ends, uniformly for every workgroup, on
@@ -1280,7 +1280,7 @@ struct BatchArgs {
 };
 
 // All async rounds of one resident partition for up to 64 sources at once,
-// in one cooperative launch (same phase / barrier structure as rounds_kernel).
+// in one persistent launch (same phase / barrier structure as rounds_kernel).
 template <int ALGO, class E, int U>
 __device__ __forceinline__ void rounds_batch_body(const BatchArgs &a, u32 me, u32 members)
 {
@@ -1949,11 +1949,9 @@ void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st)
     SUBWAY_CHECK(hipGetLastError());
 }
 
-// Grid of a cooperative kernel: every workgroup must be resident at once,
-// and workgroups are placed round-robin over the XCDs, so a kernel of the
-// copy stream that is resident when the grid starts (a gather) can hold the
-// slots a full-occupancy grid needs on its XCD for as long as the grid waits
-// at its first barrier.  One workgroup slot per CU is left free.
+// Grid of a persistent kernel: one workgroup slot per CU below the
+// co-resident maximum, the slot a kernel of the copy stream (the gather, one
+// workgroup per CU) uses, so that normally every workgroup joins.
 static int coresident_grid(int perCU, int cus)
 {
     return (perCU > 1 ? perCU - 1 : perCU) * cus;
@@ -1988,7 +1986,6 @@ int rounds_max_grid(int algo, int edgeFmt)
     SUBWAY_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
-    if (!prop.cooperativeLaunch) return 0;
     return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
@@ -2021,8 +2018,11 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.packCode = l.packCode;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
-    SUBWAY_CHECK(hipLaunchCooperativeKernel(rounds_fn_for(algo, edgeFmt), dim3((unsigned)grid), dim3(ADV_BLOCK),
-                                            params, 0, st));
+    // an ordinary launch: the membership protocol (join_launch) makes the
+    // barriers correct for whatever part of the grid is resident, and a
+    // cooperative launch is refused while another process holds the
+    // device's cooperative queue
+    SUBWAY_CHECK(hipLaunchKernel(rounds_fn_for(algo, edgeFmt), dim3((unsigned)grid), dim3(ADV_BLOCK), params, 0, st));
 }
 
 void launch_generate_masks(ull n, const ull *m1, const ull *m2, const uint32_t *outDegree, uint32_t *activeNodes,
@@ -2136,7 +2136,6 @@ int batch_max_grid(int algo, int edgeFmt, int unroll)
     SUBWAY_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     SUBWAY_CHECK(hipGetDeviceProperties(&prop, dev));
-    if (!prop.cooperativeLaunch) return 0;
     return coresident_grid(perCU, prop.multiProcessorCount);
 }
 
@@ -2166,8 +2165,8 @@ void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l
     a.packCode = l.packCode;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
-    SUBWAY_CHECK(hipLaunchCooperativeKernel(batch_fn_for(algo, edgeFmt, unroll), dim3((unsigned)grid),
-                                            dim3(ADV_BLOCK), params, 0, st));
+    SUBWAY_CHECK(hipLaunchKernel(batch_fn_for(algo, edgeFmt, unroll), dim3((unsigned)grid), dim3(ADV_BLOCK), params, 0,
+                                 st));
 }
 
 void launch_policy(const PolicyArgs &a, hipStream_t st)

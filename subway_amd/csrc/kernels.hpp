@@ -192,7 +192,7 @@ struct AdvanceArgs {
 };
 void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st);
 
-// ---- persistent async rounds (one cooperative launch per partition)
+// ---- persistent async rounds (one launch per partition)
 struct RoundsLaunch {
     ull n;
     const uint32_t *nodes;
@@ -215,7 +215,7 @@ struct RoundsLaunch {
     uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
-// below full occupancy (0: no cooperative launch).
+// below full occupancy.
 int rounds_max_grid(int algo, int edgeFmt);
 void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipStream_t st);
 

@@ -56,6 +56,51 @@ void Partitioner<E>::partition(Subgraph<E> &subgraph, const unsigned long long *
 }
 
 template <class E>
+void Partitioner<E>::enqueueRows(Subgraph<E> &subgraph, const unsigned long long *d_numActive, ull hostCount,
+                                 ull budgetEdges, ull start)
+{
+    launch_partition(subgraph.d_activeNodesPointer, subgraph.d_activeNodes, (const ull *)d_numActive, hostCount,
+                     budgetEdges, start, MAX_ROWS, (PartRow *)d_rows, (ull *)d_out, stream);
+    // the first rows travel with the count; a long table takes a second copy
+    SUBWAY_CHECK(hipMemcpyAsync(h_out, d_out, 4 * sizeof(ull), hipMemcpyDeviceToHost, stream));
+    SUBWAY_CHECK(hipMemcpyAsync(h_rows, d_rows, FIRST_ROWS * sizeof(PartRow), hipMemcpyDeviceToHost, stream));
+}
+
+// Reads the rows of the last enqueueRows; returns true when the table
+// continues past MAX_ROWS (h_out[1] is then the next start).
+template <class E>
+bool Partitioner<E>::collectRows()
+{
+    SUBWAY_CHECK(hipStreamSynchronize(stream));
+    const PartRow *rows = (const PartRow *)h_rows;
+    ull got = h_out[0];
+    numActiveNodes = h_out[2];
+    numActiveEdges = h_out[3];
+    if (got > FIRST_ROWS) {
+        SUBWAY_CHECK(hipMemcpyAsync((PartRow *)h_rows + FIRST_ROWS, (PartRow *)d_rows + FIRST_ROWS,
+                                    (got - FIRST_ROWS) * sizeof(PartRow), hipMemcpyDeviceToHost, stream));
+        SUBWAY_CHECK(hipStreamSynchronize(stream));
+    }
+    for (ull r = 0; r < got; r++) {
+        fromNode.push_back(rows[r].fromNode);
+        partitionNodeSize.push_back(rows[r].nodeCount);
+        fromEdge.push_back(rows[r].fromEdge);
+        partitionEdgeSize.push_back(rows[r].edgeCount);
+        firstVertex.push_back((uint)rows[r].firstId);
+        lastVertex.push_back((uint)rows[r].lastId);
+    }
+    return got >= MAX_ROWS;
+}
+
+template <class E>
+void Partitioner<E>::finish(Subgraph<E> &subgraph, const unsigned long long *d_numActive, ull hostCount,
+                            ull budgetEdges)
+{
+    while (collectRows()) enqueueRows(subgraph, d_numActive, hostCount, budgetEdges, h_out[1]);
+    numPartitions = (uint)fromNode.size();
+}
+
+template <class E>
 void Partitioner<E>::run(Subgraph<E> &subgraph, const unsigned long long *d_numActive, unsigned long long hostCount,
                          unsigned long long budgetEdges)
 {
@@ -63,36 +108,25 @@ void Partitioner<E>::run(Subgraph<E> &subgraph, const unsigned long long *d_numA
     numActiveNodes = hostCount;
     numActiveEdges = 0;
     if (!d_numActive && hostCount == 0) return;
-    ull start = 0;
-    for (;;) {
-        launch_partition(subgraph.d_activeNodesPointer, subgraph.d_activeNodes,
-                         (const ull *)d_numActive, hostCount, budgetEdges, start, MAX_ROWS,
-                         (PartRow *)d_rows, (ull *)d_out, stream);
-        // the first rows travel with the count; a long table takes a second copy
-        SUBWAY_CHECK(hipMemcpyAsync(h_out, d_out, 4 * sizeof(ull), hipMemcpyDeviceToHost, stream));
-        SUBWAY_CHECK(hipMemcpyAsync(h_rows, d_rows, FIRST_ROWS * sizeof(PartRow), hipMemcpyDeviceToHost, stream));
-        SUBWAY_CHECK(hipStreamSynchronize(stream));
-        const PartRow *rows = (const PartRow *)h_rows;
-        ull got = h_out[0];
-        numActiveNodes = h_out[2];
-        numActiveEdges = h_out[3];
-        if (got > FIRST_ROWS) {
-            SUBWAY_CHECK(hipMemcpyAsync((PartRow *)h_rows + FIRST_ROWS, (PartRow *)d_rows + FIRST_ROWS,
-                                        (got - FIRST_ROWS) * sizeof(PartRow), hipMemcpyDeviceToHost, stream));
-            SUBWAY_CHECK(hipStreamSynchronize(stream));
-        }
-        for (ull r = 0; r < got; r++) {
-            fromNode.push_back(rows[r].fromNode);
-            partitionNodeSize.push_back(rows[r].nodeCount);
-            fromEdge.push_back(rows[r].fromEdge);
-            partitionEdgeSize.push_back(rows[r].edgeCount);
-            firstVertex.push_back((uint)rows[r].firstId);
-            lastVertex.push_back((uint)rows[r].lastId);
-        }
-        if (got < MAX_ROWS) break;
-        start = h_out[1];
-    }
-    numPartitions = (uint)fromNode.size();
+    enqueueRows(subgraph, d_numActive, hostCount, budgetEdges, 0);
+    finish(subgraph, d_numActive, hostCount, budgetEdges);
+}
+
+template <class E>
+void Partitioner<E>::enqueue(Subgraph<E> &subgraph, const unsigned long long *d_numActive,
+                             unsigned long long budgetEdges)
+{
+    enqueueRows(subgraph, d_numActive, 0, budgetEdges, 0);
+}
+
+template <class E>
+void Partitioner<E>::collect(Subgraph<E> &subgraph, const unsigned long long *d_numActive,
+                             unsigned long long budgetEdges)
+{
+    reset();
+    numActiveNodes = 0;
+    numActiveEdges = 0;
+    finish(subgraph, d_numActive, 0, budgetEdges);
 }
 
 // shared/partitioner.cu:103-104

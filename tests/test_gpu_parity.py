@@ -204,7 +204,7 @@ ASYNC = [(a, "async") for a in ("bfs", "cc", "sssp", "sswp", "pr")]
 @pytest.mark.parametrize("rounds", ["persistent", "per-round"])
 @pytest.mark.parametrize("budget", [0, 48 << 10])
 def test_async_rounds_modes(sw, gpu, algo, mode, rounds, budget):
-    """The cooperative persistent rounds kernel and the launch-per-round
+    """The persistent rounds kernel and the launch-per-round
     path reach the same fixpoint as the oracle."""
     weighted = algo in ("sssp", "sswp")
     g = sw.Graph.rmat(13, 16, seed=21, weighted=weighted, symmetrize=(algo == "cc"))
@@ -385,7 +385,7 @@ def test_unpackable_weights_fall_back(sw, gpu):
 
 def test_bounded_wait_reports_error(sw, gpu):
     """Every device-side wait (scan lookback, grid barrier) gives up after a
-    poll limit: with a limit of 1 poll the grid barrier of the cooperative
+    poll limit: with a limit of 1 poll the grid barrier of the persistent
     rounds kernel (over a thousand workgroups) gives up, every kernel still
     runs to its end, the run fails with an error naming the wait, and a new
     run with the default limit is exact."""
@@ -454,6 +454,26 @@ def test_batch_matches_single_source(sw, gpu, algo, budget, k):
     for i, s in enumerate(sources):
         exp = O.expected(algo, ptr, adj, w, int(s))
         assert np.array_equal(vals[i], exp), (algo, budget, k, i, int(np.sum(vals[i] != exp)))
+
+
+def test_batch_graph_replays_match_eager(sw, gpu):
+    """Batched iterations start with a recorded HIP graph (union generate,
+    error words, partition table), replayed every iteration of every run;
+    with profiling the same launches run eagerly.  Rows are identical, run
+    after run, with eager runs of another session in between."""
+    g = sw.Graph.rmat(13, 16, seed=29, weighted=True)
+    sources = np.arange(0, 64 * 61, 61, dtype=np.uint32) % g.n
+    graphed = sw.BatchSession(g, "sssp", budget=48 << 10)
+    eager = sw.BatchSession(g, "sssp", budget=48 << 10, profile=True)
+    ref, st0 = eager.run(sources)
+    assert st0.global_iterations >= 3
+    for i in range(4):
+        rows, st = graphed.run(sources)
+        assert np.array_equal(rows, ref), i
+        again, _ = eager.run(sources)
+        assert np.array_equal(again, ref), i
+    graphed.close()
+    eager.close()
 
 
 def test_batch_tuner_and_repeats(sw, gpu):
