@@ -53,6 +53,7 @@ struct RunOptions {
     uint32_t spinLimit = 0; // polls before a device wait gives up (the run then fails); 0 = default
     uint32_t joinWindow = 0;   // persistent launches: membership window in 100 MHz ticks; 0 = 50000 (500 us)
     int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
+    int noResident = 0;     // 1: move active edges every iteration even when the whole graph fits one buffer
 };
 
 struct IterRecord {

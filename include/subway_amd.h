@@ -73,7 +73,10 @@ typedef struct subway_options {
                                bits; 1 keeps 8-byte {end, w8} edges */
     uint32_t join_window;   /* persistent launches: how long (ticks of the 100 MHz constant clock) the first
                                workgroup waits for the rest of the grid before closing membership; 0 = 50000 */
-    int32_t reserved[4];
+    int32_t no_resident;    /* 1: move the active edges every global iteration even when the whole graph fits
+                               one device buffer (the reference's behaviour); 0 (default): such a graph
+                               moves once per run and stays resident */
+    int32_t reserved[3];
 } subway_options;
 
 typedef struct subway_stats {

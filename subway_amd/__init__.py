@@ -49,7 +49,7 @@ class _Options(ctypes.Structure):
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
         ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("no_resident", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -219,7 +219,7 @@ def _set_policy(h, params):
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
              profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
-             join_window=0):
+             join_window=0, resident=True):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -241,6 +241,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.spin_limit = spin_limit
     o.unpacked_edges = 0 if packed else 1
     o.join_window = join_window
+    o.no_resident = 0 if resident else 1
     return o
 
 
@@ -265,10 +266,10 @@ class Session:
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
-                 packed: bool = True, params=None, join_window: int = 0):
+                 packed: bool = True, params=None, join_window: int = 0, resident: bool = True):
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window)
+                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -359,7 +360,7 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
         rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True,
-        params=None, join_window: int = 0):
+        params=None, join_window: int = 0, resident: bool = True):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
@@ -367,18 +368,21 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     device memory, the reference's sizing).  ``params``: scheduler network
     for policy="learned".  ``join_window``: how long (100 MHz ticks) a
     persistent launch waits for its workgroups before closing membership.
+    ``resident``: a graph whose edges fit one device buffer moves once per
+    run and stays there (False: the active edges move every iteration).
     """
     if params is not None:
         s = Session(graph, algo, mode=mode, budget=budget, buffers=buffers, threshold=threshold, init_pr=init_pr,
                     transfer=transfer, policy=policy, verbose=verbose, advance_grid=advance_grid, profile=profile,
                     rounds=rounds, rounds_grid=rounds_grid, max_rounds=max_rounds, theta=theta,
-                    spin_limit=spin_limit, packed=packed, params=params, join_window=join_window)
+                    spin_limit=spin_limit, packed=packed, params=params, join_window=join_window,
+                    resident=resident)
         try:
             return s.run(source)
         finally:
             s.close()
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window)
+                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

@@ -197,6 +197,8 @@ public:
                 }
             }
         }
+        // a graph that fits one buffer moves once per run and stays there
+        resident_ = !o_.noResident && !o_.batch && g_->num_edges > 0 && g_->num_edges <= maxPart_;
         if (!g_->csr->pinned && fmt_ != EDGE_SPLIT)
             throw Error(ERR_ARG, "this graph's edge list is not pinned: it moves only in the split format "
                                  "(unpacked_edges must be 0)");
@@ -365,6 +367,8 @@ private:
     void generateAndPartition();
     void errorCopies();
     hipGraphExec_t iterExec_ = nullptr;   // batched runs: the recorded start of an iteration
+    bool resident_ = false;               // the whole graph fits buffer 0
+    bool residentLoaded_ = false;         // ... and has been moved there in this run
     void buildUnits();
     void issue(size_t u);
     void compute(size_t u);
@@ -782,6 +786,25 @@ void Engine<E, G>::buildUnitsLearned()
 template <class E, class G>
 void Engine<E, G>::buildUnits()
 {
+    if (resident_) {
+        // the whole graph as one unit: moved by the first iteration's DMA,
+        // then read in place; its rounds cover every id
+        units_.clear();
+        if (part_->numPartitions == 0) return;
+        Unit u{};
+        u.part = 0;
+        u.dma = true;
+        u.hostEdge = 0;
+        u.count = g_->num_edges;
+        u.whole = true;
+        u.n = g_->num_nodes;
+        u.nodes = nullptr;
+        u.ptr = g_->d_nodePointer;
+        u.fromEdge = 0;
+        u.idBase = 0;
+        units_.push_back(u);
+        return;
+    }
     if (d_bounds_ && !onPlain_) {
         buildUnitsLearned();
         return;
@@ -838,6 +861,11 @@ void Engine<E, G>::issue(size_t idx)
     const Unit &u = units_[idx];
     const int b = (int)(idx % (size_t)sub_->numBuffers);
     SUBWAY_CHECK(hipStreamWaitEvent(sx_, evFree_[b], 0));
+    if (resident_ && residentLoaded_) {
+        SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));   // already in buffer 0
+        return;
+    }
+    if (resident_) residentLoaded_ = true;
     if (u.count) {
         if (o_.profile) pXfer_.begin(sx_);
         char *dst = (char *)sub_->d_buffers[b];
@@ -1067,6 +1095,10 @@ void Engine<E, G>::compute(size_t idx)
     const bool async = o_.mode == ASYNC;
     if (batchK_) {
         computeBatch(u, b);
+    } else if (resident_) {
+        // rounds to the fixpoint (or the round cap) on the resident graph, in
+        // both modes: the label fixpoints are unique, PR stays within its bound
+        asyncRounds(u, b);
     } else if (u.hub) {
         // one round over the oversized vertex, streamed chunk by chunk: the
         // frontier kernel runs once (clearing its label, fixing its value),
@@ -1142,6 +1174,7 @@ void Engine<E, G>::resetRun()
 {
     if (broken_) throw Error(ERR_INTERNAL, "a device wait of an earlier run gave up; the session must be rebuilt");
     st_ = RunStats();
+    residentLoaded_ = false;
     roundsCap_ = o_.maxRounds;
     theta_ = o_.theta;
     onPlain_ = false;

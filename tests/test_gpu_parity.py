@@ -200,6 +200,31 @@ def test_apps_random(sw, gpu, algo, mode, budget, transfer):
 ASYNC = [(a, "async") for a in ("bfs", "cc", "sssp", "sswp", "pr")]
 
 
+@pytest.mark.parametrize("algo,mode", APPS + [("pr", "sync"), ("pr", "async")])
+def test_resident_graph(sw, gpu, algo, mode):
+    """A graph whose edges fit one device buffer moves once per run and
+    stays resident; every partition visit then runs rounds to the fixpoint
+    on it.  Values equal the oracle's and the per-iteration transfers'."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(14, 16, seed=41, weighted=weighted, symmetrize=(algo == "cc"))
+    src = g.max_degree_vertex if algo != "cc" else 0
+    kw = {"threshold": 1e-6} if algo == "pr" else {}
+    s = sw.Session(g, algo, mode=mode, budget=8 << 20, **kw)
+    for _ in range(2):                       # the buffer is loaded again in every run
+        v, st = s.run(src)
+        assert st.h2d_bytes == st.edge_bytes * g.m
+        ptr, adj, w = g.csr()
+        if algo == "pr":
+            exp, _ = O.pr(ptr, adj, 1e-6)
+            assert (np.abs(v.astype(np.float64) - exp) / exp).max() < PR_RTOL
+        else:
+            assert np.array_equal(v, O.expected(algo, ptr, adj, w, src))
+    s.close()
+    vn, sn = check_app(sw, g, algo, mode, src=src, budget=8 << 20, resident=False, **kw)
+    if algo != "pr":
+        assert np.array_equal(v, vn)
+
+
 @pytest.mark.parametrize("algo,mode", ASYNC)
 @pytest.mark.parametrize("rounds", ["persistent", "per-round"])
 @pytest.mark.parametrize("budget", [0, 48 << 10])
@@ -252,13 +277,18 @@ def test_apps_rmat_tiny_budget(sw, gpu, algo, mode, buffers):
 
 def test_config1_bfs_sync_scale16(sw, gpu):
     """R-MAT scale 16, edge factor 16, source 0, no budget cap: bit-exact
-    against the oracle and the OpenMP build."""
+    against the oracle and the OpenMP build.  The graph fits the device, so
+    it stays resident and one visit runs the rounds to the fixpoint; with
+    per-iteration transfers (the reference's loop) the same levels take
+    several global iterations."""
     g = sw.Graph.rmat(16, 16, seed=42)
     vals, st = check_app(sw, g, "bfs", "sync", src=0)
     ptr, adj, _ = g.csr()
     omp, _, _ = O.omp_label("bfs", ptr, adj, None, 0)
     assert (vals == omp).all()
-    assert st.global_iterations >= 2
+    assert st.global_iterations == 1 and st.inner_rounds >= 2
+    vn, sn = check_app(sw, g, "bfs", "sync", src=0, resident=False)
+    assert np.array_equal(vn, vals) and sn.global_iterations >= 2
 
 
 def test_deterministic_repeats(sw, gpu):
