@@ -369,6 +369,10 @@ private:
     hipGraphExec_t iterExec_ = nullptr;   // batched runs: the recorded start of an iteration
     bool resident_ = false;               // the whole graph fits buffer 0
     bool residentLoaded_ = false;         // ... and has been moved there in this run
+    // host edge slice [hostEdge, hostEdge + count) each buffer holds from a
+    // DMA of this run (count 0: unknown); host edges never change, so a unit
+    // asking for the same slice in the same buffer needs no copy
+    ull bufHostEdge_[2] = {0, 0}, bufCount_[2] = {0, 0};
     void buildUnits();
     void issue(size_t u);
     void compute(size_t u);
@@ -866,6 +870,12 @@ void Engine<E, G>::issue(size_t idx)
         return;
     }
     if (resident_) residentLoaded_ = true;
+    if (u.dma && u.count && bufCount_[b] == u.count && bufHostEdge_[b] == u.hostEdge) {
+        SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));   // the slice is still in the buffer
+        return;
+    }
+    bufCount_[b] = u.dma ? u.count : 0;
+    bufHostEdge_[b] = u.hostEdge;
     if (u.count) {
         if (o_.profile) pXfer_.begin(sx_);
         char *dst = (char *)sub_->d_buffers[b];
@@ -1175,6 +1185,7 @@ void Engine<E, G>::resetRun()
     if (broken_) throw Error(ERR_INTERNAL, "a device wait of an earlier run gave up; the session must be rebuilt");
     st_ = RunStats();
     residentLoaded_ = false;
+    bufCount_[0] = bufCount_[1] = 0;   // every run moves its edges at least once
     roundsCap_ = o_.maxRounds;
     theta_ = o_.theta;
     onPlain_ = false;
