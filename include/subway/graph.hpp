@@ -51,6 +51,8 @@ struct HostCsr {
     uint32_t *splitDst = nullptr, *d_splitDst = nullptr;
     uint8_t *splitW = nullptr, *d_splitW = nullptr;
     bool ensureSplit();             // builds the split copy once; false when it does not apply
+    // After an unpinned read: split arrays, or the list moved into pinned memory
+    void settleEdges(bool forceSplit);
     HostCsr() = default;
     HostCsr(const HostCsr &) = delete;
 private:
@@ -67,8 +69,11 @@ public:
     void finalize();               // degrees, maxDegree, device mapping
 };
 
+// splitLayout: a .bwcsr graph that moves only in the split format (weights
+// must fit a byte); weighted lists of 16 GiB or more choose it by themselves
+// when they do not pack into one word per edge.
 template <class E>
-std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path);
+std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path, bool splitLayout = false);
 template <class E>
 void write_csr_file(const HostCsr<E> &csr, const std::string &path);
 
@@ -110,6 +115,7 @@ public:
     void ReadGraph();
 
 private:
+    bool valuePinned = false;   // value came from hipHostMalloc
     void attach();
 };
 
@@ -144,6 +150,7 @@ public:
     void ReadGraph();
 
 private:
+    bool valuePinned = false;   // value came from hipHostMalloc
     void attach();
 };
 

@@ -134,7 +134,12 @@ int32_t subway_device_count(void);
 
 /* -------- graphs (host CSR in pinned, device-mapped memory) */
 /* Graph<E>(path, isWeighted).ReadGraph(): shared/graph.cu:32-197.
- * .bcsr/.bwcsr (u32 header), .bcsr64/.bwcsr64 (u64 header), .el/.wel text. */
+ * .bcsr/.bwcsr (u32 header), .bcsr64/.bwcsr64 (u64 header), .el/.wel text.
+ * weighted: 0 no, 1 yes, 2 yes and laid out to move only in the split format
+ * (as subway_graph_rmat's 2; .bwcsr files whose weights fit a byte).  A
+ * weighted list of 16 GiB or more is read into ordinary memory and then
+ * pinned as split arrays when it does not pack into one word per edge and
+ * its weights fit a byte, else moved into pinned memory. */
 int32_t subway_load(const char *path, int32_t weighted, subway_graph **out);
 /* Build from caller arrays: node_pointer[n+1], ends[m], weights[m] or NULL. */
 int32_t subway_graph_from_csr(uint64_t n, uint64_t m, const uint64_t *node_pointer, const uint32_t *ends,

@@ -111,12 +111,16 @@ class Graph:
 
     # --- constructors
     @classmethod
-    def load(cls, path: str, weighted: bool | None = None) -> "Graph":
-        """Read .bcsr/.bwcsr (and 64-bit variants) or .el/.wel text."""
+    def load(cls, path: str, weighted: bool | None = None, split_layout: bool = False) -> "Graph":
+        """Read .bcsr/.bwcsr (and 64-bit variants) or .el/.wel text.
+        split_layout: a .bwcsr graph that moves only in the split format (as
+        weighted lists of 16 GiB or more that do not pack are read by
+        themselves); weights must fit a byte."""
         if weighted is None:
             weighted = os.path.splitext(path)[1] in (".bwcsr", ".bwcsr64", ".wel")
         h = ctypes.c_void_p()
-        check(lib.subway_load(path.encode(), int(bool(weighted)), ctypes.byref(h)))
+        w = (2 if split_layout else 1) if weighted else 0
+        check(lib.subway_load(path.encode(), w, ctypes.byref(h)))
         return cls(h)
 
     @classmethod

@@ -94,7 +94,7 @@ int32_t subway_load(const char *path, int32_t weighted, subway_graph **out)
         if (!path || !out) throw Error(ERR_ARG, "null argument");
         std::unique_ptr<subway_graph> g(new subway_graph);
         g->weighted = weighted != 0;
-        if (g->weighted) g->w = read_csr_file<OutEdgeWeighted>(path);
+        if (g->weighted) g->w = read_csr_file<OutEdgeWeighted>(path, weighted == 2);
         else g->u = read_csr_file<OutEdge>(path);
         *out = g.release();
     });

@@ -222,6 +222,64 @@ void HostCsr<E>::finalize()
     if (pinned) SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_edges, edges, 0));
 }
 
+// A weighted edge list read into ordinary memory (it may exceed what a
+// process can pin): when its weights fit a byte and ids + weight span do not
+// pack into one word (or the caller asks for it), the pinned memory becomes
+// the split arrays (5 bytes per edge) and the 8-byte list stays in ordinary
+// memory; otherwise the list moves into pinned memory as every other graph's.
+template <class E>
+void HostCsr<E>::settleEdges(bool forceSplit)
+{
+    if (pinned) return;
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) {
+        (void)hipGetLastError();
+        return;   // no GPU in this process: nothing is mapped (conversion, inspection)
+    }
+    bool split = false;
+    if constexpr (std::is_same<E, OutEdgeWeighted>::value) {
+        const int T = host_threads();
+        std::vector<uint32_t> mx((size_t)T, 0), mn((size_t)T, 0xFFFFFFFFu);
+        parallel_for(m, [&](uint64_t b, uint64_t e, int t) {
+            uint32_t x = 0, y = 0xFFFFFFFFu;
+            for (uint64_t i = b; i < e; i++) {
+                x = std::max(x, edges[i].w8);
+                y = std::min(y, edges[i].w8);
+            }
+            mx[(size_t)t] = std::max(mx[(size_t)t], x);
+            mn[(size_t)t] = std::min(mn[(size_t)t], y);
+        }, T);
+        const uint32_t maxW = m ? *std::max_element(mx.begin(), mx.end()) : 0;
+        uint32_t base = m ? *std::min_element(mn.begin(), mn.end()) : 0;
+        if (base >> 27) base = 0;
+        uint32_t db = 1;
+        while (db < 32 && (1ull << db) < n) db++;
+        const bool packs = db < 32 && ((uint64_t)(maxW - base) >> (32 - db)) == 0;
+        if (forceSplit && maxW > 0xFFu) throw Error(ERR_ARG, "the split layout needs weights that fit a byte");
+        split = forceSplit || (!packs && maxW <= 0xFFu);
+        if (split) {
+            if (!allocateSplit()) throw Error(ERR_NOMEM, "pinned allocation of the split edge arrays failed");
+            parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+                for (uint64_t i = b; i < e; i++) {
+                    splitDst[i] = edges[i].end;
+                    splitW[i] = (uint8_t)edges[i].w8;
+                }
+            }, T);
+            return;
+        }
+    }
+    const size_t bytes = (size_t)(m ? m : 1) * sizeof(E);
+    E *p = nullptr;
+    const hipError_t e = hipHostMalloc((void **)&p, bytes, hipHostMallocMapped | hipHostMallocPortable);
+    if (e != hipSuccess)
+        throw Error(ERR_NOMEM, "pinned allocation of " + std::to_string(bytes) +
+                                   " bytes for the edge list failed: " + hipGetErrorString(e));
+    parallel_for(m, [&](uint64_t b, uint64_t e2, int) { std::memcpy(p + b, edges + b, (e2 - b) * sizeof(E)); });
+    std::free(edges);
+    edges = p;
+    pinned = true;
+}
+
 // ------------------------------------------------------------ binary I/O
 static void read_exact(FILE *f, void *dst, size_t bytes, const std::string &path)
 {
@@ -255,7 +313,7 @@ static ull file_size(FILE *f)
 }
 
 template <class E>
-static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide)
+static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide, bool splitLayout)
 {
     FILE *f = fopen(path.c_str(), "rb");
     if (!f) throw Error(ERR_IO, "cannot open graph file: " + path + " (" + strerror(errno) + ")");
@@ -281,7 +339,10 @@ static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide)
         throw Error(ERR_FORMAT, "graph file size " + std::to_string(size) + " does not match header (n=" +
                                     std::to_string(n) + ", m=" + std::to_string(m) + ", expected " +
                                     std::to_string(expect) + " bytes): " + path);
-    csr.allocate(n, m);
+    // weighted lists of 16 GiB or more are read into ordinary memory and
+    // settled afterwards, when their weights are known
+    const bool settle = is_weighted_edge<E>() && (splitLayout || m * sizeof(E) >= (1ull << 34));
+    csr.allocate(n, m, !settle);
     csr.large = wide;
     if (wide) {
         read_exact(f, csr.ptr, n * 8, path);
@@ -293,9 +354,10 @@ static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide)
     read_exact(f, csr.edges, m * sizeof(E), path);
     for (ull i = 0; i < n; i++)
         if (csr.ptr[i] > m) throw Error(ERR_FORMAT, "node pointer beyond edge count: " + path);
-    csr.finalize();
     for (ull e = 0; e < m; e++)
         if (csr.edges[e].end >= n) throw Error(ERR_FORMAT, "edge endpoint out of range: " + path);
+    if (settle) csr.settleEdges(splitLayout);
+    csr.finalize();
 }
 
 // ------------------------------------------------------------- text I/O
@@ -360,18 +422,19 @@ static void read_text(HostCsr<E> &csr, const std::string &path)
 }
 
 template <class E>
-std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path)
+std::shared_ptr<HostCsr<E>> read_csr_file(const std::string &path, bool splitLayout)
 {
     auto csr = std::make_shared<HostCsr<E>>();
     std::string ext = file_extension(path);
     bool weighted = is_weighted_edge<E>();
     if (ext == "bcsr" || ext == "bcsr64") {
         if (weighted) throw Error(ERR_FORMAT, "a weighted graph needs a .bwcsr file, got " + path);
-        read_binary(*csr, path, ext == "bcsr64");
+        read_binary(*csr, path, ext == "bcsr64", false);
     } else if (ext == "bwcsr" || ext == "bwcsr64") {
         if (!weighted) throw Error(ERR_FORMAT, "an unweighted graph needs a .bcsr file, got " + path);
-        read_binary(*csr, path, ext == "bwcsr64");
+        read_binary(*csr, path, ext == "bwcsr64", splitLayout);
     } else if (ext == "el" || ext == "wel") {
+        if (splitLayout) throw Error(ERR_ARG, "the split layout is read from .bwcsr files only");
         read_text(*csr, path);
     } else {
         throw Error(ERR_FORMAT, "The graph format is not supported: " + path);
@@ -411,8 +474,8 @@ void write_csr_file(const HostCsr<E> &csr, const std::string &path)
 
 template struct HostCsr<OutEdge>;
 template struct HostCsr<OutEdgeWeighted>;
-template std::shared_ptr<HostCsr<OutEdge>> read_csr_file<OutEdge>(const std::string &);
-template std::shared_ptr<HostCsr<OutEdgeWeighted>> read_csr_file<OutEdgeWeighted>(const std::string &);
+template std::shared_ptr<HostCsr<OutEdge>> read_csr_file<OutEdge>(const std::string &, bool);
+template std::shared_ptr<HostCsr<OutEdgeWeighted>> read_csr_file<OutEdgeWeighted>(const std::string &, bool);
 template void write_csr_file<OutEdge>(const HostCsr<OutEdge> &, const std::string &);
 template void write_csr_file<OutEdgeWeighted>(const HostCsr<OutEdgeWeighted> &, const std::string &);
 
@@ -420,6 +483,31 @@ template void write_csr_file<OutEdgeWeighted>(const HostCsr<OutEdgeWeighted> &, 
 
 // =================================================================== Graph
 using subway::HostCsr;
+
+// The host result array receives the run's final device-to-host copy of V
+// values: pinned, that copy runs at the link's rate instead of through the
+// runtime's staging buffers.  Ordinary memory if the pin is refused.
+template <class T>
+static T *alloc_result(uint n, bool &pinned)
+{
+    T *p = nullptr;
+    if (hipHostMalloc((void **)&p, (size_t)(n ? n : 1) * sizeof(T), hipHostMallocPortable) == hipSuccess) {
+        pinned = true;
+        return p;
+    }
+    (void)hipGetLastError();
+    pinned = false;
+    return new T[n];
+}
+
+template <class T>
+static void free_result(T *p, bool pinned)
+{
+    if (pinned)
+        (void)hipHostFree(p);
+    else
+        delete[] p;
+}
 
 template <class E>
 Graph<E>::Graph(std::string path, bool weighted)
@@ -443,7 +531,7 @@ Graph<E>::~Graph()
 {
     delete[] label1;
     delete[] label2;
-    delete[] value;
+    free_result(value, valuePinned);
     if (d_outDegree) (void)hipFree(d_outDegree);
     if (d_value) (void)hipFree(d_value);
     if (d_label1) (void)hipFree(d_label1);
@@ -491,7 +579,7 @@ void Graph<E>::attach()
     d_edgeList = csr->d_edges;
     label1 = new bool[num_nodes];
     label2 = new bool[num_nodes];
-    value = new uint[num_nodes];
+    value = alloc_result<uint>(num_nodes, valuePinned);
     SUBWAY_CHECK(hipMalloc(&d_outDegree, (size_t)num_nodes * sizeof(uint)));
     SUBWAY_CHECK(hipMalloc(&d_value, (size_t)num_nodes * sizeof(uint)));
     SUBWAY_CHECK(hipMalloc(&d_label1, (size_t)num_nodes * sizeof(bool)));
@@ -522,7 +610,7 @@ GraphPR<E>::GraphPR(std::shared_ptr<HostCsr<E>> c) : GraphPR<E>(std::string(), s
 template <class E>
 GraphPR<E>::~GraphPR()
 {
-    delete[] value;
+    free_result(value, valuePinned);
     delete[] delta;
     if (d_outDegree) (void)hipFree(d_outDegree);
     if (d_value) (void)hipFree(d_value);
@@ -568,7 +656,7 @@ void GraphPR<E>::attach()
     edgeList = csr->edges;
     outDegree = csr->degree;
     d_edgeList = csr->d_edges;
-    value = new float[num_nodes];
+    value = alloc_result<float>(num_nodes, valuePinned);
     delta = new float[num_nodes];
     SUBWAY_CHECK(hipMalloc(&d_outDegree, (size_t)num_nodes * sizeof(uint)));
     SUBWAY_CHECK(hipMalloc(&d_value, (size_t)num_nodes * sizeof(float)));

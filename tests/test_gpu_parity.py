@@ -400,6 +400,30 @@ def test_split_layout_graph(sw, gpu, algo, mode):
         sw.run(g, algo, mode=mode, source=src, budget=96 << 10, packed=False)
 
 
+@pytest.mark.parametrize("ext", ["bwcsr", "bwcsr64"])
+def test_split_layout_file(sw, gpu, tmp_path, ext):
+    """A .bwcsr file read in the split layout (the layout a weighted file of
+    16 GiB or more that does not pack gets by itself) gives the oracle's
+    values with 5-byte transfers, the same as the file read as usual; weights
+    beyond a byte are refused in that layout."""
+    g = sw.Graph.rmat(15, 16, seed=5, weighted=True)
+    path = str(tmp_path / f"g.{ext}")
+    g.save(path)
+    gs = sw.Graph.load(path, split_layout=True)
+    src = g.max_degree_vertex
+    vs, st = check_app(sw, gs, "sssp", "async", src=src, budget=96 << 10)
+    assert st.edge_bytes == 5 and st.partitions > st.global_iterations
+    vu, _ = check_app(sw, sw.Graph.load(path), "sssp", "async", src=src, budget=96 << 10)
+    assert np.array_equal(vs, vu)
+    with pytest.raises(sw.SubwayError, match="split format"):
+        sw.run(gs, "sssp", mode="async", source=src, budget=96 << 10, packed=False)
+    heavy = sw.Graph.rmat(10, 8, seed=6, weighted=True, wmax=300)
+    hp = str(tmp_path / "heavy.bwcsr")
+    heavy.save(hp)
+    with pytest.raises(sw.SubwayError, match="fit a byte"):
+        sw.Graph.load(hp, split_layout=True)
+
+
 def test_unpackable_weights_fall_back(sw, gpu):
     """Weights that fit neither beside the ids nor in a byte keep the 8-byte
     format."""
