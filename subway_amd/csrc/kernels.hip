@@ -122,7 +122,7 @@ __device__ __forceinline__ bool scan_tile(const Op &op, u64 n, u32 tile, u32 epo
     }
     if (tile == lastTile && tid == 0) {
         op.finish((u64)s_xcnt + totc, s_xsum + tots, ctl);
-        ctl->item_sum += n;
+        atomicAdd((unsigned long long *)&ctl->item_sum, (unsigned long long)n);
     }
     __syncthreads();
     return true;
@@ -451,6 +451,19 @@ struct GenPrOp {
 };
 
 // ------------------------------------------------------- frontier ops
+// Statistics of a frontier scan, published by whichever workgroup finishes
+// it: inside a persistent launch that is a different workgroup, often on
+// another XCD, each round, and the XCDs' L2 caches are not coherent, so the
+// counters are device-scope atomics (a plain read-modify-write loses
+// increments).
+__device__ __forceinline__ void stat_counts(ScanCtl *ctl, u64 cnt, u64 sum)
+{
+    st_relaxed(&ctl->stop, cnt == 0 ? 1u : 0u);
+    if (cnt) atomicAdd((unsigned long long *)&ctl->rounds, 1ull);
+    atomicAdd((unsigned long long *)&ctl->entry_sum, (unsigned long long)cnt);
+    atomicAdd((unsigned long long *)&ctl->work_sum, (unsigned long long)sum);
+}
+
 struct FrItem {
     u32 loc;
     u32 val;
@@ -538,10 +551,7 @@ struct FrontierLabelOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        ctl->stop = (cnt == 0) ? 1u : 0u;
-        if (cnt) ctl->rounds += 1;
-        ctl->entry_sum += cnt;
-        ctl->work_sum += sum;
+        stat_counts(ctl, cnt, sum);
     }
 };
 
@@ -592,10 +602,7 @@ struct FrontierPrOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        ctl->stop = (cnt == 0) ? 1u : 0u;
-        if (cnt) ctl->rounds += 1;
-        ctl->entry_sum += cnt;
-        ctl->work_sum += sum;
+        stat_counts(ctl, cnt, sum);
     }
 };
 
@@ -690,10 +697,7 @@ struct FrontierMaskOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        ctl->stop = (cnt == 0) ? 1u : 0u;
-        if (cnt) ctl->rounds += 1;
-        ctl->entry_sum += cnt;
-        ctl->work_sum += sum;
+        stat_counts(ctl, cnt, sum);
     }
 };
 
