@@ -787,12 +787,15 @@ struct Codec<SplitEdge> {
 
 // Relaxes the edges of one work chunk [cbeg, cbeg + ADV_CHUNK) clipped to
 // [w0, wend): the chunk's entries are staged in LDS and every position finds
-// its entry by binary search.  Returns the successful relaxations of the
-// calling thread.
+// its entry by binary search.  Once the entries are staged, thread 0 claims
+// the caller's next chunk from `claimCtr` (the atomic's latency overlaps the
+// edge loads) and stores it to `s_next` (LDS) before the closing barrier;
+// ~0u when this chunk's entries were inconsistent.  Returns the successful
+// relaxations of the calling thread.
 template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, double *delta, u64 count, u64 W, u64 w0,
-                                             u64 wend, u64 shift, u32 psh, u32 *err)
+                                             u64 wend, u64 shift, u32 psh, u32 *err, u32 *claimCtr, u32 *s_next)
 {
     __shared__ u32 s_pre[ADV_CHUNK + 1];
     __shared__ u32 s_loc[ADV_CHUNK + 1];
@@ -810,9 +813,14 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     const int nk = bad ? 0 : (int)(k1 - k0 + 1);
     for (int t = tid; t < nk; t += ADV_BLOCK) read_entry<COH>(entries, k0 + t, s_loc[t], s_pre[t], s_val[t]);
     if (__syncthreads_or(bad)) {
-        if (tid == 0) (void)atomicCAS(err, 0u, WAIT_DATA);
+        if (tid == 0) {
+            (void)atomicCAS(err, 0u, WAIT_DATA);
+            *s_next = ~0u;
+        }
+        __syncthreads();
         return 0;
     }
+    const u32 next = (tid == 0) ? atomicAdd(claimCtr, 1u) : 0u;
     // every thread owns positions pbeg + tid + u * ADV_BLOCK, u < PPT; the
     // phases below keep PPT edge loads, then PPT value loads, then PPT
     // atomics in flight per thread
@@ -873,6 +881,7 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
             }
         (void)delta;
     }
+    if (tid == 0) *s_next = next;
     __syncthreads();
     return updates;
 }
@@ -895,15 +904,13 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
     const u64 nchunks = (wend + ADV_CHUNK - 1) / ADV_CHUNK - j0;
     u32 updates = 0;
 
-    for (;;) {
-        if (tid == 0) s_chunk = atomicAdd(&ctl->adv_counter, 1u);
-        __syncthreads();
-        const u64 c = s_chunk;
-        __syncthreads();
-        if (c >= nchunks) break;
+    // every thread reads the claimed chunk before the next barrier, after
+    // which thread 0 may store the following claim
+    if (tid == 0) s_chunk = atomicAdd(&ctl->adv_counter, 1u);
+    __syncthreads();
+    for (u64 c = s_chunk; c < nchunks; c = s_chunk)
         updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
-                                                 wend, shift, psh, &ctl->error);
-    }
+                                                 wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk);
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
     if (tid == 0) {
@@ -1127,15 +1134,12 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
         }
         // phase B: advance
         const u64 nchunks = (W + ADV_CHUNK - 1) / ADV_CHUNK;
-        for (;;) {
-            if (tid == 0) s_claim = atomicAdd(&a.pc->chunk_ctr[r & 1], 1u);
-            __syncthreads();
-            const u64 c = s_claim;
-            __syncthreads();
-            if (c >= nchunks) break;
+        if (tid == 0) s_claim = atomicAdd(&a.pc->chunk_ctr[r & 1], 1u);
+        __syncthreads();
+        for (u64 c = s_claim; c < nchunks; c = s_claim)
             updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
-                                              a.delta, count, W, 0, W, 0, a.packCode, wt.err);
-        }
+                                              a.delta, count, W, 0, W, 0, a.packCode, wt.err,
+                                              &a.pc->chunk_ctr[r & 1], &s_claim);
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
