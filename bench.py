@@ -197,6 +197,7 @@ def batch_bench(sw, np, g, args, budget, peak):
                     "inner_rounds": stats[-1].inner_rounds, "rounds_launches": stats[-1].rounds_launches,
                     "work_edges": stats[-1].work_edges, "work_entries": stats[-1].work_entries,
                     "items_scanned": stats[-1].work_items, "updates": stats[-1].updates,
+                    "lane_work": stats[-1].lane_work, "lane_entries": stats[-1].lane_entries,
                     "h2d_gb_per_run": round(h2d / 1e9, 4),
                     "h2d_gbps_achieved": round(h2d / (xfer_ms * 1e-3) / 1e9, 2) if xfer_ms > 0 else None,
                     "ms_breakdown": {"rounds": round(float(np.mean([s.ms_rounds for s in stats])), 3),
@@ -214,17 +215,20 @@ def batch_bench(sw, np, g, args, budget, peak):
 
 
 def batch_roofline(stats):
-    """Batched rounds kernel: per relaxed edge s(E) once plus 8 B per source
-    active at its vertex is the figure the work counters allow; they give the
-    union work (edges x active vertices), so the per-source term is priced
-    with the item count (sources active summed over items)."""
+    """Batched rounds kernel, algorithmic bytes over all rounds: per relaxed
+    edge s(E) once plus 8 B per source active at its vertex (destination
+    distance 4 + atomic 4; the lane_work counter is exactly the sum of
+    out-degree x active sources over the pushing vertices); per pushing
+    vertex its entry (id 4 + offset 8) and mask (8) plus 4 B per active
+    source (its own distance; lane_entries)."""
     eb = stats[-1].edge_bytes
     ms = sum(s.ms_rounds for s in stats)
     launches = sum(s.rounds_launches for s in stats)
     work = sum(s.work_edges for s in stats)
-    items = sum(s.work_items for s in stats)
+    lanes = sum(s.lane_work for s in stats)
+    lane_ents = sum(s.lane_entries for s in stats)
     ents = sum(s.work_entries for s in stats)
-    alg = eb * work + 8 * items + 12 * ents
+    alg = eb * work + 8 * lanes + 20 * ents + 4 * lane_ents
     achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "kernel": "rounds_batch_kernel", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": launches,

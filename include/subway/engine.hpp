@@ -88,6 +88,8 @@ struct RunStats {
     ull workEdges = 0;         // edges relaxed: sum of frontier work over all rounds
     ull workEntries = 0;       // vertices that pushed: sum over all rounds
     ull workItems = 0;         // partition items scanned: sum over all rounds
+    ull laneWork = 0;          // batched runs: (edge, active source) pairs relaxed over all rounds
+    ull laneEntries = 0;       // batched runs: active sources of the pushing vertices over all rounds
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)

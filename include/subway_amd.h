@@ -105,6 +105,8 @@ typedef struct subway_stats {
     uint32_t fallback_iteration;   /* learned policy: first iteration on the plain fallback (0 = none) */
     uint32_t edge_bytes;           /* bytes per edge moved and buffered (4 for packed weighted edges) */
     uint32_t min_members;          /* fewest member workgroups of a persistent launch in the run (0: none) */
+    uint64_t lane_work;            /* batched runs: (edge, active source) pairs relaxed over all rounds */
+    uint64_t lane_entries;         /* batched runs: active sources of the pushing vertices over all rounds */
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -131,6 +133,10 @@ const char *subway_last_error(void);
 void subway_default_options(subway_options *opt);
 /* number of visible HIP devices (0 without a GPU; never fails) */
 int32_t subway_device_count(void);
+/* sizes of subway_options, subway_stats and subway_iter_record as this
+ * library was built; a binding checks its own layouts against them (no
+ * reference counterpart) */
+void subway_struct_sizes(uint32_t *options, uint32_t *stats, uint32_t *iter_record);
 
 /* -------- graphs (host CSR in pinned, device-mapped memory) */
 /* Graph<E>(path, isWeighted).ReadGraph(): shared/graph.cu:32-197.

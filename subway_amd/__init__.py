@@ -66,6 +66,7 @@ class _Stats(ctypes.Structure):
         ("global_iterations", ctypes.c_uint32), ("trace_len", ctypes.c_uint32),
         ("work_items", ctypes.c_uint64), ("mispredictions", ctypes.c_uint32), ("fallback_iteration", ctypes.c_uint32),
         ("edge_bytes", ctypes.c_uint32), ("min_members", ctypes.c_uint32),
+        ("lane_work", ctypes.c_uint64), ("lane_entries", ctypes.c_uint64),
     ]
 
 
@@ -76,6 +77,20 @@ class _IterRecord(ctypes.Structure):
         ("features", ctypes.c_float * 12), ("heads", ctypes.c_float * 3), ("theta", ctypes.c_float),
         ("rounds_cap", ctypes.c_uint32), ("on_plain", ctypes.c_uint32),
     ]
+
+
+def _check_layouts():
+    """The ctypes mirrors above must match the library's structs byte for
+    byte; a stale library or binding fails here instead of corrupting runs."""
+    got = [ctypes.c_uint32() for _ in range(3)]
+    lib.subway_struct_sizes(*(ctypes.cast(ctypes.byref(x), _u32p) for x in got))
+    want = [ctypes.sizeof(_Options), ctypes.sizeof(_Stats), ctypes.sizeof(_IterRecord)]
+    if [x.value for x in got] != want:
+        raise ImportError(f"{LIB_PATH}: struct sizes (options, stats, iter_record) {[x.value for x in got]} "
+                          f"do not match this binding's {want}; rebuild the library")
+
+
+_check_layouts()
 
 
 def _ptr(a, t):
@@ -205,6 +220,8 @@ class RunStats:
     fallback_iteration: int = 0
     edge_bytes: int = 0
     min_members: int = 0
+    lane_work: int = 0
+    lane_entries: int = 0
     trace: list = field(default_factory=list)
 
 

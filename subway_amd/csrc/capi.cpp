@@ -81,6 +81,13 @@ void subway_default_options(subway_options *o)
     o->theta = 0.5f;
 }
 
+void subway_struct_sizes(uint32_t *options, uint32_t *stats, uint32_t *iter_record)
+{
+    if (options) *options = (uint32_t)sizeof(subway_options);
+    if (stats) *stats = (uint32_t)sizeof(subway_stats);
+    if (iter_record) *iter_record = (uint32_t)sizeof(subway_iter_record);
+}
+
 int32_t subway_device_count(void)
 {
     int n = 0;
@@ -287,6 +294,8 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->fallback_iteration = s.fallbackIteration;
     stats->edge_bytes = s.edgeBytes;
     stats->min_members = s.minMembers;
+    stats->lane_work = s.laneWork;
+    stats->lane_entries = s.laneEntries;
 }
 
 extern "C" {

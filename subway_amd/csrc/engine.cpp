@@ -418,7 +418,7 @@ private:
     RunStats loop(const std::chrono::steady_clock::time_point &t0);
     void resetRun();
     void checkWaits();
-    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0);
+    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0, ull lanes0, ull laneEnt0);
     uint *pinnedIds(ull n)
     {
         if (n > idsCap_) {
@@ -1251,6 +1251,7 @@ RunStats Engine<E, G>::run(uint source)
     const uint n = g_->num_nodes;
     ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
     ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
+    const ull lanes0 = h_ctl_->lane_work, laneEnt0 = h_ctl_->lane_entries;
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -1276,7 +1277,7 @@ RunStats Engine<E, G>::run(uint source)
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
-    finishStats(rounds0, updates0, work0, entries0, items0);
+    finishStats(rounds0, updates0, work0, entries0, items0, lanes0, laneEnt0);
     return st_;
 }
 
@@ -1299,6 +1300,7 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     launch_init_batch_state(o_.algo, n, d_dist_, d_m1_, d_m2_, 1, sc_);
     ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
     ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
+    const ull lanes0 = h_ctl_->lane_work, laneEnt0 = h_ctl_->lane_entries;
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -1316,12 +1318,13 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     batchK_ = 0;
-    finishStats(rounds0, updates0, work0, entries0, items0);
+    finishStats(rounds0, updates0, work0, entries0, items0, lanes0, laneEnt0);
     return st_;
 }
 
 template <class E, class G>
-void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0)
+void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0, ull lanes0,
+                               ull laneEnt0)
 {
     SUBWAY_CHECK(hipStreamSynchronize(sx_));
     st_.innerRounds = deviceRounds() - rounds0;
@@ -1329,6 +1332,8 @@ void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries
     st_.workEdges = h_ctl_->work_sum - work0;
     st_.workEntries = h_ctl_->entry_sum - entries0;
     st_.workItems = h_ctl_->item_sum - items0;
+    st_.laneWork = h_ctl_->lane_work - lanes0;
+    st_.laneEntries = h_ctl_->lane_entries - laneEnt0;
     st_.edgeBytes = (uint)eb_;
     if (pc_) {
         uint32_t mm = 0;

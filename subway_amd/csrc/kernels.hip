@@ -663,9 +663,11 @@ struct FrontierMaskOp {
     u64 *entryMask;
     u32 *chunkMap;
     u32 idBase;
+    u64 *laneStats;   // ScanCtl lane_work, lane_entries (every thread of the block calls load)
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
+        u64 lw = 0, lc = 0;
 #pragma unroll
         for (int j = 0; j < FR_IPT; j++) {
             u64 i = i0 + j;
@@ -683,9 +685,18 @@ struct FrontierMaskOp {
                         it[j].loc = (u32)(p0 - fromEdge);
                         it[j].id = id;
                         it[j].m = m;
+                        const u64 pc = (u64)__popcll(m);
+                        lw += (p1 - p0) * pc;
+                        lc += pc;
                     }
                 }
             }
+        }
+        lw = wave_reduce(lw);
+        lc = wave_reduce(lc);
+        if (lane_id() == 0 && lc) {
+            atomicAdd((unsigned long long *)&laneStats[0], (unsigned long long)lw);
+            atomicAdd((unsigned long long *)&laneStats[1], (unsigned long long)lc);
         }
     }
     __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &it, u64 deg) const
@@ -1315,7 +1326,7 @@ __device__ __forceinline__ void rounds_batch_body(const BatchArgs &a, u32 me, u3
             __syncthreads();
             if (t >= numTiles) break;
             FrontierMaskOp<true> op{a.nodes, a.ptr, a.fromEdge, labIn, r == 0 ? a.m2 : nullptr, a.entries, a.entryMask,
-                              a.chunkMap, a.idBase};
+                              a.chunkMap, a.idBase, &a.sc->lane_work};
             if (!scan_tile<FrontierMaskOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                wt))
                 break;
@@ -2061,7 +2072,7 @@ void launch_frontier_masks(ull n, const uint32_t *nodes, const ull *ptr, ull fro
     if (n == 0) throw Error(ERR_ARG, "frontier over an empty partition");
     u64 tiles = (n + FR_TILE - 1) / FR_TILE;
     check_tiles(s, tiles, st);
-    FrontierMaskOp<false> op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase};
+    FrontierMaskOp<false> op{nodes, ptr, fromEdge, labA, labB, entries, entryMask, chunkMap, idBase, &s.ctl->lane_work};
     hipLaunchKernelGGL((select_scan_kernel<FrontierMaskOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                        dim3(SCAN_BLOCK), 0, st, op, (u64)n, s.ctl, s.td, 1, s.spinLimit);
     SUBWAY_CHECK(hipGetLastError());

@@ -35,7 +35,13 @@ struct ScanCtl {
     // 32..255, >= 256), accumulated until the scheduler's policy kernel reads
     // and clears it
     u64 hist[4];
+    // batched frontiers: (edge, active source) pairs and active sources of
+    // the selected items, over all launches (statistics)
+    u64 lane_work;
+    u64 lane_entries;
 };
+static_assert(offsetof(ScanCtl, lane_entries) == offsetof(ScanCtl, lane_work) + 8,
+              "lane_work, lane_entries form one two-word block");
 static_assert(offsetof(ScanCtl, total) == offsetof(ScanCtl, count) + 8 &&
                   offsetof(ScanCtl, error) == offsetof(ScanCtl, count) + 16,
               "count, total and error form one [count, total, error] block of u64 words");

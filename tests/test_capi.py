@@ -34,6 +34,17 @@ def test_python_bindings_cover_the_header(sw):
     assert sorted(_lib.EXPORTED) == declared_symbols()
 
 
+def test_struct_layouts_match_the_library(sw):
+    """The ctypes mirrors of subway_options / subway_stats /
+    subway_iter_record have the sizes the library was built with (the import
+    refuses a mismatch; this names the sizes)."""
+    from subway_amd import _Options, _Stats, _IterRecord, _lib
+    got = [ctypes.c_uint32() for _ in range(3)]
+    _lib.lib.subway_struct_sizes(*(ctypes.cast(ctypes.byref(x), ctypes.POINTER(ctypes.c_uint32)) for x in got))
+    assert [x.value for x in got] == [ctypes.sizeof(_Options), ctypes.sizeof(_Stats), ctypes.sizeof(_IterRecord)]
+    assert ctypes.sizeof(_Stats) == _Stats.lane_entries.offset + 8
+
+
 def test_missing_file_is_an_io_error(sw, tmp_path):
     with pytest.raises(sw.SubwayError) as e:
         sw.Graph.load(str(tmp_path / "nope.bcsr"))

@@ -508,6 +508,13 @@ def test_batch_matches_single_source(sw, gpu, algo, budget, k):
     for i, s in enumerate(sources):
         exp = O.expected(algo, ptr, adj, w, int(s))
         assert np.array_equal(vals[i], exp), (algo, budget, k, i, int(np.sum(vals[i] != exp)))
+    # per-source work counters: one source bit per pushing vertex when k = 1,
+    # at most k otherwise, and every successful source update was a relaxation
+    assert st.work_entries <= st.lane_entries <= k * st.work_entries
+    assert st.work_edges <= st.lane_work <= k * st.work_edges
+    assert st.updates <= st.lane_work
+    if k == 1:
+        assert (st.lane_work, st.lane_entries) == (st.work_edges, st.work_entries)
 
 
 def test_batch_graph_replays_match_eager(sw, gpu):
