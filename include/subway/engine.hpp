@@ -54,6 +54,7 @@ struct RunOptions {
     uint32_t joinWindow = 0;   // persistent launches: membership window in 100 MHz ticks; 0 = 50000 (500 us)
     int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
     int noResident = 0;     // 1: move active edges every iteration even when the whole graph fits one buffer
+    uint residentMinBytes = 0;   // learned schedule: least active edge bytes for an in-place visit (0: 8 MiB)
 };
 
 struct IterRecord {

@@ -75,8 +75,11 @@ typedef struct subway_options {
                                workgroup waits for the rest of the grid before closing membership; 0 = 50000 */
     int32_t no_resident;    /* 1: move the active edges every global iteration even when the whole graph fits
                                one device buffer (the reference's behaviour); 0 (default): such a graph
-                               moves once per run and stays resident */
-    int32_t reserved[3];
+                               moves once per run and stays resident, and the learned schedule visits
+                               ranges whose whole edge slice a buffer still holds in place */
+    uint32_t resident_min_bytes;  /* learned schedule: in-place visits need at least this many bytes of active
+                               edges in the resident ranges; 0 = 8 MiB */
+    int32_t reserved[2];
 } subway_options;
 
 typedef struct subway_stats {

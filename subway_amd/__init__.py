@@ -49,7 +49,7 @@ class _Options(ctypes.Structure):
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
         ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
-        ("no_resident", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("no_resident", ctypes.c_int32), ("resident_min_bytes", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -240,7 +240,7 @@ def _set_policy(h, params):
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
              profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
-             join_window=0, resident=True):
+             join_window=0, resident=True, resident_min_bytes=0):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -263,6 +263,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.unpacked_edges = 0 if packed else 1
     o.join_window = join_window
     o.no_resident = 0 if resident else 1
+    o.resident_min_bytes = resident_min_bytes
     return o
 
 
@@ -287,10 +288,16 @@ class Session:
                  threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
-                 packed: bool = True, params=None, join_window: int = 0, resident: bool = True):
+                 packed: bool = True, params=None, join_window: int = 0, resident: bool = True,
+                 resident_min_bytes: int = 0):
+        """``resident``: a graph whose edges fit one buffer stays there, and
+        the learned schedule visits in place the ranges whose whole edge slice
+        a buffer still holds, when they hold ``resident_min_bytes`` (0: 8 MiB)
+        of active edges or more."""
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident)
+                     profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
+                     resident_min_bytes)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -235,6 +235,7 @@ static RunOptions to_options(const subway_options *o)
     r.joinWindow = o->join_window;
     r.unpackedEdges = o->unpacked_edges;
     r.noResident = o->no_resident;
+    r.residentMinBytes = o->resident_min_bytes;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;

@@ -117,6 +117,9 @@ struct Unit {
     ull fromEdge;       // subtracted from ptr to get buffer offsets
     uint idBase;        // first id of a whole range
     bool whole;         // whole id range resident (learned policy)
+    // a DMA of every edge of the ids [idLo, idHi) (whole ranges, fully active
+    // contiguous runs): the buffer then holds that id range's whole slice
+    uint idLo, idHi;
 };
 
 // One compute and one copy stream per device, shared by every engine of the
@@ -373,6 +376,10 @@ private:
     // DMA of this run (count 0: unknown); host edges never change, so a unit
     // asking for the same slice in the same buffer needs no copy
     ull bufHostEdge_[2] = {0, 0}, bufCount_[2] = {0, 0};
+    // id range whose whole edge slice the buffer holds (idHi > idLo), and the
+    // rotation of the unit -> buffer map of the current iteration
+    uint bufIdLo_[2] = {0, 0}, bufIdHi_[2] = {0, 0};
+    size_t bufShift_ = 0;
     void buildUnits();
     void issue(size_t u);
     void compute(size_t u);
@@ -684,19 +691,64 @@ void Engine<E, G>::applyPolicy(IterRecord &rec)
 // unit up to one buffer.  The other ranges' active slices are packed into
 // compacted units of at most one buffer (DMA when they form one fully active
 // contiguous run, else gathered).  Vertices larger than a buffer stream in
-// chunks.
+// chunks.  Before all of that, ranges whose whole edge slice a buffer still
+// holds from an earlier unit of the run are visited in place when their
+// active edges come to kResidentMinBytes or more: those units go first, each
+// at the position the iteration's rotation of the unit -> buffer map gives
+// its buffer, and need no transfer.  Below that the transfer saved (about
+// 150 us of the host link) does not pay for the visit's own persistent
+// launch and its scans of every id of the ranges in each round.
+static const ull kResidentMinBytes = 8ull << 20;
 template <class E, class G>
 void Engine<E, G>::buildUnitsLearned()
 {
     units_.clear();
+    bufShift_ = 0;
     const ull *np = g_->nodePointer;
     const size_t P = bounds_.size() - 1;
-    uint part = 0;
+    const int nb = sub_->numBuffers;
+    std::vector<char> done(P, 0);
+    Unit res[2] = {};
+    bool hasRes[2] = {false, false};
+    for (int b = 0; b < nb && !o_.noResident; b++) {
+        const uint lo = bufIdLo_[b], hi = bufIdHi_[b];
+        if (hi <= lo) continue;
+        // the slice's ranges (units are built from whole ranges)
+        const auto i0 = std::lower_bound(bounds_.begin(), bounds_.end(), lo) - bounds_.begin();
+        const auto i1 = std::lower_bound(bounds_.begin(), bounds_.end(), hi) - bounds_.begin();
+        if ((size_t)i1 > P || bounds_[(size_t)i0] != lo || bounds_[(size_t)i1] != hi) continue;
+        ull activeEdges = 0;
+        for (auto i = i0; i < i1; i++) activeEdges += h_rstats_[4 * i + 2];
+        const ull minBytes = o_.residentMinBytes ? (ull)o_.residentMinBytes : kResidentMinBytes;
+        if (activeEdges * (ull)eb_ < minBytes) continue;
+        for (auto i = i0; i < i1; i++) done[(size_t)i] = 1;
+        Unit &u = res[b];
+        u.dma = true;
+        u.hostEdge = np[lo];
+        u.count = np[hi] - np[lo];
+        u.whole = true;
+        u.n = hi - lo;
+        u.nodes = nullptr;
+        u.ptr = g_->d_nodePointer + lo;
+        u.fromEdge = np[lo];
+        u.idBase = lo;
+        u.idLo = lo;
+        u.idHi = hi;
+        hasRes[b] = true;
+    }
+    // resident units first; a lone one in buffer 1 rotates the map by one
+    if (hasRes[0]) units_.push_back(res[0]);
+    if (nb > 1 && hasRes[1]) {
+        if (!hasRes[0]) bufShift_ = 1;
+        units_.push_back(res[1]);
+    }
+    uint part = (uint)units_.size();
+    for (uint k = 0; k < part; k++) units_[k].part = k;
     // open compacted group
     bool open = false;
     ull gLo = 0, gHi = 0, gEdges = 0, gBase = 0;
     bool gFull = true;
-    uint gFirstRange = 0;
+    uint gFirstRange = 0, gLastRange = 0;
     // open whole group: vertex ids [wB0, wB1)
     bool wOpen = false;
     uint wB0 = 0, wB1 = 0;
@@ -712,6 +764,10 @@ void Engine<E, G>::buildUnitsLearned()
         u.dma = gFull || gEdges == 0;
         u.hostEdge = np[bounds_[gFirstRange]];
         if (o_.transfer == TRANSFER_ZEROCOPY && gEdges) u.dma = false;
+        if (u.dma && gFull) {
+            u.idLo = bounds_[gFirstRange];
+            u.idHi = bounds_[gLastRange + 1];
+        }
         setItems(u, gLo, gHi - gLo, gBase);
         units_.push_back(u);
         open = false;
@@ -729,10 +785,19 @@ void Engine<E, G>::buildUnitsLearned()
         u.ptr = g_->d_nodePointer + wB0;
         u.fromEdge = np[wB0];
         u.idBase = wB0;
+        u.idLo = wB0;
+        u.idHi = wB1;
         units_.push_back(u);
         wOpen = false;
     };
     for (size_t i = 0; i < P; i++) {
+        if (done[i]) {
+            // visited in place: a compacted group must not span its active
+            // vertices (they lie between the group's slices in the active list)
+            flush();
+            flushWhole();
+            continue;
+        }
         const ull lo = h_rstats_[4 * i], hi = h_rstats_[4 * i + 1], ea = h_rstats_[4 * i + 2];
         const ull base = h_rstats_[4 * i + 3];
         const uint b0 = bounds_[i], b1 = bounds_[i + 1];
@@ -776,11 +841,13 @@ void Engine<E, G>::buildUnitsLearned()
             gBase = base;
             gFull = full;
             gFirstRange = (uint)i;
+            gLastRange = (uint)i;
         } else {
             // the active slices of consecutive ranges are adjacent in the active list
             gFull = gFull && full;
             gHi = hi;
             gEdges += ea;
+            gLastRange = (uint)i;
         }
     }
     flush();
@@ -790,6 +857,7 @@ void Engine<E, G>::buildUnitsLearned()
 template <class E, class G>
 void Engine<E, G>::buildUnits()
 {
+    bufShift_ = 0;
     if (resident_) {
         // the whole graph as one unit: moved by the first iteration's DMA,
         // then read in place; its rounds cover every id
@@ -863,7 +931,7 @@ template <class E, class G>
 void Engine<E, G>::issue(size_t idx)
 {
     const Unit &u = units_[idx];
-    const int b = (int)(idx % (size_t)sub_->numBuffers);
+    const int b = (int)((idx + bufShift_) % (size_t)sub_->numBuffers);
     SUBWAY_CHECK(hipStreamWaitEvent(sx_, evFree_[b], 0));
     if (resident_ && residentLoaded_) {
         SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));   // already in buffer 0
@@ -876,6 +944,9 @@ void Engine<E, G>::issue(size_t idx)
     }
     bufCount_[b] = u.dma ? u.count : 0;
     bufHostEdge_[b] = u.hostEdge;
+    const bool slice = u.dma && u.count && u.idHi > u.idLo;
+    bufIdLo_[b] = slice ? u.idLo : 0u;
+    bufIdHi_[b] = slice ? u.idHi : 0u;
     if (u.count) {
         if (o_.profile) pXfer_.begin(sx_);
         char *dst = (char *)sub_->d_buffers[b];
@@ -1095,7 +1166,7 @@ template <class E, class G>
 void Engine<E, G>::compute(size_t idx)
 {
     const Unit &u = units_[idx];
-    const int b = (int)(idx % (size_t)sub_->numBuffers);
+    const int b = (int)((idx + bufShift_) % (size_t)sub_->numBuffers);
     SUBWAY_CHECK(hipStreamWaitEvent(sc_, evLoaded_[b], 0));
     uint8_t *l1 = nullptr, *l2 = nullptr;
     if constexpr (!isPR) {
@@ -1186,6 +1257,8 @@ void Engine<E, G>::resetRun()
     st_ = RunStats();
     residentLoaded_ = false;
     bufCount_[0] = bufCount_[1] = 0;   // every run moves its edges at least once
+    bufIdLo_[0] = bufIdLo_[1] = bufIdHi_[0] = bufIdHi_[1] = 0;
+    bufShift_ = 0;
     roundsCap_ = o_.maxRounds;
     theta_ = o_.theta;
     onPlain_ = false;

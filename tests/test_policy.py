@@ -169,3 +169,41 @@ def test_trained_network_gains_over_plain(sw, gpu):
     tl, vl = _median_ms(learned, src, 5)
     assert np.array_equal(vp, vl)
     assert tl <= 1.05 * tp, (tp, tl)
+
+
+@pytest.mark.parametrize("algo,mode", [("sssp", "async"), ("bfs", "async"), ("cc", "async"), ("pr", "async"),
+                                       ("sswp", "sync"), ("bfs", "sync")])
+def test_learned_schedule_reads_resident_ranges_in_place(sw, gpu, algo, mode):
+    """Under the learned schedule a range whose whole edge slice a buffer
+    still holds from an earlier unit of the run is visited in place.  When
+    the edges fit the two buffers, an async run transfers in its first
+    iteration only (every vertex is active there, so the ranges move whole)
+    when any active edge makes an in-place visit worth it (1 byte here);
+    values equal the plain schedule's (PR: within the oracle bound); and
+    resident=False moves the later iterations' edges again (the slice cache
+    alone still skips a unit identical to the buffer's last one)."""
+    g = _rmat(sw, 14, 9, weighted=algo in ("sssp", "sswp"))
+    src = g.max_degree_vertex
+    kw = dict(mode=mode, budget=int(g.m * 4 * 1.3), threshold=1e-6 if algo == "pr" else 0.01)
+    vp, _ = sw.Session(g, algo, **kw).run(src)
+    vl, st = sw.Session(g, algo, policy="learned", resident_min_bytes=1, **kw).run(src)
+    vo, so = sw.Session(g, algo, policy="learned", resident=False, **kw).run(src)
+    ptr, adj, w = g.csr()
+    if algo == "pr":
+        exp, _ = O.pr(ptr, adj, 1e-6)
+        for v in (vp, vl, vo):
+            assert (np.abs(v.astype(np.float64) - exp) / exp).max() < 1e-4
+    else:
+        exp = O.expected(algo, ptr, adj, w, src)
+        for v in (vp, vl, vo):
+            assert np.array_equal(v, exp)
+    later = sum(r["h2d_bytes"] for r in st.trace[1:])
+    later_off = sum(r["h2d_bytes"] for r in so.trace[1:])
+    assert later <= later_off
+    if mode == "async":
+        assert st.global_iterations >= 2 and so.global_iterations >= 2
+        assert st.trace[0]["h2d_bytes"] == g.m * st.edge_bytes
+        assert later == 0
+        if algo in ("sssp", "bfs"):
+            # their later frontiers are sparse: without reuse they are gathered again
+            assert later_off > 0
