@@ -238,26 +238,38 @@ def batch_roofline(stats):
 
 def roofline(np, stats, args, budget):
     """Roofline of the dominant kernel.  Persistent async rounds: one
-    cooperative rounds_kernel launch per partition visit does every inner
-    round (frontier + advance).  Algorithmic bytes, summed over the rounds of
-    a launch: 22 B per labelled active vertex (id 4 + offset 8 + degree 4 +
-    label r/w 2 + value 4), s(E) + 4 B per relaxed edge (edge as buffered +
-    destination value), 9 B per successful update (atomic RMW 8 + label 1).
-    The frontier pass over unlabelled items is not counted.  Launch-per-round
-    mode: the advance_kernel, edges and updates only."""
+    rounds_kernel launch per partition visit does every inner round (frontier
+    + advance).  Algorithmic bytes, summed over the rounds: label algorithms
+    22 B per labelled active vertex (id 4 + offset 8 + degree 4 + label r/w 2
+    + value 4), s(E) + 4 B per relaxed edge (edge as buffered + destination
+    value), 9 B per successful update (atomic RMW 8 + label 1); PageRank
+    (FP64 residual) 48 B per pushing vertex (id 4 + offset 8 + degree 4 +
+    residual read and clear 16 + running-sum RMW 16) and s(E) + 16 B per
+    edge (FP64 atomic add RMW).  The frontier pass over unlabelled items is
+    not counted.  Partition visits run either as one persistent rounds_kernel
+    launch or as stand-alone frontier + advance launch pairs (a round cap of
+    one, launch-per-round mode, oversized vertices); the run's counters tell
+    the pairs' share apart, and the kind with more kernel time is reported,
+    priced with its own work only."""
     eb = stats[-1].edge_bytes
     rounds_ms = sum(s.ms_rounds for s in stats)
     rounds_launches = sum(s.rounds_launches for s in stats)
-    work = sum(s.work_edges for s in stats)
-    upd = sum(s.updates for s in stats)
-    ents = sum(s.work_entries for s in stats)
-    if rounds_launches and rounds_ms > 0:
+    pair_ms = sum(s.ms_advance + s.ms_frontier for s in stats)
+    adv_launches = sum(s.advance_launches for s in stats)
+    if rounds_launches and rounds_ms >= pair_ms:
         kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
-        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
+        work = sum(s.work_edges - s.pair_work_edges for s in stats)
+        upd = sum(s.updates - s.pair_updates for s in stats)
+        ents = sum(s.work_entries - s.pair_work_entries for s in stats)
     else:
-        kern, k_ms = "advance_kernel", sum(s.ms_advance for s in stats)
-        k_launches = sum(s.advance_launches for s in stats)
-        alg_bytes = (eb + 4) * work + 9 * upd
+        kern, k_ms, k_launches = "advance_kernel+frontier_kernel", pair_ms, adv_launches
+        work = sum(s.pair_work_edges for s in stats)
+        upd = sum(s.pair_updates for s in stats)
+        ents = sum(s.pair_work_entries for s in stats)
+    if args.algo == "pr":
+        alg_bytes = 48 * ents + (eb + 16) * work
+    else:
+        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     traffic = None
     if (kern == "rounds_kernel" and args.algo == "sssp" and args.mode == "async" and args.scale == 24

@@ -91,6 +91,12 @@ struct RunStats {
     ull workItems = 0;         // partition items scanned: sum over all rounds
     ull laneWork = 0;          // batched runs: (edge, active source) pairs relaxed over all rounds
     ull laneEntries = 0;       // batched runs: active sources of the pushing vertices over all rounds
+    // the part of workEdges, workEntries and updates done by stand-alone
+    // frontier + advance launches (round cap of one, launch-per-round mode,
+    // oversized vertices) rather than persistent rounds launches
+    ull pairWorkEdges = 0;
+    ull pairWorkEntries = 0;
+    ull pairUpdates = 0;
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)

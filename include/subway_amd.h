@@ -110,6 +110,12 @@ typedef struct subway_stats {
     uint32_t min_members;          /* fewest member workgroups of a persistent launch in the run (0: none) */
     uint64_t lane_work;            /* batched runs: (edge, active source) pairs relaxed over all rounds */
     uint64_t lane_entries;         /* batched runs: active sources of the pushing vertices over all rounds */
+    /* the part of work_edges, work_entries and updates done by stand-alone
+     * frontier + advance launches (round cap of one, launch-per-round mode,
+     * oversized vertices) rather than persistent rounds launches */
+    uint64_t pair_work_edges;
+    uint64_t pair_work_entries;
+    uint64_t pair_updates;
 } subway_stats;
 
 typedef struct subway_iter_record {

@@ -67,6 +67,7 @@ class _Stats(ctypes.Structure):
         ("work_items", ctypes.c_uint64), ("mispredictions", ctypes.c_uint32), ("fallback_iteration", ctypes.c_uint32),
         ("edge_bytes", ctypes.c_uint32), ("min_members", ctypes.c_uint32),
         ("lane_work", ctypes.c_uint64), ("lane_entries", ctypes.c_uint64),
+        ("pair_work_edges", ctypes.c_uint64), ("pair_work_entries", ctypes.c_uint64), ("pair_updates", ctypes.c_uint64),
     ]
 
 
@@ -222,6 +223,9 @@ class RunStats:
     min_members: int = 0
     lane_work: int = 0
     lane_entries: int = 0
+    pair_work_edges: int = 0
+    pair_work_entries: int = 0
+    pair_updates: int = 0
     trace: list = field(default_factory=list)
 
 

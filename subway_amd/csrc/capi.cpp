@@ -297,6 +297,9 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->min_members = s.minMembers;
     stats->lane_work = s.laneWork;
     stats->lane_entries = s.laneEntries;
+    stats->pair_work_edges = s.pairWorkEdges;
+    stats->pair_work_entries = s.pairWorkEntries;
+    stats->pair_updates = s.pairUpdates;
 }
 
 extern "C" {

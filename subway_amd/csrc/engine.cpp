@@ -425,7 +425,10 @@ private:
     RunStats loop(const std::chrono::steady_clock::time_point &t0);
     void resetRun();
     void checkWaits();
-    void finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0, ull lanes0, ull laneEnt0);
+    // statistics counters at the start of the run (host mirror of the scan control block)
+    ScanCtl ctl0_{};
+    void startStats();
+    void finishStats();
     uint *pinnedIds(ull n)
     {
         if (n > idsCap_) {
@@ -1322,9 +1325,7 @@ RunStats Engine<E, G>::run(uint source)
     resetRun();
     initState();
     const uint n = g_->num_nodes;
-    ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
-    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
-    const ull lanes0 = h_ctl_->lane_work, laneEnt0 = h_ctl_->lane_entries;
+    startStats();
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -1350,7 +1351,7 @@ RunStats Engine<E, G>::run(uint source)
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
-    finishStats(rounds0, updates0, work0, entries0, items0, lanes0, laneEnt0);
+    finishStats();
     return st_;
 }
 
@@ -1371,9 +1372,7 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     std::memcpy(h_sources_, sources, (size_t)k * sizeof(uint32_t));
     SUBWAY_CHECK(hipMemcpyAsync(d_sources_, h_sources_, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, sc_));
     launch_init_batch_state(o_.algo, n, d_dist_, d_m1_, d_m2_, 1, sc_);
-    ull rounds0 = deviceRounds(), updates0 = h_ctl_->updates;
-    ull work0 = h_ctl_->work_sum, entries0 = h_ctl_->entry_sum, items0 = h_ctl_->item_sum;
-    const ull lanes0 = h_ctl_->lane_work, laneEnt0 = h_ctl_->lane_entries;
+    startStats();
     for (int b = 0; b < 2; b++) {
         SUBWAY_CHECK(hipEventRecord(evFree_[b], sc_));
         SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
@@ -1391,22 +1390,32 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     batchK_ = 0;
-    finishStats(rounds0, updates0, work0, entries0, items0, lanes0, laneEnt0);
+    finishStats();
     return st_;
 }
 
 template <class E, class G>
-void Engine<E, G>::finishStats(ull rounds0, ull updates0, ull work0, ull entries0, ull items0, ull lanes0,
-                               ull laneEnt0)
+void Engine<E, G>::startStats()
+{
+    (void)deviceRounds();   // refreshes the host mirror
+    ctl0_ = *h_ctl_;
+}
+
+template <class E, class G>
+void Engine<E, G>::finishStats()
 {
     SUBWAY_CHECK(hipStreamSynchronize(sx_));
-    st_.innerRounds = deviceRounds() - rounds0;
-    st_.updates = h_ctl_->updates - updates0;
-    st_.workEdges = h_ctl_->work_sum - work0;
-    st_.workEntries = h_ctl_->entry_sum - entries0;
-    st_.workItems = h_ctl_->item_sum - items0;
-    st_.laneWork = h_ctl_->lane_work - lanes0;
-    st_.laneEntries = h_ctl_->lane_entries - laneEnt0;
+    st_.innerRounds = deviceRounds() - ctl0_.rounds;
+    const ScanCtl &c1 = *h_ctl_;
+    st_.updates = c1.updates - ctl0_.updates;
+    st_.workEdges = c1.work_sum - ctl0_.work_sum;
+    st_.workEntries = c1.entry_sum - ctl0_.entry_sum;
+    st_.workItems = c1.item_sum - ctl0_.item_sum;
+    st_.laneWork = c1.lane_work - ctl0_.lane_work;
+    st_.laneEntries = c1.lane_entries - ctl0_.lane_entries;
+    st_.pairWorkEdges = c1.pair_work - ctl0_.pair_work;
+    st_.pairWorkEntries = c1.pair_entries - ctl0_.pair_entries;
+    st_.pairUpdates = c1.pair_updates - ctl0_.pair_updates;
     st_.edgeBytes = (uint)eb_;
     if (pc_) {
         uint32_t mm = 0;

@@ -455,13 +455,19 @@ struct GenPrOp {
 // it: inside a persistent launch that is a different workgroup, often on
 // another XCD, each round, and the XCDs' L2 caches are not coherent, so the
 // counters are device-scope atomics (a plain read-modify-write loses
-// increments).
+// increments).  Stand-alone scans (COH false) also count into the pair_*
+// words, so the persistent launches' share can be told apart.
+template <bool COH>
 __device__ __forceinline__ void stat_counts(ScanCtl *ctl, u64 cnt, u64 sum)
 {
     st_relaxed(&ctl->stop, cnt == 0 ? 1u : 0u);
     if (cnt) atomicAdd((unsigned long long *)&ctl->rounds, 1ull);
     atomicAdd((unsigned long long *)&ctl->entry_sum, (unsigned long long)cnt);
     atomicAdd((unsigned long long *)&ctl->work_sum, (unsigned long long)sum);
+    if constexpr (!COH) {
+        atomicAdd((unsigned long long *)&ctl->pair_entries, (unsigned long long)cnt);
+        atomicAdd((unsigned long long *)&ctl->pair_work, (unsigned long long)sum);
+    }
 }
 
 struct FrItem {
@@ -551,7 +557,7 @@ struct FrontierLabelOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        stat_counts(ctl, cnt, sum);
+        stat_counts<COH>(ctl, cnt, sum);
     }
 };
 
@@ -602,7 +608,7 @@ struct FrontierPrOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        stat_counts(ctl, cnt, sum);
+        stat_counts<COH>(ctl, cnt, sum);
     }
 };
 
@@ -708,7 +714,7 @@ struct FrontierMaskOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        stat_counts(ctl, cnt, sum);
+        stat_counts<COH>(ctl, cnt, sum);
     }
 };
 
@@ -923,7 +929,10 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
         updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
                                                  wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk);
     u32 wsum = wave_reduce(updates);
-    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
+    if (lane_id() == 0 && wsum) {
+        atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
+        atomicAdd((unsigned long long *)&ctl->pair_updates, (unsigned long long)wsum);
+    }
     if (tid == 0) {
         u32 d = fetch_add_relaxed(&ctl->adv_done, 1u);
         if (d == gridDim.x - 1) {
@@ -1399,7 +1408,10 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_batch_kernel(const Entry *e
         updates += advance_chunk_batch<ALGO, E, 4, false>(j, entries, entryMask, chunkMap, edges, dist, maskOut, count, W,
                                                    w0, wend, shift, psh, &ctl->error);
     u32 wsum = wave_reduce(updates);
-    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
+    if (lane_id() == 0 && wsum) {
+        atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
+        atomicAdd((unsigned long long *)&ctl->pair_updates, (unsigned long long)wsum);
+    }
 }
 
 // Batched initial state: every distance INF (SSWP: 0), masks cleared

@@ -39,6 +39,11 @@ struct ScanCtl {
     // the selected items, over all launches (statistics)
     u64 lane_work;
     u64 lane_entries;
+    // the part of entry_sum, work_sum and updates that stand-alone
+    // (non-persistent) frontier and advance launches contributed
+    u64 pair_entries;
+    u64 pair_work;
+    u64 pair_updates;
 };
 static_assert(offsetof(ScanCtl, lane_entries) == offsetof(ScanCtl, lane_work) + 8,
               "lane_work, lane_entries form one two-word block");

@@ -400,6 +400,24 @@ def test_split_layout_graph(sw, gpu, algo, mode):
         sw.run(g, algo, mode=mode, source=src, budget=96 << 10, packed=False)
 
 
+def test_pair_counters_split_the_work(sw, gpu):
+    """The work of stand-alone frontier + advance launches is counted apart
+    from the persistent launches': all of it in launch-per-round mode and
+    with a round cap of one, none of it in a persistent run without
+    oversized vertices; the values are the oracle's in all three."""
+    g = sw.Graph.rmat(14, 16, seed=3, weighted=True)
+    src = g.max_degree_vertex
+    kw = dict(src=src, budget=128 << 10)
+    for extra in (dict(rounds="per-round"), dict(max_rounds=1)):
+        _, st = check_app(sw, g, "sssp", "async", **kw, **extra)
+        assert st.work_edges > 0
+        assert (st.pair_work_edges, st.pair_work_entries, st.pair_updates) == \
+            (st.work_edges, st.work_entries, st.updates), extra
+    _, st = check_app(sw, g, "sssp", "async", **kw)
+    assert st.work_edges > 0
+    assert (st.pair_work_edges, st.pair_work_entries, st.pair_updates) == (0, 0, 0)
+
+
 @pytest.mark.parametrize("ext", ["bwcsr", "bwcsr64"])
 def test_split_layout_file(sw, gpu, tmp_path, ext):
     """A .bwcsr file read in the split layout (the layout a weighted file of
