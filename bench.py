@@ -311,8 +311,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is not built "
-              "-- see the sharding plan in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the multi-GPU path (disjoint "
+              "partitions per GPU with a per-iteration all-reduce of vertex values, or batched sources split "
+              "across GPUs) is not built", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -627,8 +627,10 @@ void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, u
 // whole-range density threshold and the inner-round cap.  The plain policy is
 // the fallback and the judge: a prediction more than 2x off the active edges
 // the iteration then shows is a misprediction, and once more than half of at
-// least three judged predictions missed, the rest of the run goes to the
-// plain schedule.  Without parameters the options' fixed theta and cap apply.
+// least eight judged predictions missed, the rest of the run goes to the
+// plain schedule (iterations with under 1e-4 of the edges active, predicted
+// and actual, are not judged).  Without parameters the options' fixed theta
+// and cap apply.
 // The schedules of the training sweep: whole-range density thresholds and
 // inner-round caps.  Head 1 is theta; head 2 is log2(cap) / 16.
 static const double kThetas[4] = {0.25, 0.5, 0.8, 2.0};
