@@ -88,6 +88,24 @@ def test_64bit_variant_round_trip(sw, tmp_path, weighted):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("ext", ["bwcsr", "bwcsr64"])
+def test_split_layout_read_keeps_the_csr(sw, tmp_path, ext):
+    """A .bwcsr file read for the split layout goes through ordinary memory
+    first; the CSR read back is the file's (without a GPU nothing is pinned;
+    the split arrays themselves are checked by the GPU tests).  Text files
+    refuse the layout."""
+    g = sw.Graph.rmat(11, 8, seed=5, weighted=True)
+    p = tmp_path / f"g.{ext}"
+    g.save(str(p))
+    g2 = sw.Graph.load(str(p), split_layout=True)
+    for a, b in zip(g.csr(), g2.csr()):
+        assert np.array_equal(a, b)
+    t = tmp_path / "g.wel"
+    t.write_text("0 1 3\n1 2 4\n")
+    with pytest.raises(sw.SubwayError, match="split layout"):
+        sw.Graph.load(str(t), split_layout=True)
+
+
 def test_legacy_layout_fields(sw, tmp_path):
     g = sw.Graph.rmat(10, 4, seed=9)
     p = tmp_path / "g.bcsr"
