@@ -349,10 +349,11 @@ class BatchSession:
 
     def __init__(self, graph: Graph, algo: str = "sssp", budget: int = 0, buffers: int = 2, transfer: str = "auto",
                  unroll: int = 0, rounds_grid: int = 0, profile: bool = False, policy: str = "plain",
-                 theta: float = 0.5, packed: bool = True, params=None, join_window: int = 0):
+                 theta: float = 0.5, packed: bool = True, params=None, join_window: int = 0,
+                 resident: bool = True, resident_min_bytes: int = 0):
         self.graph, self.algo = graph, algo
         o = _options(algo, "async", 0, budget, buffers, 0.01, 0.15, transfer, policy, False, 0, profile,
-                     "persistent", rounds_grid, 0, theta, 0, packed, join_window)
+                     "persistent", rounds_grid, 0, theta, 0, packed, join_window, resident, resident_min_bytes)
         o.batch = 1
         o.batch_unroll = unroll
         h = ctypes.c_void_p()

@@ -207,3 +207,37 @@ def test_learned_schedule_reads_resident_ranges_in_place(sw, gpu, algo, mode):
         if algo in ("sssp", "bfs"):
             # their later frontiers are sparse: without reuse they are gathered again
             assert later_off > 0
+
+
+@pytest.mark.parametrize("buffers", [1, 2])
+def test_resident_ranges_with_transfers_in_the_same_iteration(sw, gpu, buffers):
+    """Resident ranges visited in place beside units that still transfer
+    (the graph does not fit the buffers): with one buffer the in-place visit
+    comes first and the transfers then reuse that buffer, with two the unit
+    -> buffer map may rotate; values are the oracle's and the later
+    iterations move no more than without the reuse."""
+    g = _rmat(sw, 15, 4)
+    src = g.max_degree_vertex
+    kw = dict(mode="async", budget=int(g.m * 4 * 0.7), buffers=buffers, policy="learned")
+    vl, st = sw.Session(g, "sssp", resident_min_bytes=1, **kw).run(src)
+    vo, so = sw.Session(g, "sssp", resident=False, **kw).run(src)
+    ptr, adj, w = g.csr()
+    exp = O.sssp(ptr, adj, w, src)
+    assert np.array_equal(vl, exp) and np.array_equal(vo, exp)
+    assert sum(r["h2d_bytes"] for r in st.trace[1:]) <= sum(r["h2d_bytes"] for r in so.trace[1:])
+
+
+def test_batched_learned_schedule_with_resident_ranges(sw, gpu):
+    """Batched runs under the learned schedule visit resident ranges in place
+    too (the persistent batched kernel over a whole id range): every source's
+    row is the oracle's."""
+    g = _rmat(sw, 14, 12)
+    ptr, adj, w = g.csr()
+    rng = np.random.default_rng(3)
+    sources = rng.choice(np.nonzero(np.diff(ptr) > 0)[0], 16).astype(np.uint32)
+    bs = sw.BatchSession(g, "sssp", budget=int(g.m * 4 * 1.3), policy="learned", resident_min_bytes=1)
+    rows, st = bs.run(sources)
+    for i, s in enumerate(sources):
+        assert np.array_equal(rows[i], O.sssp(ptr, adj, w, int(s))), i
+    assert st.global_iterations >= 2
+    assert sum(r["h2d_bytes"] for r in st.trace[1:]) == 0
