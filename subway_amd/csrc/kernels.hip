@@ -760,46 +760,73 @@ __device__ __forceinline__ u32 edge_weight(const E &e)
     else return 1u;
 }
 
-// Storage format -> logical edge of a partition buffer.
+// Storage format -> logical edge of a partition buffer, in two steps: `load`
+// issues the memory reads of edge i, `decode` turns what they returned into
+// the edge.  The advance issues every load of a phase before decoding any:
+// a use of a loaded value makes the compiler wait for that load there, so
+// decoding right after each load would serialize the phase.
 template <class S>
 struct Codec;
 template <>
 struct Codec<OutEdge> {
     typedef OutEdge T;
-    __device__ __forceinline__ static T get(const OutEdge *p, u64 i, u32) { return p[i]; }
+    typedef OutEdge Raw;
+    __device__ __forceinline__ static Raw load(const OutEdge *p, u64 i, u32) { return p[i]; }
+    __device__ __forceinline__ static T decode(const Raw &r, u32) { return r; }
+    __device__ __forceinline__ static T get(const OutEdge *p, u64 i, u32 c) { return decode(load(p, i, c), c); }
 };
 template <>
 struct Codec<OutEdgeWeighted> {
     typedef OutEdgeWeighted T;
-    __device__ __forceinline__ static T get(const OutEdgeWeighted *p, u64 i, u32) { return p[i]; }
+    typedef OutEdgeWeighted Raw;
+    __device__ __forceinline__ static Raw load(const OutEdgeWeighted *p, u64 i, u32) { return p[i]; }
+    __device__ __forceinline__ static T decode(const Raw &r, u32) { return r; }
+    __device__ __forceinline__ static T get(const OutEdgeWeighted *p, u64 i, u32 c)
+    {
+        return decode(load(p, i, c), c);
+    }
 };
 template <>
 struct Codec<PackedEdge> {
     typedef OutEdgeWeighted T;
+    typedef u32 Raw;
+    __device__ __forceinline__ static Raw load(const PackedEdge *p, u64 i, u32) { return p[i].bits; }
     // code = destination bits | weight base << 5
-    __device__ __forceinline__ static T get(const PackedEdge *p, u64 i, u32 code)
+    __device__ __forceinline__ static T decode(const Raw &x, u32 code)
     {
-        const u32 x = p[i].bits;
         const u32 sh = code & 31u;
         T e;
         e.end = x & ((1u << sh) - 1u);
         e.w8 = (x >> sh) + (code >> 5);
         return e;
     }
+    __device__ __forceinline__ static T get(const PackedEdge *p, u64 i, u32 c) { return decode(load(p, i, c), c); }
 };
 
 template <>
 struct Codec<SplitEdge> {
     typedef OutEdgeWeighted T;
+    struct Raw {
+        u32 dst;
+        u32 w;
+    };
     // cap = the buffer's capacity in edges: the weight of edge i is byte i of
     // the array that starts at byte 4 * cap
-    __device__ __forceinline__ static T get(const SplitEdge *p, u64 i, u32 cap)
+    __device__ __forceinline__ static Raw load(const SplitEdge *p, u64 i, u32 cap)
+    {
+        Raw r;
+        r.dst = p[i].dst;
+        r.w = reinterpret_cast<const uint8_t *>(p)[4ull * cap + i];
+        return r;
+    }
+    __device__ __forceinline__ static T decode(const Raw &r, u32)
     {
         T e;
-        e.end = p[i].dst;
-        e.w8 = reinterpret_cast<const uint8_t *>(p)[4ull * cap + i];
+        e.end = r.dst;
+        e.w8 = r.w;
         return e;
     }
+    __device__ __forceinline__ static T get(const SplitEdge *p, u64 i, u32 c) { return decode(load(p, i, c), c); }
 };
 
 // Relaxes the edges of one work chunk [cbeg, cbeg + ADV_CHUNK) clipped to
@@ -840,17 +867,19 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     const u32 next = (tid == 0) ? atomicAdd(claimCtr, 1u) : 0u;
     // every thread owns positions pbeg + tid + u * ADV_BLOCK, u < PPT; the
     // phases below keep PPT edge loads, then PPT value loads, then PPT
-    // atomics in flight per thread
+    // atomics in flight per thread.  Values a guarded load or atomic returns
+    // get no default: they are read only under the same guard, so no merge
+    // of the returned value with a default (which makes the compiler wait
+    // for the memory operation right there) is needed.
     constexpr int PPT = ADV_CHUNK / ADV_BLOCK;
     static_assert(ADV_CHUNK % ADV_BLOCK == 0, "a chunk is a whole number of positions per thread");
-    u32 dst[PPT], cv[PPT];
+    typename Codec<E>::Raw raw[PPT];
+    u32 sv[PPT];
     bool ok[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; u++) {
         const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
         ok[u] = p < pend;
-        dst[u] = 0;
-        cv[u] = 0;
         if (ok[u]) {
             const u32 pp = (u32)p;
             int lo = 0, hi = nk - 1;
@@ -860,12 +889,19 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
                 else hi = mid - 1;
             }
             const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-            const auto e = Codec<E>::get(edges, eidx, psh);
-            dst[u] = e.end;
-            if constexpr (ALGO == ALGO_PR) cv[u] = s_val[lo];
-            else cv[u] = Rule<ALGO>::cand(s_val[lo], edge_weight(e));
+            raw[u] = Codec<E>::load(edges, eidx, psh);
+            sv[u] = s_val[lo];
         }
     }
+    u32 dst[PPT], cv[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; u++)
+        if (ok[u]) {
+            const auto e = Codec<E>::decode(raw[u], psh);
+            dst[u] = e.end;
+            if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
+            else cv[u] = Rule<ALGO>::cand(sv[u], edge_weight(e));
+        }
     u32 updates = 0;
     if constexpr (ALGO == ALGO_PR) {
         // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
@@ -881,14 +917,23 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     } else {
         u32 cur[PPT];
 #pragma unroll
-        for (int u = 0; u < PPT; u++) cur[u] = ok[u] ? value[dst[u]] : 0u;
-        u32 old[PPT];
+        for (int u = 0; u < PPT; u++)
+            if (ok[u]) cur[u] = value[dst[u]];
+        // every test before the first atomic: a test between two atomics
+        // would wait for the earlier atomic as well
         bool tried[PPT];
 #pragma unroll
-        for (int u = 0; u < PPT; u++) {
-            tried[u] = ok[u] && Rule<ALGO>::better(cv[u], cur[u]);
-            old[u] = tried[u] ? Rule<ALGO>::atomic(&value[dst[u]], cv[u]) : 0u;
-        }
+        for (int u = 0; u < PPT; u++) tried[u] = ok[u] && Rule<ALGO>::better(cv[u], cur[u]);
+        u32 old[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (tried[u]) old[u] = Rule<ALGO>::atomic(&value[dst[u]], cv[u]);
+        // the returned values are taken only after every atomic is issued:
+        // the compiler would otherwise fold each improvement test into its
+        // atomic's branch and wait for that atomic there
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (tried[u]) asm volatile("" : "+v"(old[u]));
         // a label is set only where the atomic's old value shows the improvement
 #pragma unroll
         for (int u = 0; u < PPT; u++)
