@@ -1273,56 +1273,147 @@ __device__ __forceinline__ u32 advance_chunk_batch(u64 j, const Entry *entries, 
             if (b_pre[mid] <= pq) lo = mid;
             else hi = mid - 1;
         }
-        int k = lo, kcur = -1;
-        u32 du = 0;
-        for (u64 p = q0; p < q1; p += U) {
-            u32 vv[U], ww[U], dd[U], dv[U];
-            u64 mm[U];
-            bool ok[U];
+        // up to 8 positions per step: every load and atomic of a phase is
+        // issued before any result is used; at 16 and more the arrays this
+        // needs spill under the four-wave register limit, and a spill store
+        // of a loaded register waits for the load, so those steps keep the
+        // per-position first phase
+        if constexpr (U <= 8) {
+            int k = lo, kcur = -1;
+            u32 du = 0;
+            for (u64 p = q0; p < q1; p += U) {
+                // phase 1: entries (LDS), raw edge loads and, where the entry
+                // changes, the pushing vertex's row; nothing loaded is used
+                // yet (guarded loads and atomics get no default value: they
+                // are read only under their guard)
+                typename Codec<E>::Raw raw[U];
+                u32 ldu[U];
+                u64 mm[U];
+                bool ok[U], fresh[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const u64 pp = p + u;
-                ok[u] = pp < q1;
-                mm[u] = 0;
-                if (ok[u]) {
-                    while (k + 1 < nk && b_pre[k + 1] <= (u32)pp) k++;
-                    if (k != kcur) {
-                        kcur = k;
-                        du = mld<COH>(&dist[(u64)b_id[k] * WAVE + lane]);
+                for (int u = 0; u < U; u++) {
+                    const u64 pp = p + u;
+                    ok[u] = pp < q1;
+                    fresh[u] = false;
+                    if (ok[u]) {
+                        while (k + 1 < nk && b_pre[k + 1] <= (u32)pp) k++;
+                        if (k != kcur) {
+                            kcur = k;
+                            fresh[u] = true;
+                            ldu[u] = mld<COH>(&dist[(u64)b_id[k] * WAVE + lane]);
+                        }
+                        const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
+                        raw[u] = Codec<E>::load(edges, eidx, psh);
+                        mm[u] = b_msk[k];
                     }
-                    const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
-                    const auto e = Codec<E>::get(edges, eidx, psh);
-                    vv[u] = e.end;
-                    ww[u] = edge_weight(e);
-                    mm[u] = b_msk[k];
+                }
+                // phase 2: decode, destination rows of the active sources
+                u32 vv[U], ww[U], dv[U];
+                bool act[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    act[u] = false;
+                    if (ok[u]) {
+                        const auto e = Codec<E>::decode(raw[u], psh);
+                        vv[u] = e.end;
+                        ww[u] = edge_weight(e);
+                        act[u] = ((mm[u] >> lane) & 1ull) != 0;
+                        if (act[u]) dv[u] = dist[(u64)vv[u] * WAVE + lane];
+                    }
+                }
+                // the pushing rows, taken only now that every load of the step
+                // is issued (the memory clobber keeps the loads above on this
+                // side; otherwise each take moves into its load's branch and
+                // waits for that load there)
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (fresh[u]) asm volatile("" : "+v"(ldu[u])::"memory");
+                u32 dd[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (fresh[u]) du = ldu[u];
                     dd[u] = du;
                 }
-            }
+                // every test before the first atomic, then all U atomics, then
+                // their results (taken after the last atomic is issued)
+                u32 cc[U], ol[U];
+                bool tr[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                dv[u] = 0;
-                if (ok[u] && ((mm[u] >> lane) & 1ull)) dv[u] = dist[(u64)vv[u] * WAVE + lane];
-            }
-            // all U atomics are issued before any result is used
-            u32 cc[U], ol[U];
-            bool tr[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                tr[u] = false;
-                cc[u] = 0;
-                if (ok[u] && ((mm[u] >> lane) & 1ull)) {
-                    cc[u] = Rule<ALGO>::cand(dd[u], ww[u]);
-                    tr[u] = Rule<ALGO>::better(cc[u], dv[u]);
+                for (int u = 0; u < U; u++) {
+                    tr[u] = false;
+                    if (act[u]) {
+                        cc[u] = Rule<ALGO>::cand(dd[u], ww[u]);
+                        tr[u] = Rule<ALGO>::better(cc[u], dv[u]);
+                    }
                 }
-                ol[u] = tr[u] ? Rule<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], cc[u]) : 0u;
-            }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const bool imp = tr[u] && Rule<ALGO>::better(cc[u], ol[u]);
-                const unsigned long long bal = __ballot(imp);
-                if (bal) {
-                    if (lane == 0) atomicOr((unsigned long long *)&maskOut[vv[u]], bal);
-                    updates += (lane == 0) ? (u32)__popcll(bal) : 0u;
+                for (int u = 0; u < U; u++)
+                    if (tr[u]) ol[u] = Rule<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], cc[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (tr[u]) asm volatile("" : "+v"(ol[u]));
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool imp = tr[u] && Rule<ALGO>::better(cc[u], ol[u]);
+                    const unsigned long long bal = __ballot(imp);
+                    if (bal) {
+                        if (lane == 0) atomicOr((unsigned long long *)&maskOut[vv[u]], bal);
+                        updates += (lane == 0) ? (u32)__popcll(bal) : 0u;
+                    }
+                }
+            }
+        } else {
+            int k = lo, kcur = -1;
+            u32 du = 0;
+            for (u64 p = q0; p < q1; p += U) {
+                u32 vv[U], ww[U], dd[U], dv[U];
+                u64 mm[U];
+                bool ok[U];
+    #pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const u64 pp = p + u;
+                    ok[u] = pp < q1;
+                    mm[u] = 0;
+                    if (ok[u]) {
+                        while (k + 1 < nk && b_pre[k + 1] <= (u32)pp) k++;
+                        if (k != kcur) {
+                            kcur = k;
+                            du = mld<COH>(&dist[(u64)b_id[k] * WAVE + lane]);
+                        }
+                        const u64 eidx = (u64)b_loc[k] + ((u32)pp - b_pre[k]) - shift;
+                        const auto e = Codec<E>::get(edges, eidx, psh);
+                        vv[u] = e.end;
+                        ww[u] = edge_weight(e);
+                        mm[u] = b_msk[k];
+                        dd[u] = du;
+                    }
+                }
+    #pragma unroll
+                for (int u = 0; u < U; u++) {
+                    dv[u] = 0;
+                    if (ok[u] && ((mm[u] >> lane) & 1ull)) dv[u] = dist[(u64)vv[u] * WAVE + lane];
+                }
+                // all U atomics are issued before any result is used
+                u32 cc[U], ol[U];
+                bool tr[U];
+    #pragma unroll
+                for (int u = 0; u < U; u++) {
+                    tr[u] = false;
+                    cc[u] = 0;
+                    if (ok[u] && ((mm[u] >> lane) & 1ull)) {
+                        cc[u] = Rule<ALGO>::cand(dd[u], ww[u]);
+                        tr[u] = Rule<ALGO>::better(cc[u], dv[u]);
+                    }
+                    ol[u] = tr[u] ? Rule<ALGO>::atomic(&dist[(u64)vv[u] * WAVE + lane], cc[u]) : 0u;
+                }
+    #pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool imp = tr[u] && Rule<ALGO>::better(cc[u], ol[u]);
+                    const unsigned long long bal = __ballot(imp);
+                    if (bal) {
+                        if (lane == 0) atomicOr((unsigned long long *)&maskOut[vv[u]], bal);
+                        updates += (lane == 0) ? (u32)__popcll(bal) : 0u;
+                    }
                 }
             }
         }
