@@ -722,6 +722,11 @@ void Engine<E, G>::buildUnitsLearned()
         const auto i0 = std::lower_bound(bounds_.begin(), bounds_.end(), lo) - bounds_.begin();
         const auto i1 = std::lower_bound(bounds_.begin(), bounds_.end(), hi) - bounds_.begin();
         if ((size_t)i1 > P || bounds_[(size_t)i0] != lo || bounds_[(size_t)i1] != hi) continue;
+        // both buffers can hold overlapping slices (one unit moved into each
+        // in different iterations): the overlap is visited once
+        bool taken = false;
+        for (auto i = i0; i < i1; i++) taken = taken || done[(size_t)i];
+        if (taken) continue;
         ull activeEdges = 0;
         for (auto i = i0; i < i1; i++) activeEdges += h_rstats_[4 * i + 2];
         const ull minBytes = o_.residentMinBytes ? (ull)o_.residentMinBytes : kResidentMinBytes;
