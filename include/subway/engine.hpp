@@ -97,6 +97,8 @@ struct RunStats {
     ull pairWorkEdges = 0;
     ull pairWorkEntries = 0;
     ull pairUpdates = 0;
+    uint residentVisits = 0;   // learned schedule: units visited in place on a slice a buffer still held
+    uint residentOverlaps = 0; // ... resident slices skipped because an overlapping one was visited
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)
@@ -142,6 +144,12 @@ template <class E>
 RunStats runPageRank(GraphPR<E> &graph, const RunOptions &opt);
 
 const char *algo_name(int algo);
+
+// Destroys the compute/copy stream pair every engine of the process shares
+// (after draining it).  Call it only when no session or engine is alive, e.g.
+// at the end of a program; a later engine makes a new pair.  Also runs by
+// itself at process exit, before the HIP runtime's teardown.
+void release_shared_streams();
 
 }  // namespace subway
 

@@ -63,8 +63,19 @@ public:
     HostCsr &operator=(const HostCsr &) = delete;
     ~HostCsr();
     // allocates ptr, edges, degree; pinEdges = false keeps the edge list in
-    // ordinary memory (a graph that moves only in the split format)
-    void allocate(ull n, ull m, bool pinEdges = true);
+    // ordinary memory (a graph that moves only in the split format);
+    // withEdges = false allocates no 8-byte list at all: the graph's edges
+    // are then only the split arrays (allocateSplit), read through edgeAt()
+    void allocate(ull n, ull m, bool pinEdges = true, bool withEdges = true);
+    // edge i from whichever host layout the graph holds
+    E edgeAt(ull i) const
+    {
+        if (edges) return edges[i];
+        E e{};
+        e.end = splitDst[i];
+        if constexpr (sizeof(E) == 8) reinterpret_cast<uint32_t *>(&e)[1] = splitW[i];
+        return e;
+    }
     bool allocateSplit();          // pinned split arrays, filled by the caller; false if not allocated
     void finalize();               // degrees, maxDegree, device mapping
 };

@@ -116,6 +116,11 @@ typedef struct subway_stats {
     uint64_t pair_work_edges;
     uint64_t pair_work_entries;
     uint64_t pair_updates;
+    /* learned schedule: partition visits made in place on a slice a buffer
+     * still held, and resident slices skipped because another buffer's
+     * slice overlapping them was visited in place in the same iteration */
+    uint32_t resident_visits;
+    uint32_t resident_overlaps;
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -147,6 +152,14 @@ int32_t subway_device_count(void);
  * reference counterpart) */
 void subway_struct_sizes(uint32_t *options, uint32_t *stats, uint32_t *iter_record);
 
+/* Releases the library's process-wide device resources (the compute/copy
+ * stream pair the sessions share) while the HIP runtime is still whole.
+ * Call it after every session and graph has been freed, at the end of a
+ * program; a session created afterwards makes new streams.  Without it the
+ * same happens in an atexit handler.  No reference counterpart (the
+ * reference frees nothing, K12). */
+void subway_shutdown(void);
+
 /* -------- graphs (host CSR in pinned, device-mapped memory) */
 /* Graph<E>(path, isWeighted).ReadGraph(): shared/graph.cu:32-197.
  * .bcsr/.bwcsr (u32 header), .bcsr64/.bwcsr64 (u64 header), .el/.wel text.
@@ -167,6 +180,23 @@ int32_t subway_graph_from_csr(uint64_t n, uint64_t m, const uint64_t *node_point
 int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
                           int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax,
                           subway_graph **out);
+/* The same R-MAT graph as subway_graph_rmat, byte for byte, built on the
+ * GPU (records in generation order, a stable radix sort by source, node
+ * pointers from the key boundaries) and copied into pinned host memory: for
+ * scales where the host build takes minutes.  Needs about 9 bytes of device
+ * memory per generated edge record, twice (SUBWAY_ERR_NOMEM when the device
+ * lacks it) and weights that fit a byte.  A graph laid out for the split
+ * format (weighted = 2, or large weighted graphs that do not pack) keeps
+ * only the split arrays on the host, no 8-byte list.  No reference
+ * counterpart (synthetic input). */
+int32_t subway_graph_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                                 int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax,
+                                 subway_graph **out);
+/* Zero-copy view of a split-layout graph's host arrays: node_pointer[n+1],
+ * destinations ends[m] and byte weights[m] (pinned), valid while the graph
+ * lives.  SUBWAY_ERR_ARG when the graph holds no split arrays. */
+int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_pointer, const uint32_t **ends,
+                                const uint8_t **weights);
 /* tools/converter.cpp:88-93,157-162 file layouts; extension picks the variant */
 int32_t subway_graph_save(const subway_graph *g, const char *path);
 int32_t subway_graph_info(const subway_graph *g, uint64_t *n, uint64_t *m, int32_t *weighted,

@@ -43,10 +43,35 @@ static uint64_t gather_lists(vec *lists, int nt, uint32_t *out)
     return total;
 }
 
+/* Weights come from w (uint32) or, for graphs held as split arrays, w8
+ * (one byte per edge); exactly one of them is used. */
+static inline uint32_t weight_of(const uint32_t *w, const uint8_t *w8, uint64_t e)
+{
+    return w ? w[e] : (uint32_t)w8[e];
+}
+
+static uint64_t label_run(int algo, uint64_t n, const uint64_t *ptr, const uint32_t *adj, const uint32_t *w,
+                          const uint8_t *w8, uint32_t src, uint32_t *value, uint64_t *edges_out);
+
 /* Returns the number of rounds; *edges_out counts relaxed edges. */
 uint64_t omp_label_run(int algo, uint64_t n, const uint64_t *ptr,
                        const uint32_t *adj, const uint32_t *w, uint32_t src,
                        uint32_t *value, uint64_t *edges_out)
+{
+    return label_run(algo, n, ptr, adj, w, NULL, src, value, edges_out);
+}
+
+/* The same over byte weights (a graph held as split destination and weight
+ * arrays, e.g. the 8.59 G-edge R-MAT). */
+uint64_t omp_label_run_w8(int algo, uint64_t n, const uint64_t *ptr,
+                          const uint32_t *adj, const uint8_t *w8, uint32_t src,
+                          uint32_t *value, uint64_t *edges_out)
+{
+    return label_run(algo, n, ptr, adj, NULL, w8, src, value, edges_out);
+}
+
+static uint64_t label_run(int algo, uint64_t n, const uint64_t *ptr, const uint32_t *adj, const uint32_t *w,
+                          const uint8_t *w8, uint32_t src, uint32_t *value, uint64_t *edges_out)
 {
     uint32_t *front = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
     uint8_t *inNext = (uint8_t *)calloc(n ? n : 1, 1);
@@ -85,8 +110,11 @@ uint64_t omp_label_run(int algo, uint64_t n, const uint64_t *ptr,
                     uint32_t old = __atomic_load_n(&value[v], __ATOMIC_RELAXED);
                     if (algo == ALGO_BFS) cand = val + 1;
                     else if (algo == ALGO_CC) cand = val;
-                    else if (algo == ALGO_SSSP) cand = val + w[e];
-                    else cand = val < w[e] ? val : w[e];
+                    else if (algo == ALGO_SSSP) cand = val + weight_of(w, w8, e);
+                    else {
+                        const uint32_t we = weight_of(w, w8, e);
+                        cand = val < we ? val : we;
+                    }
                     edges++;
                     for (;;) {
                         better = (algo == ALGO_SSWP) ? (cand > old) : (cand < old);

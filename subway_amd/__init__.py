@@ -13,8 +13,10 @@ device edge budget that drives partitioning.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -68,6 +70,7 @@ class _Stats(ctypes.Structure):
         ("edge_bytes", ctypes.c_uint32), ("min_members", ctypes.c_uint32),
         ("lane_work", ctypes.c_uint64), ("lane_entries", ctypes.c_uint64),
         ("pair_work_edges", ctypes.c_uint64), ("pair_work_entries", ctypes.c_uint64), ("pair_updates", ctypes.c_uint64),
+        ("resident_visits", ctypes.c_uint32), ("resident_overlaps", ctypes.c_uint32),
     ]
 
 
@@ -98,6 +101,26 @@ def _ptr(a, t):
     return a.ctypes.data_as(t) if a is not None else None
 
 
+# Every live graph and session, so that the exit hook can free them while the
+# HIP runtime is still whole (interpreter teardown would otherwise run their
+# finalizers in arbitrary order, possibly after other modules are gone).
+_live_sessions = weakref.WeakSet()
+_live_graphs = weakref.WeakSet()
+
+
+@atexit.register
+def _shutdown():
+    """Close every session, then free every graph, then release the
+    library's shared streams (subway_shutdown).  Runs at interpreter exit,
+    before any finalizer of module teardown; the objects' own close()/__del__
+    are no-ops afterwards."""
+    for s in list(_live_sessions):
+        s.close()
+    for g in list(_live_graphs):
+        g.free()
+    lib.subway_shutdown()
+
+
 def device_count() -> int:
     """Number of visible HIP devices (0 when there is no GPU)."""
     return int(lib.subway_device_count())
@@ -118,12 +141,17 @@ class Graph:
                                     ctypes.byref(md), ctypes.byref(mv)))
         self.n, self.m, self.weighted = int(n.value), int(m.value), bool(w.value)
         self.max_degree, self.max_degree_vertex = int(md.value), int(mv.value)
+        _live_graphs.add(self)
 
-    def __del__(self):
+    def free(self):
+        """Releases the host CSR (sessions built on it keep their own reference)."""
         h = getattr(self, "_h", None)
         if h:
-            lib.subway_graph_free(h)
             self._h = None
+            lib.subway_graph_free(h)
+
+    def __del__(self):
+        self.free()
 
     # --- constructors
     @classmethod
@@ -167,15 +195,30 @@ class Graph:
     @classmethod
     def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
              seed: int = 42, weighted: bool = False, symmetrize: bool = False, wmin: int = 1,
-             wmax: int = 64, split_layout: bool = False) -> "Graph":
+             wmax: int = 64, split_layout: bool = False, build: str = "auto") -> "Graph":
         """split_layout: lay a weighted graph out as one that moves only in the
-        split format (pinned destination and weight arrays, the 8-byte list in
-        ordinary memory), as large weighted graphs that do not pack are by
-        themselves."""
+        split format (pinned destination and weight arrays), as large weighted
+        graphs that do not pack are by themselves.
+
+        build: "host" (threads of this process), "device" (the GPU builds the
+        same CSR, byte for byte, and copies it to pinned host memory; a
+        split-layout graph then keeps no 8-byte list) or "auto" (the device
+        for 2^24 generated edges or more when a GPU is visible and has the
+        memory, else the host)."""
+        if build not in ("auto", "host", "device"):
+            raise ValueError(f"build must be auto, host or device, not {build!r}")
         h = ctypes.c_void_p()
         w = (2 if split_layout else 1) if weighted else 0
-        check(lib.subway_graph_rmat(scale, edge_factor, a, b, c, seed, w, int(symmetrize),
-                                    wmin, wmax, ctypes.byref(h)))
+        args = (scale, edge_factor, a, b, c, seed, w, int(symmetrize), wmin, wmax, ctypes.byref(h))
+        use_device = build == "device" or (
+            build == "auto" and (edge_factor << scale) >= (1 << 24) and wmax <= 255 and device_count() > 0)
+        if use_device:
+            rc = lib.subway_graph_rmat_device(*args)
+            if rc == 0:
+                return cls(h)
+            if build == "device" or rc != -5:      # only a lack of device memory falls back to the host
+                check(rc)
+        check(lib.subway_graph_rmat(*args))
         return cls(h)
 
     # --- accessors
@@ -189,6 +232,22 @@ class Graph:
         w = np.empty(self.m, dtype=np.uint32) if self.weighted else None
         check(lib.subway_graph_export(self._h, _ptr(ptr, _u64p), _ptr(ends, _u32p), _ptr(w, _u32p)))
         return ptr, ends, w
+
+    def split_view(self):
+        """Zero-copy read-only views (node_pointer u64[n+1], ends u32[m],
+        weights u8[m]) of a split-layout graph's host arrays; each view keeps
+        the graph alive."""
+        p, e, w = _u64p(), _u32p(), _u8p()
+        check(lib.subway_graph_split_view(self._h, ctypes.byref(p), ctypes.byref(e), ctypes.byref(w)))
+        out = []
+        for ptr, count, ct, dt in ((p, self.n + 1, ctypes.c_uint64, np.uint64), (e, self.m, ctypes.c_uint32, np.uint32),
+                                   (w, self.m, ctypes.c_uint8, np.uint8)):
+            buf = (ct * max(count, 1)).from_address(ctypes.addressof(ptr.contents))
+            buf._graph = self
+            a = np.frombuffer(buf, dtype=dt)[:count]
+            a.flags.writeable = False
+            out.append(a)
+        return tuple(out)
 
     def out_degree(self):
         ptr, _, _ = self.csr()
@@ -226,6 +285,8 @@ class RunStats:
     pair_work_edges: int = 0
     pair_work_entries: int = 0
     pair_updates: int = 0
+    resident_visits: int = 0
+    resident_overlaps: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -305,6 +366,7 @@ class Session:
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        _live_sessions.add(self)
         self._rec = (_IterRecord * _TRACE_CAP)()
         if params is not None:
             self.set_policy(params)
@@ -316,8 +378,12 @@ class Session:
 
     def run(self, source: int = 0, copy: bool = True):
         """One query.  Returns (values, RunStats).  With copy=False the values
-        are a read-only view of the session's host result array, valid until
-        the next run (no copy of V values)."""
+        are a read-only view of the session's host result array (no copy of V
+        values): the view keeps the session alive, holds this run's values
+        until the session's next run, and is invalid once close() has been
+        called explicitly."""
+        if not getattr(self, "_h", None):
+            raise SubwayError(-1, "the session is closed")
         st = _Stats()
         dt = np.float32 if self.algo == "pr" else np.uint32
         if copy:
@@ -329,6 +395,7 @@ class Session:
         if not copy:
             ptr = lib.subway_session_values(self._h)
             buf = (ctypes.c_uint8 * (self.graph.n * 4)).from_address(ptr)
+            buf._session = self   # the array's base keeps the session (and the result array) alive
             vals = np.frombuffer(buf, dtype=dt)
             vals.flags.writeable = False
         return vals, _stats(st, self._rec)
@@ -359,6 +426,7 @@ class BatchSession:
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        _live_sessions.add(self)
         if params is not None:
             _set_policy(self._h, params)
 

@@ -43,11 +43,15 @@ _SIGS = {
     "subway_last_error": (ctypes.c_char_p, []),
     "subway_default_options": (None, [_vp]),
     "subway_device_count": (_i32, []),
+    "subway_shutdown": (None, []),
     "subway_struct_sizes": (None, [_u32p, _u32p, _u32p]),
     "subway_load": (_i32, [ctypes.c_char_p, _i32, ctypes.POINTER(_vp)]),
     "subway_graph_from_csr": (_i32, [_u64, _u64, _u64p, _u32p, _u32p, ctypes.POINTER(_vp)]),
     "subway_graph_rmat": (_i32, [_i32, _i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _i32,
                                  _i32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "subway_graph_rmat_device": (_i32, [_i32, _i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64,
+                                        _i32, _i32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "subway_graph_split_view": (_i32, [_vp, ctypes.POINTER(_u64p), ctypes.POINTER(_u32p), ctypes.POINTER(_u8p)]),
     "subway_graph_save": (_i32, [_vp, ctypes.c_char_p]),
     "subway_graph_info": (_i32, [_vp, _u64p, _u64p, _i32p, _u32p, _u32p]),
     "subway_graph_export": (_i32, [_vp, _u64p, _u32p, _u32p]),

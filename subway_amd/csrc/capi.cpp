@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "host_util.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "subway/engine.hpp"
@@ -19,6 +20,9 @@ namespace subway {
 template <class E>
 std::shared_ptr<HostCsr<E>> rmat_csr(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
                                      bool symmetrize, uint32_t wmin, uint32_t wmax, bool splitLayout);
+template <class E>
+std::shared_ptr<HostCsr<E>> rmat_csr_gpu(int scale, int edgeFactor, double a, double b, double c, uint64_t seed,
+                                         bool symmetrize, uint32_t wmin, uint32_t wmax, bool splitLayout);
 }
 
 using namespace subway;
@@ -87,6 +91,8 @@ void subway_struct_sizes(uint32_t *options, uint32_t *stats, uint32_t *iter_reco
     if (stats) *stats = (uint32_t)sizeof(subway_stats);
     if (iter_record) *iter_record = (uint32_t)sizeof(subway_iter_record);
 }
+
+void subway_shutdown(void) { release_shared_streams(); }
 
 int32_t subway_device_count(void)
 {
@@ -157,6 +163,35 @@ int32_t subway_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b
     });
 }
 
+int32_t subway_graph_rmat_device(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                                 int32_t weighted, int32_t symmetrize, uint32_t wmin, uint32_t wmax,
+                                 subway_graph **out)
+{
+    return guard([&] {
+        if (!out) throw Error(ERR_ARG, "null argument");
+        std::unique_ptr<subway_graph> g(new subway_graph);
+        g->weighted = weighted != 0;
+        if (g->weighted)
+            g->w = rmat_csr_gpu<OutEdgeWeighted>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax,
+                                                 weighted == 2);
+        else
+            g->u = rmat_csr_gpu<OutEdge>(scale, edge_factor, a, b, c, seed, symmetrize != 0, wmin, wmax, false);
+        *out = g.release();
+    });
+}
+
+int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_pointer, const uint32_t **ends,
+                                const uint8_t **weights)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        if (!g->weighted || !g->w->splitDst) throw Error(ERR_ARG, "the graph holds no split edge arrays");
+        if (node_pointer) *node_pointer = reinterpret_cast<const uint64_t *>(g->w->ptr);
+        if (ends) *ends = g->w->splitDst;
+        if (weights) *weights = g->w->splitW;
+    });
+}
+
 int32_t subway_graph_save(const subway_graph *g, const char *path)
 {
     return guard([&] {
@@ -196,15 +231,18 @@ int32_t subway_graph_export(const subway_graph *g, uint64_t *node_pointer, uint3
         const ull n = g->n(), m = g->m();
         const ull *ptr = g->weighted ? g->w->ptr : g->u->ptr;
         if (node_pointer) std::memcpy(node_pointer, ptr, (n + 1) * sizeof(ull));
-        for (ull e = 0; e < m; e++) {
-            if (g->weighted) {
-                if (ends) ends[e] = g->w->edges[e].end;
-                if (weights) weights[e] = g->w->edges[e].w8;
-            } else {
-                if (ends) ends[e] = g->u->edges[e].end;
-                if (weights) weights[e] = 1;
+        parallel_for(m, [&](uint64_t b, uint64_t e1, int) {
+            for (ull e = b; e < e1; e++) {
+                if (g->weighted) {
+                    const OutEdgeWeighted x = g->w->edgeAt(e);
+                    if (ends) ends[e] = x.end;
+                    if (weights) weights[e] = x.w8;
+                } else {
+                    if (ends) ends[e] = g->u->edges[e].end;
+                    if (weights) weights[e] = 1;
+                }
             }
-        }
+        });
     });
 }
 
@@ -300,6 +338,8 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->pair_work_edges = s.pairWorkEdges;
     stats->pair_work_entries = s.pairWorkEntries;
     stats->pair_updates = s.pairUpdates;
+    stats->resident_visits = s.residentVisits;
+    stats->resident_overlaps = s.residentOverlaps;
 }
 
 extern "C" {

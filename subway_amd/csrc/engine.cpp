@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <map>
@@ -127,19 +128,53 @@ struct Unit {
 // with two streams per session, a second session would put its copy and
 // compute streams on one queue and lose the transfer/compute overlap.
 // Runs of different sessions on one device therefore serialise on the pair.
+//
+// The pairs are destroyed explicitly, while the HIP runtime is still whole:
+// by release_shared_streams() (the C ABI's subway_shutdown, which the Python
+// package calls from its exit hook after closing every session and graph),
+// or else by an atexit handler registered when the first pair is made.  That
+// handler runs before the runtime's own teardown (atexit order is the
+// reverse of registration, and the runtime was initialised before the first
+// stream existed), so the runtime never has to reclaim live streams of this
+// library during process exit.
+static std::mutex g_streamMu;
+static std::map<int, std::pair<hipStream_t, hipStream_t>> *g_streamPairs = nullptr;
+
+static void destroy_stream_pairs()
+{
+    std::lock_guard<std::mutex> lock(g_streamMu);
+    if (!g_streamPairs) return;
+    for (auto &kv : *g_streamPairs) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) cur = kv.first;
+        (void)hipSetDevice(kv.first);
+        (void)hipStreamSynchronize(kv.second.first);
+        (void)hipStreamSynchronize(kv.second.second);
+        (void)hipStreamDestroy(kv.second.first);
+        (void)hipStreamDestroy(kv.second.second);
+        (void)hipSetDevice(cur);
+    }
+    delete g_streamPairs;
+    g_streamPairs = nullptr;
+}
+
 static void shared_streams(hipStream_t &compute, hipStream_t &copy)
 {
-    static std::mutex mu;
-    static std::map<int, std::pair<hipStream_t, hipStream_t>> pairs;
+    static bool atexitRegistered = false;
     int dev = 0;
     SUBWAY_CHECK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = pairs.find(dev);
-    if (it == pairs.end()) {
+    std::lock_guard<std::mutex> lock(g_streamMu);
+    if (!g_streamPairs) g_streamPairs = new std::map<int, std::pair<hipStream_t, hipStream_t>>();
+    auto it = g_streamPairs->find(dev);
+    if (it == g_streamPairs->end()) {
         hipStream_t a = nullptr, b = nullptr;
         SUBWAY_CHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
         SUBWAY_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
-        it = pairs.emplace(dev, std::make_pair(a, b)).first;
+        it = g_streamPairs->emplace(dev, std::make_pair(a, b)).first;
+        if (!atexitRegistered) {
+            std::atexit(destroy_stream_pairs);
+            atexitRegistered = true;
+        }
     }
     compute = it->second.first;
     copy = it->second.second;
@@ -726,7 +761,10 @@ void Engine<E, G>::buildUnitsLearned()
         // in different iterations): the overlap is visited once
         bool taken = false;
         for (auto i = i0; i < i1; i++) taken = taken || done[(size_t)i];
-        if (taken) continue;
+        if (taken) {
+            st_.residentOverlaps++;
+            continue;
+        }
         ull activeEdges = 0;
         for (auto i = i0; i < i1; i++) activeEdges += h_rstats_[4 * i + 2];
         const ull minBytes = o_.residentMinBytes ? (ull)o_.residentMinBytes : kResidentMinBytes;
@@ -745,6 +783,7 @@ void Engine<E, G>::buildUnitsLearned()
         u.idLo = lo;
         u.idHi = hi;
         hasRes[b] = true;
+        st_.residentVisits++;
     }
     // resident units first; a lone one in buffer 1 rotates the map by one
     if (hasRes[0]) units_.push_back(res[0]);
@@ -1515,6 +1554,8 @@ RunStats runPageRank(GraphPR<E> &graph, const RunOptions &opt)
     Engine<E, GraphPR<E>> eng(graph, o);
     return eng.run(0);
 }
+
+void release_shared_streams() { destroy_stream_pairs(); }
 
 template std::unique_ptr<Session> make_session<OutEdge>(std::shared_ptr<HostCsr<OutEdge>>, const RunOptions &);
 template std::unique_ptr<Session> make_session<OutEdgeWeighted>(std::shared_ptr<HostCsr<OutEdgeWeighted>>,

@@ -166,7 +166,7 @@ bool HostCsr<E>::allocateSplit()
 }
 
 template <class E>
-void HostCsr<E>::allocate(ull n_, ull m_, bool pinEdges)
+void HostCsr<E>::allocate(ull n_, ull m_, bool pinEdges, bool withEdges)
 {
     if (n_ == 0) throw Error(ERR_FORMAT, "graph has no vertices");
     if (n_ > 0xFFFFFFFFull) throw Error(ERR_FORMAT, "vertex ids are 32-bit; graph has too many vertices");
@@ -179,6 +179,11 @@ void HostCsr<E>::allocate(ull n_, ull m_, bool pinEdges)
         throw Error(ERR_NOMEM, "host allocation of the CSR offsets failed");
     }
     size_t bytes = (size_t)(m ? m : 1) * sizeof(E);
+    if (!withEdges) {
+        edges = nullptr;
+        pinned = false;
+        return;
+    }
     if (!pinEdges) {
         edges = static_cast<E *>(std::malloc(bytes));
         if (!edges) throw Error(ERR_NOMEM, "host allocation of the edge list failed");
@@ -469,7 +474,19 @@ void write_csr_file(const HostCsr<E> &csr, const std::string &path)
         for (ull i = 0; i < csr.n; i++) p32[i] = (uint32_t)csr.ptr[i];
         write_exact(f, p32.data(), csr.n * 4, path);
     }
-    write_exact(f, csr.edges, csr.m * sizeof(E), path);
+    if (csr.edges) {
+        write_exact(f, csr.edges, csr.m * sizeof(E), path);
+        return;
+    }
+    // split arrays only: the file layout is built in blocks
+    std::vector<E> blk((size_t)1 << 22);
+    for (ull e0 = 0; e0 < csr.m; e0 += blk.size()) {
+        const ull cnt = std::min<ull>(blk.size(), csr.m - e0);
+        parallel_for(cnt, [&](uint64_t b, uint64_t e, int) {
+            for (uint64_t i = b; i < e; i++) blk[i] = csr.edgeAt(e0 + i);
+        });
+        write_exact(f, blk.data(), cnt * sizeof(E), path);
+    }
 }
 
 template struct HostCsr<OutEdge>;

@@ -37,6 +37,8 @@ _lib.oracle_partition.argtypes = [_u64, _u64p, _u64, _u64, _u64p, _u64p, _u64p, 
 _lib.oracle_partition.restype = _u64
 _lib.omp_label_run.argtypes = [ctypes.c_int, _u64, _u64p, _u32p, _u32p, ctypes.c_uint32, _u32p, _u64p]
 _lib.omp_label_run.restype = _u64
+_lib.omp_label_run_w8.argtypes = [ctypes.c_int, _u64, _u64p, _u32p, _u8p, ctypes.c_uint32, _u32p, _u64p]
+_lib.omp_label_run_w8.restype = _u64
 _lib.omp_pr_run.argtypes = [_u64, _u64p, _u32p, ctypes.c_float, ctypes.c_float, _f32p, _f32p, _u64p]
 _lib.omp_pr_run.restype = _u64
 _lib.omp_num_threads.restype = ctypes.c_int
@@ -47,7 +49,7 @@ def _c(a, dtype):
 
 
 def _p(a, t):
-    return a.ctypes.data_as(t)
+    return a.ctypes.data_as(t) if a is not None else None
 
 
 def bfs(ptr, adj, src):
@@ -113,13 +115,22 @@ ALGO = {"bfs": 0, "cc": 1, "sssp": 2, "sswp": 3}
 
 
 def omp_label(algo, ptr, adj, w, src):
+    """OpenMP baseline of one label algorithm.  w: uint32 weights, uint8
+    weights (a graph held as split arrays) or None (BFS and CC read none)."""
     ptr, adj = _c(ptr, np.uint64), _c(adj, np.uint32)
-    w = _c(w if w is not None else np.ones(len(adj), np.uint32), np.uint32)
     n = len(ptr) - 1
     out = np.empty(n, np.uint32)
     edges = ctypes.c_uint64()
-    rounds = _lib.omp_label_run(ALGO[algo], n, _p(ptr, _u64p), _p(adj, _u32p), _p(w, _u32p), src,
-                                _p(out, _u32p), ctypes.byref(edges))
+    if w is None and algo in ("sssp", "sswp"):
+        raise ValueError(f"{algo} needs weights")
+    if w is not None and w.dtype == np.uint8:
+        w8 = _c(w, np.uint8)
+        rounds = _lib.omp_label_run_w8(ALGO[algo], n, _p(ptr, _u64p), _p(adj, _u32p), _p(w8, _u8p), src,
+                                       _p(out, _u32p), ctypes.byref(edges))
+    else:
+        w32 = None if w is None else _c(w, np.uint32)
+        rounds = _lib.omp_label_run(ALGO[algo], n, _p(ptr, _u64p), _p(adj, _u32p), _p(w32, _u32p), src,
+                                    _p(out, _u32p), ctypes.byref(edges))
     return out, int(rounds), int(edges.value)
 
 
