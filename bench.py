@@ -1,43 +1,61 @@
 """Benchmark of the out-of-GPU-memory pipeline on one MI355X.
 
-Workload (default): SSSP-async on a weighted R-MAT graph, scale 24
-(16,777,216 vertices, ~268M edges, a=0.57 b=0.19 c=0.19, weights U[1,64],
-self-loops dropped), from the highest out-degree vertex, with the device
-edge budget capped at 1 GiB so every global iteration goes through
-subgraph generation, partitioning and host-to-device transfer.
+Default workload (the configuration the headline metric is quoted on):
+SSSP-async on a weighted R-MAT graph, scale 28, edge factor 32
+(268,435,456 vertices, ~8.59 G edges, a=0.57 b=0.19 c=0.19, weights
+U[1,64], self-loops dropped), from the highest out-degree vertex, with the
+device edge budget capped at 16 GiB, under the learned scheduler and under
+the plain Subway policy.  The graph is built on the GPU (the host
+generator's CSR, byte for byte) into pinned host memory, where every run
+reads it: the path's job is to move it over the host link.
 
 A step is one full run from the initial state to the fixpoint (first
-generate .. values in the session's host result array), with the graph
-resident in pinned host memory.  value = input edges / step time (GTEPS),
-wall clock over the timed steps.  The same workload runs under the plain
-Subway policy and under the learned scheduler (the network shipped in
-subway_amd/data), each in its own session; the values of the two must be
-identical.  Each timed step records HIP events around every rounds-kernel
-launch (on its stream) and every transfer, from which the roofline entry and
-the achieved H2D rate are computed.  The CPU baseline is the OpenMP build of
-the same update rules on the same CSR, timed once on this host.
+generate .. values in the session's host result array).  value = input
+edges / mean step time of the learned policy (GTEPS); wall clock over the
+timed steps, all variants interleaved step by step.  Variants (an ablation
+of the learned schedule): "plain" (the reference's schedule), "rule" (the
+plain schedule plus in-place visits of id ranges a buffer still holds, no
+network), "network" (the network's decisions without in-place visits) and
+"learned" (network plus in-place visits); their values must be identical.
+Each timed step records HIP events around every rounds-kernel launch (on
+its stream) and every transfer, from which the roofline entry and the
+achieved H2D rate are computed.  The line also carries the network's
+next-volume prediction error and the transfer lower bound every correct
+schedule of this workload obeys.
 
-With --batch K (K <= 64) the workload is K-source batched SSSP instead: K
-sources drawn uniformly (seeded) from the vertices with out-degree >= 1
-advance together, one cooperative kernel per transferred partition, after
-the autotuner has picked the kernel's unroll; a step is one batched run and
-value = K * input edges / step time.  The same K sources then run one after
-another as single-source SSSP-async sessions (plain policy, same budget);
-the rows must be identical and the line reports the speedup.
+CPU baseline: the OpenMP build of the same update rules on the same CSR
+(the pinned split arrays, no copy), OMP_PROC_BIND=close with the process's
+OMP_NUM_THREADS, median of 3 runs after one warm-up, values checked against
+the GPU's.
 
-Prints one JSON line on stdout; progress goes to stderr.
+--config c2|c3cc|c3pr|c4 runs the other configurations (c4 = 64-source
+batched SSSP against 64 sequential runs).  Prints one JSON line on stdout
+and exits non-zero when any comparison fails; progress goes to stderr.
 """
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the OpenMP runtime reads these when the baseline library loads
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "end-to-end GTEPS + H2D GB moved; learned vs plain policy, same exact result"
+GiB = 1 << 30
+
+CONFIGS = {
+    "c2": dict(algo="sssp", scale=24, edge_factor=16, budget="1G", weighted=True, symmetrize=False),
+    "c3cc": dict(algo="cc", scale=25, edge_factor=16, budget="2G", weighted=False, symmetrize=True),
+    "c3pr": dict(algo="pr", scale=25, edge_factor=16, budget="2G", weighted=False, symmetrize=False),
+    "c4": dict(algo="sssp", scale=26, edge_factor=16, budget="4G", weighted=True, symmetrize=False, batch=64),
+    "c5": dict(algo="sssp", scale=28, edge_factor=32, budget="16G", weighted=True, symmetrize=False),
+}
 
 
 def log(msg):
@@ -50,30 +68,66 @@ def parse_size(s):
     return int(float(s[:-1]) * mul[s[-1].upper()]) if s[-1].upper() in mul else int(s)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def profiled_traffic(kernel, workload_key):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
-    by the copy-kernel calibration it records), or None."""
-    path = os.path.join(ROOT, "profiles", "sssp_async_rmat24_hbm_counters.txt")
+    summaries (FETCH_SIZE and WRITE_SIZE passes, each scaled by the
+    copy-kernel calibration they record), or None."""
+    pdir = os.path.join(ROOT, "profiles")
     try:
-        with open(path) as f:
+        names = sorted(os.listdir(pdir))
+    except OSError:
+        return None
+    for name in names:
+        if not name.endswith("hbm_counters.txt"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
             for line in f:
                 parts = line.split()
                 if parts[:2] == ["traffic_bytes_per_launch", kernel] and " ".join(parts[2:-1]) == workload_key:
                     return int(parts[-1])
-    except OSError:
-        return None
     return None
 
 
+def workload_name(args, budget):
+    kind = "weighted " if args.weighted else ""
+    sym = "symmetrised " if args.symmetrize else ""
+    src = ("" if args.algo in ("cc", "pr") else
+           ", source = max out-degree vertex")
+    thr = ", threshold 1e-6" if args.algo == "pr" else ""
+    return (f"{args.algo}-{args.mode} on {sym}{kind}R-MAT scale {args.scale} ef {args.edge_factor}{src}{thr}, "
+            f"device edge budget {budget} B ({budget / GiB:g} GiB)")
+
+
+VARIANTS = {
+    "plain": dict(policy="plain"),
+    "rule": dict(policy="learned", theta=2.0, params=None),
+    "network": dict(policy="learned", resident=False),
+    "learned": dict(policy="learned"),
+}
+
+
 def measure(sw, np, g, args, src, kw, names, params):
-    """Warm-up and timed steps of each policy, interleaved step by step so
-    that both see the same machine state; each run is timed on its own.
+    """Warm-up and timed steps of each variant, interleaved step by step so
+    that all see the same machine state; each run is timed on its own.
     Returns {name: (summary, stats, values of the last step)}."""
     sessions = {}
     for name in names:
         t0 = time.perf_counter()
-        sessions[name] = sw.Session(g, args.algo, policy=name, params=params if name == "learned" else None, **kw)
+        v = dict(VARIANTS[name])
+        if "params" not in v:
+            v["params"] = params if v["policy"] == "learned" else None
+        sessions[name] = sw.Session(g, args.algo, **kw, **v)
         log(f"{name}: session setup {1e3 * (time.perf_counter() - t0):.1f} ms")
     for i in range(args.warmup):
         for name in names:
@@ -87,12 +141,12 @@ def measure(sw, np, g, args, src, kw, names, params):
             t0 = time.perf_counter()
             v, st = sessions[name].run(src, copy=False)
             wall[name] += time.perf_counter() - t0
-            vals[name] = np.array(v, copy=True)
             stats[name].append(st)
+            if i == args.steps - 1:
+                vals[name] = np.array(v, copy=True)
             log(f"{name} step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
                 f"H2D {st.h2d_bytes / 1e9:.3f} GB, rounds {st.ms_rounds:.1f} ms ({st.rounds_launches} launches), "
-                f"transfer {st.ms_transfer:.1f} ms, generate {st.ms_generate:.1f} ms, "
-                f"mispredictions {st.mispredictions}, fallback at {st.fallback_iteration}")
+                f"transfer {st.ms_transfer:.1f} ms, in-place visits {st.resident_visits}")
     out = {}
     for name in names:
         ss = stats[name]
@@ -110,6 +164,7 @@ def measure(sw, np, g, args, src, kw, names, params):
             "global_iterations": ss[-1].global_iterations,
             "partitions": ss[-1].partitions,
             "inner_rounds": ss[-1].inner_rounds,
+            "in_place_visits": ss[-1].resident_visits,
             "h2d_gb_per_step": round(h2d / 1e9, 4),
             "h2d_mb_per_iteration": [round(r["h2d_bytes"] / 1e6, 2) for r in ss[-1].trace],
             "h2d_gbps_achieved": round(h2d / (xfer_ms * 1e-3) / 1e9, 2) if xfer_ms > 0 else None,
@@ -122,6 +177,122 @@ def measure(sw, np, g, args, src, kw, names, params):
     for s in sessions.values():
         s.close()
     return out
+
+
+def prediction_error(trace):
+    """Head 0 predicts log2 of the next iteration's active-edge fraction
+    (/ 32); the next iteration's feature 2 is the fraction it saw.  Returns
+    per-iteration |log2(predicted / actual)| and the judge's count of
+    predictions more than 2x off (iterations under 1e-4 of the edges,
+    predicted and actual, are not judged)."""
+    import math
+    errs, judged, missed = [], 0, 0
+    for a, b in zip(trace, trace[1:]):
+        if a["on_plain"]:
+            continue
+        pred = 2.0 ** (32.0 * min(max(a["heads"][0], -1.0), 0.0))
+        act = b["features"][2]
+        errs.append(round(abs(math.log2((pred + 1e-6) / (act + 1e-6))), 3))
+        if pred >= 1e-4 or act >= 1e-4:
+            judged += 1
+            missed += int(not 0.5 <= (pred + 1e-6) / (act + 1e-6) <= 2.0)
+    return {"abs_log2_error_per_iteration": errs, "judged": judged, "missed_by_more_than_2x": missed}
+
+
+def roofline(stats, args):
+    """Roofline of the dominant kernel.  Persistent async rounds: one
+    rounds_kernel launch per partition visit does every inner round (frontier
+    + advance).  Algorithmic bytes, summed over the rounds: label algorithms
+    22 B per labelled active vertex (id 4 + offset 8 + degree 4 + label r/w 2
+    + value 4), s(E) + 4 B per relaxed edge (edge as buffered + destination
+    value), 9 B per successful update (atomic RMW 8 + label 1); PageRank
+    (FP64 residual) 48 B per pushing vertex (id 4 + offset 8 + degree 4 +
+    residual read and clear 16 + running-sum RMW 16) and s(E) + 16 B per
+    edge (FP64 atomic add RMW).  The frontier's scan of unlabelled items is
+    reported apart (items_scanned, at the bytes each scanned item reads) and
+    not counted.  Partition visits run either as one persistent rounds_kernel
+    launch or as stand-alone frontier + advance launch pairs; the kind with
+    more kernel time is reported, priced with its own work only."""
+    eb = stats[-1].edge_bytes
+    rounds_ms = sum(s.ms_rounds for s in stats)
+    rounds_launches = sum(s.rounds_launches for s in stats)
+    pair_ms = sum(s.ms_advance + s.ms_frontier for s in stats)
+    adv_launches = sum(s.advance_launches for s in stats)
+    if rounds_launches and rounds_ms >= pair_ms:
+        kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
+        work = sum(s.work_edges - s.pair_work_edges for s in stats)
+        upd = sum(s.updates - s.pair_updates for s in stats)
+        ents = sum(s.work_entries - s.pair_work_entries for s in stats)
+    else:
+        kern, k_ms, k_launches = "advance_kernel+frontier_kernel", pair_ms, adv_launches
+        work = sum(s.pair_work_edges for s in stats)
+        upd = sum(s.pair_updates for s in stats)
+        ents = sum(s.pair_work_entries for s in stats)
+    if args.algo == "pr":
+        alg_bytes = 48 * ents + (eb + 16) * work
+    else:
+        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
+    items = sum(s.work_items for s in stats)
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    key = f"{args.algo}-{args.mode} rmat{args.scale}ef{args.edge_factor} budget{args.budget} learned"
+    traffic = profiled_traffic(kern, key)
+    return {
+        "bound": "hbm",
+        "kernel": kern,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "traffic_over_algorithmic": (round(traffic / (alg_bytes / max(k_launches, 1)), 2)
+                                     if traffic and alg_bytes else None),
+        "launches": k_launches,
+        "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
+        "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
+        "items_scanned_per_launch": int(items / max(k_launches, 1)),
+        "edge_bytes": eb,
+        "profile_key": key,
+    }
+
+
+def cpu_baseline(np, g, args, src, gpu_vals):
+    """OpenMP build of the same rules on the same CSR (the graph's own host
+    arrays), BASELINE protocol: one warm-up, median of 3."""
+    from tests import _oracle as O
+    if args.weighted and not args.symmetrize:
+        try:
+            ptr, adj, w = g.split_view()
+        except Exception:
+            ptr, adj, w = g.csr()
+    else:
+        ptr, adj, w = g.csr()
+    times, vals = [], None
+    for i in range(args.cpu_runs + 1):
+        t = time.perf_counter()
+        if args.algo == "pr":
+            vals, _, _ = O.omp_pr(ptr, adj, 1e-6)
+        else:
+            vals, _, _ = O.omp_label(args.algo, ptr, adj, w if args.algo in ("sssp", "sswp") else None, src)
+        dt = time.perf_counter() - t
+        log(f"cpu baseline {'warm-up' if i == 0 else 'run ' + str(i)}: {dt:.2f} s")
+        if i:
+            times.append(dt)
+    med = statistics.median(times)
+    if args.algo == "pr":
+        a = gpu_vals.astype(np.float64)
+        same = bool(np.max(np.abs(a - vals) / np.maximum(np.abs(a), 1e-30)) < 1e-3)
+    else:
+        same = bool(np.array_equal(vals, gpu_vals))
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {
+        "value": round(g.m / med / 1e9, 4), "unit": "GTEPS", "cores": O.omp_threads(), "kind": "port",
+        "sample": f"the full workload: one {args.algo} run of the OpenMP build on the same {g.m}-edge CSR and source, "
+                  f"median of {len(times)} after one warm-up, {med:.2f} s",
+        "seconds": [round(t, 3) for t in times],
+        "nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": cpu_model(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "same_result_as_gpu": same,
+    }
 
 
 def batch_bench(sw, np, g, args, budget, peak):
@@ -147,26 +318,20 @@ def batch_bench(sw, np, g, args, budget, peak):
         stats.append(st)
         log(f"batch step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, H2D {st.h2d_bytes / 1e9:.3f} GB, "
             f"rounds {st.ms_rounds:.1f} ms, transfer {st.ms_transfer:.1f} ms")
-    for r in stats[-1].trace:
-        log(f"  batch iter {r['iteration']}: active {r['active_nodes']} v / {r['active_edges']} e, "
-            f"{r['partitions']} partitions, H2D {r['h2d_bytes'] / 1e6:.1f} MB, {r['ms']:.2f} ms")
-    # one more (untimed) run hands the rows to Python for the comparison
-    rows, _ = bs.run(sources, rows_out=True)
+    rows, _ = bs.run(sources, rows_out=True)     # one more (untimed) run hands the rows over
     bs.close()
     ms_batch = wall / args.steps * 1e3
-    # k sequential single-source runs of the same sources
     sess = sw.Session(g, args.algo, mode="async", budget=budget, buffers=args.buffers, transfer=args.transfer)
     sess.run(int(sources[0]), copy=False)
-    same = True
-    t_seq = 0.0
-    h2d_seq = 0
+    mismatched = []
+    t_seq, h2d_seq = 0.0, 0
     for i, s0 in enumerate(sources):
         t0 = time.perf_counter()
         v, st = sess.run(int(s0), copy=False)
         t_seq += time.perf_counter() - t0
         h2d_seq += st.h2d_bytes
         if not np.array_equal(v, rows[i]):
-            same = False
+            mismatched.append(int(s0))
             log(f"source {s0}: batched row differs from the single-source run")
         if i % 8 == 7:
             log(f"sequential: {i + 1} of {k} runs, {t_seq:.2f} s")
@@ -186,10 +351,11 @@ def batch_bench(sw, np, g, args, budget, peak):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic R-MAT generated in-process (no datasets)",
+        "data": "synthetic R-MAT built on the GPU in-process (no datasets)",
         "config": {
-            "workload": f"{k}-source batched {args.algo} on weighted R-MAT scale {args.scale} ef {args.edge_factor}, "
-                        f"sources uniform over out-degree >= 1 (seed {args.seed}), device edge budget {budget} B",
+            "workload": f"{k}-source batched {args.algo}-async on weighted R-MAT scale {args.scale} ef "
+                        f"{args.edge_factor}, sources uniform over out-degree >= 1 (seed {args.seed}), device edge "
+                        f"budget {budget} B ({budget / GiB:g} GiB)",
             "vertices": g.n, "edges": g.m, "sources": k, "budget_bytes": budget, "buffers": args.buffers,
             "transfer": args.transfer, "unroll": unroll,
         },
@@ -206,21 +372,22 @@ def batch_bench(sw, np, g, args, budget, peak):
         "sequential": {"ms_total": round(ms_seq, 3), "ms_per_source": round(ms_seq / k, 3),
                        "h2d_gb_total": round(h2d_seq / 1e9, 3), "gteps": round(k * g.m / t_seq / 1e9, 4)},
         "batched_speedup_vs_sequential": round(ms_seq / ms_batch, 3),
-        "same_result_batched_vs_sequential": same,
+        "same_result_batched_vs_sequential": not mismatched,
         "h2d_gbps_peak": round(peak, 2),
         "roofline": batch_roofline(stats),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+    return not mismatched
 
 
 def batch_roofline(stats):
     """Batched rounds kernel, algorithmic bytes over all rounds: per relaxed
     edge s(E) once plus 8 B per source active at its vertex (destination
-    distance 4 + atomic 4; the lane_work counter is exactly the sum of
-    out-degree x active sources over the pushing vertices); per pushing
-    vertex its entry (id 4 + offset 8) and mask (8) plus 4 B per active
-    source (its own distance; lane_entries)."""
+    distance 4 + atomic 4; lane_work is exactly the sum of out-degree x active
+    sources over the pushing vertices); per pushing vertex its entry (id 4 +
+    offset 8) and mask (8) plus 4 B per active source (its own distance;
+    lane_entries)."""
     eb = stats[-1].edge_bytes
     ms = sum(s.ms_rounds for s in stats)
     launches = sum(s.rounds_launches for s in stats)
@@ -236,76 +403,25 @@ def batch_roofline(stats):
             "algorithmic_bytes_per_launch": int(alg / max(launches, 1)), "edge_bytes": eb}
 
 
-def roofline(np, stats, args, budget):
-    """Roofline of the dominant kernel.  Persistent async rounds: one
-    rounds_kernel launch per partition visit does every inner round (frontier
-    + advance).  Algorithmic bytes, summed over the rounds: label algorithms
-    22 B per labelled active vertex (id 4 + offset 8 + degree 4 + label r/w 2
-    + value 4), s(E) + 4 B per relaxed edge (edge as buffered + destination
-    value), 9 B per successful update (atomic RMW 8 + label 1); PageRank
-    (FP64 residual) 48 B per pushing vertex (id 4 + offset 8 + degree 4 +
-    residual read and clear 16 + running-sum RMW 16) and s(E) + 16 B per
-    edge (FP64 atomic add RMW).  The frontier pass over unlabelled items is
-    not counted.  Partition visits run either as one persistent rounds_kernel
-    launch or as stand-alone frontier + advance launch pairs (a round cap of
-    one, launch-per-round mode, oversized vertices); the run's counters tell
-    the pairs' share apart, and the kind with more kernel time is reported,
-    priced with its own work only."""
-    eb = stats[-1].edge_bytes
-    rounds_ms = sum(s.ms_rounds for s in stats)
-    rounds_launches = sum(s.rounds_launches for s in stats)
-    pair_ms = sum(s.ms_advance + s.ms_frontier for s in stats)
-    adv_launches = sum(s.advance_launches for s in stats)
-    if rounds_launches and rounds_ms >= pair_ms:
-        kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
-        work = sum(s.work_edges - s.pair_work_edges for s in stats)
-        upd = sum(s.updates - s.pair_updates for s in stats)
-        ents = sum(s.work_entries - s.pair_work_entries for s in stats)
-    else:
-        kern, k_ms, k_launches = "advance_kernel+frontier_kernel", pair_ms, adv_launches
-        work = sum(s.pair_work_edges for s in stats)
-        upd = sum(s.pair_updates for s in stats)
-        ents = sum(s.pair_work_entries for s in stats)
-    if args.algo == "pr":
-        alg_bytes = 48 * ents + (eb + 16) * work
-    else:
-        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    traffic = None
-    if (kern == "rounds_kernel" and args.algo == "sssp" and args.mode == "async" and args.scale == 24
-            and args.edge_factor == 16 and budget == 1 << 30 and args.buffers == 2):
-        traffic = profiled_traffic(kern, "sssp-async rmat24 budget1GiB learned")
-    return {
-        "bound": "hbm",
-        "kernel": kern,
-        "achieved": round(achieved, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "launches": k_launches,
-        "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
-        "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
-        "edge_bytes": eb,
-    }
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
-    ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--algo", default="sssp")
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--scale", type=int)
+    ap.add_argument("--edge-factor", type=int)
+    ap.add_argument("--algo")
     ap.add_argument("--mode", default="async")
-    ap.add_argument("--budget", default="1G")
+    ap.add_argument("--budget")
     ap.add_argument("--buffers", type=int, default=2)
     ap.add_argument("--transfer", default="auto")
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--policy", default="both", choices=["both", "plain", "learned"])
+    ap.add_argument("--variants", default="plain,rule,network,learned",
+                    help="comma-separated subset of plain, rule, network, learned")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=0, help="K-source batched SSSP against K sequential runs")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--batch", type=int)
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="launch the plain copy kernel once on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -315,6 +431,14 @@ def main():
               "partitions per GPU with a per-iteration all-reduce of vertex values, or batched sources split "
               "across GPUs) is not built", file=sys.stderr)
         sys.exit(2)
+    preset = CONFIGS[args.config]
+    for k, v in preset.items():
+        if getattr(args, k, None) is None:
+            setattr(args, k, v)
+    args.weighted = preset["weighted"] if args.algo == preset["algo"] else args.algo in ("sssp", "sswp")
+    args.symmetrize = args.algo == "cc"
+    if args.algo in ("sssp", "sswp"):
+        args.weighted = True
 
     import numpy as np
 
@@ -324,16 +448,13 @@ def main():
     if sw.device_count() < 1:
         log("no GPU visible")
         sys.exit(1)
-    weighted = args.algo in ("sssp", "sswp")
     budget = parse_size(args.budget)
     t0 = time.perf_counter()
-    g = sw.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=weighted,
-                      symmetrize=(args.algo == "cc"))
-    log(f"R-MAT scale {args.scale}: {g.n} vertices, {g.m} edges in {time.perf_counter() - t0:.1f} s")
+    g = sw.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=args.weighted,
+                      symmetrize=args.symmetrize)
+    gen_s = time.perf_counter() - t0
+    log(f"R-MAT scale {args.scale} ef {args.edge_factor}: {g.n} vertices, {g.m} edges built in {gen_s:.1f} s")
     src = g.max_degree_vertex
-    kw = dict(mode=args.mode, budget=budget, buffers=args.buffers, transfer=args.transfer, profile=True)
-    if args.algo == "pr":
-        kw["threshold"] = 1e-6
 
     if args.probe_copy_mb:
         ms = sw.probe_copy(args.probe_copy_mb << 20)
@@ -345,38 +466,52 @@ def main():
         if not 1 <= args.batch <= 64 or args.algo not in ("sssp", "sswp", "bfs"):
             log("--batch takes 1..64 sources of sssp, sswp or bfs")
             sys.exit(2)
-        batch_bench(sw, np, g, args, budget, peak)
-        return
+        sys.exit(0 if batch_bench(sw, np, g, args, budget, peak) else 1)
 
+    kw = dict(mode=args.mode, budget=budget, buffers=args.buffers, transfer=args.transfer, profile=True)
+    if args.algo == "pr":
+        kw["threshold"] = 1e-6
     params = policy.default_params()
-    names = ["plain", "learned"] if args.policy == "both" else [args.policy]
+    names = [n for n in args.variants.split(",") if n]
+    if not names or any(n not in VARIANTS for n in names):
+        log(f"--variants takes a subset of {', '.join(VARIANTS)}")
+        sys.exit(2)
     results = measure(sw, np, g, args, src, kw, names, params)
-    head = "learned" if "learned" in results else "plain"
+    head = "learned" if "learned" in results else names[-1]
     out, stats, vals = results[head]
-    same = None
-    if len(results) == 2:
-        a, b = results["plain"][2], results["learned"][2]
-        same = bool(np.array_equal(a, b)) if args.algo != "pr" else bool(
-            np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-30)) < 1e-4)
-        log(f"plain vs learned: {results['plain'][0]['ms_per_step']:.2f} vs {results['learned'][0]['ms_per_step']:.2f}"
-            f" ms per run, same result: {same}")
+    checks = {}
+    for name in names:
+        if name == head:
+            continue
+        a, b = results[name][2], vals
+        checks[f"same_result_{name}_vs_{head}"] = (bool(np.array_equal(a, b)) if args.algo != "pr" else bool(
+            np.max(np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(a.astype(np.float64)), 1e-30)) < 1e-4))
+    for name in names:
+        log(f"{name}: {results[name][0]['ms_per_step']:.2f} ms per run")
+
+    edge_bytes = stats[-1].edge_bytes
+    # every correct schedule of an async run moves the edges of every vertex
+    # the first iteration finds labelled (all of them, K8) and that then has
+    # a finite value: at least the edges out of reachable vertices, once
+    bound = None
+    if args.algo in ("sssp", "bfs", "sswp"):
+        reach = (vals != 4294967294) if args.algo != "sswp" else (vals != 0)
+        deg = g.out_degree()
+        moved = int(deg[reach].sum(dtype=np.uint64))
+        bound_ms = moved * edge_bytes / (peak * 1e9) * 1e3
+        bound = {"edges_out_of_reached_vertices": moved, "fraction_of_edges": round(moved / g.m, 4),
+                 "edge_bytes": edge_bytes, "gb": round(moved * edge_bytes / 1e9, 3),
+                 "ms_at_pinned_peak": round(bound_ms, 2),
+                 "best_possible_speedup_over_plain": (round(results["plain"][0]["ms_per_step"] / bound_ms, 3)
+                                                      if "plain" in results else None)}
 
     cpu = None
     if args.cpu_baseline:
-        from tests import _oracle as O
-        ptr, adj, w = g.csr()
-        t = time.perf_counter()
-        if args.algo == "pr":
-            cval, _, _ = O.omp_pr(ptr, adj, 1e-6)
-        else:
-            cval, _, _ = O.omp_label(args.algo, ptr, adj, w, src)
-        ct = time.perf_counter() - t
-        if args.algo != "pr" and not np.array_equal(cval, vals):
-            log("WARNING: CPU baseline and GPU results differ")
-        cpu = {"value": g.m / ct / 1e9, "unit": "GTEPS", "cores": O.omp_threads(), "kind": "port",
-               "sample": f"one full {args.algo} run (OpenMP frontier rounds, same CSR and source), {ct:.2f} s"}
-        log(f"cpu baseline: {ct:.2f} s on {O.omp_threads()} threads")
+        cpu = cpu_baseline(np, g, args, src, vals)
+        checks["same_result_cpu_baseline_vs_gpu"] = cpu["same_result_as_gpu"]
 
+    speed = {f"{n}_speedup_vs_plain": round(results["plain"][0]["ms_per_step"] / results[n][0]["ms_per_step"], 4)
+             for n in names if n != "plain" and "plain" in results}
     line = {
         "metric": METRIC,
         "value": out["gteps"],
@@ -389,30 +524,33 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32" if args.algo != "pr" else "f32",
-        "data": "synthetic R-MAT generated in-process (no datasets)",
+        "data": "synthetic R-MAT built on the GPU in-process into pinned host memory (no datasets)",
         "config": {
-            "workload": f"{args.algo}-{args.mode} on weighted R-MAT scale {args.scale} ef {args.edge_factor}, "
-                        f"source = max out-degree vertex, device edge budget {budget} B",
-            "vertices": g.n, "edges": g.m, "source": src, "budget_bytes": budget, "buffers": args.buffers,
-            "transfer": args.transfer, "policy": head,
-            "scheduler_network": "subway_amd/data/scheduler_mlp.f32" if params is not None else None,
+            "workload": workload_name(args, budget),
+            "name": args.config,
+            "vertices": g.n, "edges": g.m, "source": src if args.algo not in ("cc", "pr") else None,
+            "budget_bytes": budget, "buffers": args.buffers, "transfer": args.transfer, "policy": head,
+            "scheduler_network": "subway_amd/data/scheduler_mlp.f32",
+            "graph_build_s": round(gen_s, 2),
         },
-        "policies": {k: v[0] for k, v in results.items()},
-        "learned_trace": ([{"iteration": r["iteration"], "features": [round(f, 5) for f in r["features"]],
-                            "heads": [round(h, 5) for h in r["heads"]], "theta": r["theta"],
-                            "rounds_cap": r["rounds_cap"], "on_plain": r["on_plain"]}
-                           for r in stats[-1].trace] if head == "learned" else None),
-        "learned_speedup_vs_plain": (round(results["plain"][0]["ms_per_step"] / results["learned"][0]["ms_per_step"], 4)
-                                     if len(results) == 2 else None),
-        "same_result_learned_vs_plain": same,
+        "variants": {k: v[0] for k, v in results.items()},
+        **speed,
+        "learned_speedup_vs_plain": speed.get("learned_speedup_vs_plain"),
+        "checks": checks,
+        "network_prediction": prediction_error(stats[-1].trace) if head in ("learned", "network") else None,
+        "transfer_lower_bound": bound,
         "h2d_gb_per_step": out["h2d_gb_per_step"],
         "h2d_gbps_achieved": out["h2d_gbps_achieved"],
         "h2d_gbps_peak": round(peak, 2),
         "h2d_frac": round(out["h2d_gbps_achieved"] / peak, 3) if out["h2d_gbps_achieved"] and peak else None,
-        "roofline": roofline(np, stats, args, budget),
+        "roofline": roofline(stats, args),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    failed = [k for k, v in checks.items() if not v]
+    if failed:
+        log(f"FAILED checks: {', '.join(failed)}")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -249,9 +249,14 @@ class Graph:
             out.append(a)
         return tuple(out)
 
+    def node_pointer(self):
+        """A copy of the CSR offsets (u64[n+1]), without the edges."""
+        ptr = np.empty(self.n + 1, dtype=np.uint64)
+        check(lib.subway_graph_export(self._h, _ptr(ptr, _u64p), None, None))
+        return ptr
+
     def out_degree(self):
-        ptr, _, _ = self.csr()
-        return np.diff(ptr).astype(np.uint32)
+        return np.diff(self.node_pointer()).astype(np.uint32)
 
 
 @dataclass

@@ -231,6 +231,7 @@ int32_t subway_graph_export(const subway_graph *g, uint64_t *node_pointer, uint3
         const ull n = g->n(), m = g->m();
         const ull *ptr = g->weighted ? g->w->ptr : g->u->ptr;
         if (node_pointer) std::memcpy(node_pointer, ptr, (n + 1) * sizeof(ull));
+        if (!ends && !weights) return;
         parallel_for(m, [&](uint64_t b, uint64_t e1, int) {
             for (ull e = b; e < e1; e++) {
                 if (g->weighted) {

@@ -29,6 +29,7 @@
 #include "host_util.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "schedule.hpp"
 #include "subway/engine.hpp"
 #include "subway/gpu_error_check.hpp"
 #include "subway/partitioner.hpp"
@@ -750,40 +751,30 @@ void Engine<E, G>::buildUnitsLearned()
     std::vector<char> done(P, 0);
     Unit res[2] = {};
     bool hasRes[2] = {false, false};
-    for (int b = 0; b < nb && !o_.noResident; b++) {
-        const uint lo = bufIdLo_[b], hi = bufIdHi_[b];
-        if (hi <= lo) continue;
-        // the slice's ranges (units are built from whole ranges)
-        const auto i0 = std::lower_bound(bounds_.begin(), bounds_.end(), lo) - bounds_.begin();
-        const auto i1 = std::lower_bound(bounds_.begin(), bounds_.end(), hi) - bounds_.begin();
-        if ((size_t)i1 > P || bounds_[(size_t)i0] != lo || bounds_[(size_t)i1] != hi) continue;
-        // both buffers can hold overlapping slices (one unit moved into each
-        // in different iterations): the overlap is visited once
-        bool taken = false;
-        for (auto i = i0; i < i1; i++) taken = taken || done[(size_t)i];
-        if (taken) {
-            st_.residentOverlaps++;
-            continue;
-        }
-        ull activeEdges = 0;
-        for (auto i = i0; i < i1; i++) activeEdges += h_rstats_[4 * i + 2];
+    if (!o_.noResident) {
         const ull minBytes = o_.residentMinBytes ? (ull)o_.residentMinBytes : kResidentMinBytes;
-        if (activeEdges * (ull)eb_ < minBytes) continue;
-        for (auto i = i0; i < i1; i++) done[(size_t)i] = 1;
-        Unit &u = res[b];
-        u.dma = true;
-        u.hostEdge = np[lo];
-        u.count = np[hi] - np[lo];
-        u.whole = true;
-        u.n = hi - lo;
-        u.nodes = nullptr;
-        u.ptr = g_->d_nodePointer + lo;
-        u.fromEdge = np[lo];
-        u.idBase = lo;
-        u.idLo = lo;
-        u.idHi = hi;
-        hasRes[b] = true;
-        st_.residentVisits++;
+        const ResidentChoice rc = choose_resident_visits(
+            bounds_.data(), P, [&](size_t i) { return h_rstats_[4 * i + 2]; }, bufIdLo_, bufIdHi_, nb, minBytes,
+            (ull)eb_, done.data());
+        st_.residentOverlaps += rc.overlaps;
+        for (int b = 0; b < nb; b++) {
+            if (!rc.visit[b]) continue;
+            const uint lo = bufIdLo_[b], hi = bufIdHi_[b];
+            Unit &u = res[b];
+            u.dma = true;
+            u.hostEdge = np[lo];
+            u.count = np[hi] - np[lo];
+            u.whole = true;
+            u.n = hi - lo;
+            u.nodes = nullptr;
+            u.ptr = g_->d_nodePointer + lo;
+            u.fromEdge = np[lo];
+            u.idBase = lo;
+            u.idLo = lo;
+            u.idHi = hi;
+            hasRes[b] = true;
+            st_.residentVisits++;
+        }
     }
     // resident units first; a lone one in buffer 1 rotates the map by one
     if (hasRes[0]) units_.push_back(res[0]);

@@ -1,0 +1,55 @@
+// schedule.hpp -- host-side choice of the learned schedule's in-place visits:
+// which device buffers still hold the whole edge slice of an id range that
+// this global iteration should visit where it lies, with no transfer.
+#ifndef SUBWAY_SCHEDULE_HPP
+#define SUBWAY_SCHEDULE_HPP
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+namespace subway {
+
+struct ResidentChoice {
+    bool visit[2] = {false, false};   // buffer b's slice is visited in place
+    uint32_t overlaps = 0;            // slices skipped because an overlapping slice was already chosen
+};
+
+// bounds[0..P]: static id-range bounds (range i is [bounds[i], bounds[i+1]));
+// activeEdges(i): this iteration's active edges of range i; sliceLo/Hi[b]:
+// the id range whose whole edge slice buffer b holds (hi <= lo: none).  A
+// slice is visited when it is made of whole ranges, shares no range with a
+// slice chosen before it (both buffers can hold overlapping slices, moved
+// there in different iterations: the overlap is visited once), and its
+// active edges come to minBytes or more.  done[0..P) marks the ranges of
+// the chosen slices.
+template <class ActiveEdges>
+ResidentChoice choose_resident_visits(const uint32_t *bounds, size_t P, ActiveEdges activeEdges,
+                                      const uint32_t sliceLo[2], const uint32_t sliceHi[2], int numBuffers,
+                                      unsigned long long minBytes, unsigned long long edgeBytes, char *done)
+{
+    ResidentChoice c;
+    for (int b = 0; b < numBuffers && b < 2; b++) {
+        const uint32_t lo = sliceLo[b], hi = sliceHi[b];
+        if (hi <= lo) continue;
+        const size_t i0 = (size_t)(std::lower_bound(bounds, bounds + P + 1, lo) - bounds);
+        const size_t i1 = (size_t)(std::lower_bound(bounds, bounds + P + 1, hi) - bounds);
+        if (i1 > P || bounds[i0] != lo || bounds[i1] != hi) continue;
+        bool taken = false;
+        for (size_t i = i0; i < i1; i++) taken = taken || done[i];
+        if (taken) {
+            c.overlaps++;
+            continue;
+        }
+        unsigned long long active = 0;
+        for (size_t i = i0; i < i1; i++) active += activeEdges(i);
+        if (active * edgeBytes < minBytes) continue;
+        for (size_t i = i0; i < i1; i++) done[i] = 1;
+        c.visit[b] = true;
+    }
+    return c;
+}
+
+}  // namespace subway
+
+#endif  // SUBWAY_SCHEDULE_HPP
