@@ -246,6 +246,18 @@ int32_t subway_session_tune_batch(subway_session *s, const uint32_t *sources, in
  * active_nodes[n], active_nodes_pointer[n+1]. */
 int32_t subway_generate(uint64_t n, const uint8_t *label1, const uint8_t *label2, const uint32_t *out_degree,
                         uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active);
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  subway_generate over arrays the caller already holds in
+ * DEVICE memory (label1[n], label2[n], out_degree[n] in; active_nodes[n],
+ * active_nodes_pointer[n+1] out); the active count lands in *num_active
+ * (host).  Its scratch is a fixed file-scope device array of 4 MiB + 256 B
+ * per GPU, held until the process ends, so n is at most 2^32 - 1
+ * (SUBWAY_ERR_ARG beyond).  Runs on the null stream and returns after the
+ * count is on the host.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU race. */
+int32_t subway_generate_noalloc(uint64_t n, const uint8_t *label1, const uint8_t *label2, const uint32_t *out_degree,
+                                uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active);
 /* SubgraphGenerator::generate(GraphPR&, ..., acc): shared/subgraph_generator.cu:209-243 */
 int32_t subway_generate_pr(uint64_t n, const float *delta, float acc, const uint32_t *out_degree,
                            uint32_t *active_nodes, uint64_t *active_nodes_pointer, uint64_t *num_active);

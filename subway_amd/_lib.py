@@ -65,6 +65,7 @@ _SIGS = {
     "subway_session_run_batch": (_i32, [_vp, _u32p, _i32, _u32p, _vp, _vp, _u32]),
     "subway_session_tune_batch": (_i32, [_vp, _u32p, _i32, _i32p, ctypes.POINTER(ctypes.c_double)]),
     "subway_generate": (_i32, [_u64, _u8p, _u8p, _u32p, _u32p, _u64p, _u64p]),
+    "subway_generate_noalloc": (_i32, [_u64, _vp, _vp, _vp, _vp, _vp, _u64p]),
     "subway_generate_pr": (_i32, [_u64, _f32p, ctypes.c_float, _u32p, _u32p, _u64p, _u64p]),
     "subway_partition": (_i32, [_u64, _u64p, _u32p, _u64, _u64, _u64p, _u64p]),
     "subway_gather": (_i32, [_vp, _u64, _u32p, _vp, _u64p]),

@@ -37,6 +37,12 @@ struct subway_graph {
 
 static thread_local std::string g_last_error;
 
+namespace subway {
+// the calling thread's message for subway_last_error (entry points defined
+// in other translation units report through it)
+void set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
+}  // namespace subway
+
 template <class F>
 static int32_t guard(F &&f)
 {
