@@ -145,7 +145,9 @@ public:
     float *delta;
     uint *d_outDegree;
     float *d_value;
-    double *d_delta;   // FP64 residual: an FP32 sum at a hub rounds away shares below half its ulp
+    float *d_delta;    // the reference's FP32 residual (shared/graph.cuh:53); the engine keeps its own FP64
+                       // residual while it runs (an FP32 sum at a hub rounds away shares below half its ulp)
+                       // and leaves the final residual here, rounded to FP32
     ull *d_nodePointer;
     E *d_edgeList;
     std::string graphFormat;

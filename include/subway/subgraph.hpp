@@ -5,9 +5,12 @@
 // d_activeNodesPointer, written by SubgraphGenerator).  The device edge
 // budget is split into `numBuffers` partition buffers so the transfer of
 // partition i+1 overlaps the processing of partition i; max_partition_size
-// is the capacity of one buffer in edges.  Host mirrors (activeNodes,
-// activeNodesPointer) are filled by copyToHost() on request; activeEdgeList
-// is a pinned staging area used only by the host-gather transfer path.
+// is the capacity of one buffer in edges.  SubgraphGenerator::generate()
+// leaves the host mirrors valid as the reference does: activeNodes and
+// activeNodesPointer (numActiveNodes + 1 offsets) and activeEdgeList, the
+// active vertices' edges gathered into pinned memory sized to the active
+// edges (grown when a later iteration has more).  The engine's own loop
+// keeps everything on the device and never fills them.
 #ifndef SUBWAY_SUBGRAPH_HPP
 #define SUBWAY_SUBGRAPH_HPP
 
@@ -23,7 +26,7 @@ public:
 
     uint *activeNodes;          // host mirror (pinned), filled by copyToHost()
     ull *activeNodesPointer;    // host mirror (pinned), numActiveNodes+1 entries
-    E *activeEdgeList;          // pinned staging (host-gather path), lazily allocated
+    E *activeEdgeList;          // pinned: the active vertices' edges after generate()
 
     uint *d_activeNodes;
     ull *d_activeNodesPointer;
@@ -44,11 +47,13 @@ public:
     Subgraph &operator=(const Subgraph &) = delete;
 
     void copyToHost();              // active list + offsets -> host mirrors
-    E *stagingBuffer(int b);        // pinned staging buffer b (allocated on demand)
+    E *ensureActiveEdgeList(ull edges);   // activeEdgeList with room for `edges` edges
+    E *stagingBuffer(int b);        // pinned staging buffer b of the host-gather transfers (on demand)
 
 private:
     void init(ull budgetBytes, int nbuf);
     E *staging[2];
+    ull activeEdgeCap = 0;
 };
 
 #endif  // SUBWAY_SUBGRAPH_HPP

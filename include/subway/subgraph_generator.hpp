@@ -1,13 +1,14 @@
 // subgraph_generator.hpp -- SubgraphGenerator<E> (drop-in for
 // shared/subgraph_generator.cuh:13-30).
 //
-// generate() runs one single-pass gfx950 kernel: activeness flags
-// (label1 || label2, or delta > acc), a wave64 + workgroup scan of the flags
-// and of the masked degrees, a decoupled-lookback prefix across workgroups,
-// and the compaction into d_activeNodes / d_activeNodesPointer (64-bit
-// offsets).  Like the reference it returns with numActiveNodes set; unlike
-// the reference it moves no edge data -- the edges of each partition are
-// gathered when the partition is transferred.
+// generate() runs the gfx950 selection (activeness flags label1 || label2,
+// or delta > acc, reduced per 16384-vertex chunk, the chunk totals scanned,
+// the chunks compacted with wave64 + workgroup scans) into d_activeNodes /
+// d_activeNodesPointer (64-bit offsets), and -- as the reference does on
+// return (shared/subgraph_generator.cu:143-197) -- leaves numActiveNodes,
+// numActiveEdges, the host activeNodes / activeNodesPointer and the gathered
+// host activeEdgeList valid.  enqueue() is the device-only half the
+// engine's loop uses (no host copies; partitions are gathered when moved).
 #ifndef SUBWAY_SUBGRAPH_GENERATOR_HPP
 #define SUBWAY_SUBGRAPH_GENERATOR_HPP
 

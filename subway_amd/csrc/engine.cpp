@@ -251,7 +251,10 @@ public:
         SUBWAY_CHECK(hipHostMalloc((void **)&h_err_, 4 * sizeof(uint32_t), 0));
         std::memset(h_err_, 0, 4 * sizeof(uint32_t));
         fr_.spinLimit = o_.spinLimit ? o_.spinLimit : dev::SPIN_LIMIT_DEFAULT;
-        if constexpr (isPR) SUBWAY_CHECK(hipMalloc(&d_sum_, (size_t)g_->num_nodes * sizeof(double)));
+        if constexpr (isPR) {
+            SUBWAY_CHECK(hipMalloc(&d_sum_, (size_t)g_->num_nodes * sizeof(double)));
+            SUBWAY_CHECK(hipMalloc(&d_resid_, (size_t)g_->num_nodes * sizeof(double)));
+        }
         gen_->scanScratch().spinLimit = fr_.spinLimit;
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
@@ -350,6 +353,7 @@ public:
         if (h_ctl_) (void)hipHostFree(h_ctl_);
         if (h_err_) (void)hipHostFree(h_err_);
         if (d_sum_) (void)hipFree(d_sum_);
+        if (d_resid_) (void)hipFree(d_resid_);
         if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
@@ -384,6 +388,7 @@ private:
     ScanCtl *h_ctl_ = nullptr;
     uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
     double *d_sum_ = nullptr;     // PR: FP64 running sum of each vertex's absorbed residual
+    double *d_resid_ = nullptr;   // PR: FP64 residual (delta) while a run is under way
     bool broken_ = false;         // a device wait gave up: the device state is no longer trusted
     ull maxPart_ = 0;       // edges per device buffer (in the buffer's edge format)
     int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
@@ -486,7 +491,7 @@ void Engine<E, G>::initState()
 {
     const uint n = g_->num_nodes;
     if constexpr (isPR) {
-        launch_init_pr_state(n, d_sum_, g_->d_delta, o_.initPR, sc_);
+        launch_init_pr_state(n, d_sum_, d_resid_, o_.initPR, sc_);
     } else {
         launch_init_label_state(o_.algo, o_.mode == ASYNC, n, g_->d_value, (uint8_t *)g_->d_label1,
                                 (uint8_t *)g_->d_label2, src_, sc_);
@@ -542,7 +547,8 @@ void Engine<E, G>::generateAndPartition()
         launch_generate_masks(g_->num_nodes, d_m1_, d_m2_, g_->d_outDegree, sub_->d_activeNodes,
                               sub_->d_activeNodesPointer, gen_->scanScratch(), sc_);
     else if constexpr (isPR)
-        gen_->enqueue(*g_, *sub_, o_.acc);
+        launch_generate_pr(g_->num_nodes, d_resid_, (double)o_.acc, g_->d_outDegree, sub_->d_activeNodes,
+                           sub_->d_activeNodesPointer, gen_->scanScratch(), sc_);
     else
         gen_->enqueue(*g_, *sub_);
     if (o_.profile) pGen_.end(sc_);
@@ -1095,7 +1101,7 @@ void Engine<E, G>::frontier(int first, uint8_t *labA, uint8_t *labB, const Unit 
     a.labA = labA;
     a.labB = labB;
     if constexpr (isPR) {
-        a.delta = g_->d_delta;
+        a.delta = d_resid_;
         a.sum = d_sum_;
         a.acc = o_.acc;
     } else {
@@ -1120,7 +1126,7 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf)
     a.chunkMap = chunkMap_;
     a.edges = sub_->d_buffers[buf];
     if constexpr (isPR) {
-        a.delta = g_->d_delta;
+        a.delta = d_resid_;
     } else {
         a.value = g_->d_value;
         a.labelOut = labOut;
@@ -1163,7 +1169,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.l1 = l1;
         l.l2 = l2;
         if constexpr (isPR) {
-            l.delta = g_->d_delta;
+            l.delta = d_resid_;
             l.sum = d_sum_;
             l.acc = o_.acc;
         } else {
@@ -1382,7 +1388,7 @@ RunStats Engine<E, G>::run(uint source)
     }
     loop(t0);
     if constexpr (isPR) {
-        launch_finish_pr(n, g_->d_value, d_sum_, sc_);
+        launch_finish_pr(n, g_->d_value, d_sum_, g_->d_delta, d_resid_, sc_);
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, sc_));
     } else
         SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));

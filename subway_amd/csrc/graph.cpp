@@ -677,7 +677,7 @@ void GraphPR<E>::attach()
     delta = new float[num_nodes];
     SUBWAY_CHECK(hipMalloc(&d_outDegree, (size_t)num_nodes * sizeof(uint)));
     SUBWAY_CHECK(hipMalloc(&d_value, (size_t)num_nodes * sizeof(float)));
-    SUBWAY_CHECK(hipMalloc(&d_delta, (size_t)num_nodes * sizeof(double)));
+    SUBWAY_CHECK(hipMalloc(&d_delta, (size_t)num_nodes * sizeof(float)));
     SUBWAY_CHECK(hipMalloc(&d_nodePointer, ((size_t)num_nodes + 1) * sizeof(ull)));
     SUBWAY_CHECK(hipMemcpy(d_nodePointer, nodePointer, ((size_t)num_nodes + 1) * sizeof(ull),
                            hipMemcpyHostToDevice));

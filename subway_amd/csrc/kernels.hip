@@ -1837,10 +1837,12 @@ __global__ void init_pr_state_kernel(u64 n, double *sum, double *delta, double i
 }
 
 // FP32 result values from the FP64 running sums
-__global__ void finish_pr_kernel(u64 n, float *value, const double *sum)
+__global__ void finish_pr_kernel(u64 n, float *value, const double *sum, float *deltaOut, const double *resid)
 {
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
         value[i] = (float)sum[i];
+        if (deltaOut) deltaOut[i] = (float)resid[i];
+    }
 }
 
 // Plain streaming copy (16 B per lane and iteration); used to calibrate
@@ -2100,9 +2102,10 @@ void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStr
     SUBWAY_CHECK(hipGetLastError());
 }
 
-void launch_finish_pr(ull n, float *value, const double *sum, hipStream_t st)
+void launch_finish_pr(ull n, float *value, const double *sum, float *deltaOut, const double *resid, hipStream_t st)
 {
-    hipLaunchKernelGGL(finish_pr_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value, sum);
+    hipLaunchKernelGGL(finish_pr_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value, sum,
+                       deltaOut, resid);
     SUBWAY_CHECK(hipGetLastError());
 }
 

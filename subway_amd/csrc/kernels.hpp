@@ -261,8 +261,10 @@ void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_
 // PR: the running sum and the residual are FP64 (an FP32 residual at a hub
 // rounds away shares below half its ulp).
 void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStream_t st);
-// PR result: FP32 values from the running sums
-void launch_finish_pr(ull n, float *value, const double *sum, hipStream_t st);
+// PR result: FP32 values from the running sums, and (deltaOut non-null) the
+// FP32 copy of the final FP64 residual
+void launch_finish_pr(ull n, float *value, const double *sum, float *deltaOut, const double *resid,
+                      hipStream_t st);
 
 // Streaming copy of `bytes` (multiple of 16) for counter calibration.
 void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st);

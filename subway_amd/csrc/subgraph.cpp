@@ -75,6 +75,7 @@ Subgraph<E>::~Subgraph()
 {
     if (activeNodes) (void)hipHostFree(activeNodes);
     if (activeNodesPointer) (void)hipHostFree(activeNodesPointer);
+    if (activeEdgeList) (void)hipHostFree(activeEdgeList);
     for (int b = 0; b < 2; b++) {
         if (staging[b]) (void)hipHostFree(staging[b]);
         if (d_buffers[b]) (void)hipFree(d_buffers[b]);
@@ -102,8 +103,22 @@ E *Subgraph<E>::stagingBuffer(int b)
     if (b < 0 || b >= numBuffers) throw Error(subway::ERR_ARG, "staging buffer index out of range");
     if (!staging[b])
         SUBWAY_CHECK(hipHostMalloc((void **)&staging[b], (size_t)max_partition_size * sizeof(E), 0));
-    if (b == 0) activeEdgeList = staging[0];
     return staging[b];
+}
+
+template <class E>
+E *Subgraph<E>::ensureActiveEdgeList(ull edges)
+{
+    if (edges > activeEdgeCap || !activeEdgeList) {
+        if (activeEdgeList) SUBWAY_CHECK(hipHostFree(activeEdgeList));
+        activeEdgeList = nullptr;
+        // grow in steps of an eighth of the graph, never past it
+        ull cap = std::max<ull>(edges, std::min<ull>(num_edges, activeEdgeCap + num_edges / 8 + 1));
+        cap = std::max<ull>(cap, 1);
+        SUBWAY_CHECK(hipHostMalloc((void **)&activeEdgeList, (size_t)cap * sizeof(E), 0));
+        activeEdgeCap = cap;
+    }
+    return activeEdgeList;
 }
 
 // shared/subgraph.cu:44-45

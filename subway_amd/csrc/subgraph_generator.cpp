@@ -3,7 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstring>
 #include <string>
+
+#include "host_util.hpp"
 
 #include "kernels.hpp"
 #include "subway/gpu_error_check.hpp"
@@ -50,7 +53,7 @@ void SubgraphGenerator<E>::enqueue(Graph<E> &graph, Subgraph<E> &subgraph)
 template <class E>
 void SubgraphGenerator<E>::enqueue(GraphPR<E> &graph, Subgraph<E> &subgraph, float acc)
 {
-    launch_generate_pr(graph.num_nodes, graph.d_delta, (double)acc, graph.d_outDegree, subgraph.d_activeNodes,
+    launch_generate_pr(graph.num_nodes, graph.d_delta, acc, graph.d_outDegree, subgraph.d_activeNodes,
                        subgraph.d_activeNodesPointer, *scratch, stream);
 }
 
@@ -68,11 +71,36 @@ void SubgraphGenerator<E>::readback(Subgraph<E> &subgraph)
     subgraph.numActiveEdges = h_totals[1];
 }
 
+// The host side of the reference's generate (shared/subgraph_generator.cu:
+// 143-197): the active list and its offsets copied to the host mirrors, and
+// every active vertex's edges gathered from the host CSR into the pinned
+// activeEdgeList at its offset, by host threads over blocks of the list
+// (the whole list: no dropped tail).
+template <class E>
+static void gather_active_edges(const subway::HostCsr<E> &csr, Subgraph<E> &sg)
+{
+    sg.copyToHost();
+    E *out = sg.ensureActiveEdgeList(sg.numActiveEdges);
+    const uint *ids = sg.activeNodes;
+    const ull *at = sg.activeNodesPointer;
+    subway::parallel_for(sg.numActiveNodes, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t i = b; i < e; i++) {
+            const ull p0 = csr.ptr[ids[i]], d = csr.ptr[ids[i] + 1] - p0;
+            if (csr.edges) {
+                std::memcpy(out + at[i], csr.edges + p0, d * sizeof(E));
+            } else {
+                for (ull k = 0; k < d; k++) out[at[i] + k] = csr.edgeAt(p0 + k);
+            }
+        }
+    });
+}
+
 template <class E>
 void SubgraphGenerator<E>::generate(Graph<E> &graph, Subgraph<E> &subgraph)
 {
     enqueue(graph, subgraph);
     readback(subgraph);
+    gather_active_edges(*graph.csr, subgraph);
 }
 
 template <class E>
@@ -80,6 +108,7 @@ void SubgraphGenerator<E>::generate(GraphPR<E> &graph, Subgraph<E> &subgraph, fl
 {
     enqueue(graph, subgraph, acc);
     readback(subgraph);
+    gather_active_edges(*graph.csr, subgraph);
 }
 
 // shared/subgraph_generator.cu:287-288
