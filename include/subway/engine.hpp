@@ -55,6 +55,7 @@ struct RunOptions {
     int unpackedEdges = 0;  // 1: weighted edges stay 8 bytes; 0: one 32-bit word when ids and weights fit
     int noResident = 0;     // 1: move active edges every iteration even when the whole graph fits one buffer
     uint residentMinBytes = 0;   // learned schedule: least active edge bytes for an in-place visit (0: 8 MiB)
+    int pushOnly = 0;       // BFS on a resident graph: 1 = top-down steps only (no direction switching)
 };
 
 struct IterRecord {
@@ -99,6 +100,8 @@ struct RunStats {
     ull pairUpdates = 0;
     uint residentVisits = 0;   // learned schedule: units visited in place on a slice a buffer still held
     uint residentOverlaps = 0; // ... resident slices skipped because an overlapping one was visited
+    uint pullRounds = 0;       // direction-optimizing BFS: bottom-up steps
+    ull pullFound = 0;         // ... vertices they found
     uint mispredictions = 0;   // learned policy: iterations whose active edges missed the prediction by > 2x
     uint fallbackIteration = 0;   // learned policy: first iteration scheduled by the plain fallback (0: none)
     uint edgeBytes = 0;        // bytes per edge in transfers and device buffers (4 when packed)

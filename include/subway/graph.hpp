@@ -53,6 +53,9 @@ struct HostCsr {
     bool ensureSplit();             // builds the split copy once; false when it does not apply
     // After an unpinned read: split arrays, or the list moved into pinned memory
     void settleEdges(bool forceSplit);
+    // Takes a pinned packed copy built elsewhere (a reader that never holds
+    // the 8-byte list); ensurePacked() then returns `shift`.
+    void adoptPacked(uint32_t *pinnedWords, uint32_t shift, uint32_t base);
     HostCsr() = default;
     HostCsr(const HostCsr &) = delete;
 private:

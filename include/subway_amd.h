@@ -79,7 +79,9 @@ typedef struct subway_options {
                                ranges whose whole edge slice a buffer still holds in place */
     uint32_t resident_min_bytes;  /* learned schedule: in-place visits need at least this many bytes of active
                                edges in the resident ranges; 0 = 8 MiB */
-    int32_t reserved[2];
+    int32_t push_only;      /* BFS on a resident graph: 0 (default) switches between top-down push steps and
+                               bottom-up pull steps by frontier size (direction-optimizing BFS); 1 pushes only */
+    int32_t reserved[1];
 } subway_options;
 
 typedef struct subway_stats {
@@ -121,6 +123,11 @@ typedef struct subway_stats {
      * slice overlapping them was visited in place in the same iteration */
     uint32_t resident_visits;
     uint32_t resident_overlaps;
+    /* direction-optimizing BFS on a resident graph: bottom-up steps taken and
+     * the vertices they found */
+    uint32_t pull_rounds;
+    uint32_t pad0;
+    uint64_t pull_found;
 } subway_stats;
 
 typedef struct subway_iter_record {

@@ -51,7 +51,8 @@ class _Options(ctypes.Structure):
         ("rounds_mode", ctypes.c_int32), ("rounds_grid", ctypes.c_int32), ("max_rounds", ctypes.c_int32),
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
         ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
-        ("no_resident", ctypes.c_int32), ("resident_min_bytes", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 2),
+        ("no_resident", ctypes.c_int32), ("resident_min_bytes", ctypes.c_uint32), ("push_only", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -71,6 +72,7 @@ class _Stats(ctypes.Structure):
         ("lane_work", ctypes.c_uint64), ("lane_entries", ctypes.c_uint64),
         ("pair_work_edges", ctypes.c_uint64), ("pair_work_entries", ctypes.c_uint64), ("pair_updates", ctypes.c_uint64),
         ("resident_visits", ctypes.c_uint32), ("resident_overlaps", ctypes.c_uint32),
+        ("pull_rounds", ctypes.c_uint32), ("pad0", ctypes.c_uint32), ("pull_found", ctypes.c_uint64),
     ]
 
 
@@ -292,6 +294,9 @@ class RunStats:
     pair_updates: int = 0
     resident_visits: int = 0
     resident_overlaps: int = 0
+    pull_rounds: int = 0
+    pad0: int = 0
+    pull_found: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -310,7 +315,7 @@ def _set_policy(h, params):
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
              profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
-             join_window=0, resident=True, resident_min_bytes=0):
+             join_window=0, resident=True, resident_min_bytes=0, push_only=False):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -334,6 +339,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.join_window = join_window
     o.no_resident = 0 if resident else 1
     o.resident_min_bytes = resident_min_bytes
+    o.push_only = int(push_only)
     return o
 
 
@@ -359,15 +365,16 @@ class Session:
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
                  packed: bool = True, params=None, join_window: int = 0, resident: bool = True,
-                 resident_min_bytes: int = 0):
+                 resident_min_bytes: int = 0, push_only: bool = False):
         """``resident``: a graph whose edges fit one buffer stays there, and
         the learned schedule visits in place the ranges whose whole edge slice
         a buffer still holds, when they hold ``resident_min_bytes`` (0: 8 MiB)
-        of active edges or more."""
+        of active edges or more.  ``push_only``: BFS on a resident graph
+        takes top-down steps only (no bottom-up steps for dense frontiers)."""
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                      profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
-                     resident_min_bytes)
+                     resident_min_bytes, push_only)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -466,7 +473,7 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
         rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True,
-        params=None, join_window: int = 0, resident: bool = True):
+        params=None, join_window: int = 0, resident: bool = True, push_only: bool = False):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
@@ -476,19 +483,21 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     persistent launch waits for its workgroups before closing membership.
     ``resident``: a graph whose edges fit one device buffer moves once per
     run and stays there (False: the active edges move every iteration).
+    ``push_only``: BFS on a resident graph without bottom-up steps.
     """
     if params is not None:
         s = Session(graph, algo, mode=mode, budget=budget, buffers=buffers, threshold=threshold, init_pr=init_pr,
                     transfer=transfer, policy=policy, verbose=verbose, advance_grid=advance_grid, profile=profile,
                     rounds=rounds, rounds_grid=rounds_grid, max_rounds=max_rounds, theta=theta,
                     spin_limit=spin_limit, packed=packed, params=params, join_window=join_window,
-                    resident=resident)
+                    resident=resident, push_only=push_only)
         try:
             return s.run(source)
         finally:
             s.close()
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
-                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident)
+                 profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
+                 push_only=push_only)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

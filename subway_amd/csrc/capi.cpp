@@ -281,6 +281,7 @@ static RunOptions to_options(const subway_options *o)
     r.unpackedEdges = o->unpacked_edges;
     r.noResident = o->no_resident;
     r.residentMinBytes = o->resident_min_bytes;
+    r.pushOnly = o->push_only;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -347,6 +348,8 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->pair_updates = s.pairUpdates;
     stats->resident_visits = s.residentVisits;
     stats->resident_overlaps = s.residentOverlaps;
+    stats->pull_rounds = s.pullRounds;
+    stats->pull_found = s.pullFound;
 }
 
 extern "C" {

@@ -238,9 +238,12 @@ public:
         }
         // a graph that fits one buffer moves once per run and stays there
         resident_ = !o_.noResident && !o_.batch && g_->num_edges > 0 && g_->num_edges <= maxPart_;
-        if (!g_->csr->pinned && fmt_ != EDGE_SPLIT)
-            throw Error(ERR_ARG, "this graph's edge list is not pinned: it moves only in the split format "
-                                 "(unpacked_edges must be 0)");
+        if constexpr (!isPR) {
+            if (resident_ && o_.algo == BFS && !o_.pushOnly) buildInEdges();
+        }
+        if (!g_->csr->pinned && fmt_ != EDGE_SPLIT && fmt_ != EDGE_PACKED)
+            throw Error(ERR_ARG, "this graph keeps no pinned 8-byte edge list: it moves only in the split or "
+                                 "packed format (unpacked_edges must be 0)");
         alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
         ull maxEntries = std::min<ull>((ull)g_->num_nodes, maxPart_) + 1;
         SUBWAY_CHECK(hipMalloc(&entries_, maxEntries * sizeof(Entry)));
@@ -354,6 +357,8 @@ public:
         if (h_err_) (void)hipHostFree(h_err_);
         if (d_sum_) (void)hipFree(d_sum_);
         if (d_resid_) (void)hipFree(d_resid_);
+        if (d_inPtr_) (void)hipFree(d_inPtr_);
+        if (d_inAdj_) (void)hipFree(d_inAdj_);
         if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
@@ -389,6 +394,12 @@ private:
     uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
     double *d_sum_ = nullptr;     // PR: FP64 running sum of each vertex's absorbed residual
     double *d_resid_ = nullptr;   // PR: FP64 residual (delta) while a run is under way
+    // BFS on a resident graph: the in-edges (CSC) for bottom-up steps
+    ull *d_inPtr_ = nullptr;
+    uint32_t *d_inAdj_ = nullptr;
+    void buildInEdges();
+    void bfsDirectionOptimizing(const Unit &un, int buf);
+    bool pullCounted_ = false;
     bool broken_ = false;         // a device wait gave up: the device state is no longer trusted
     ull maxPart_ = 0;       // edges per device buffer (in the buffer's edge format)
     int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
@@ -1208,6 +1219,96 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
     }
 }
 
+// In-edges of the resident graph for the bottom-up steps of BFS: in-degree
+// counts, their prefix and a scatter of every edge's source, by host threads
+// with relaxed atomic counters (the order inside an in-list does not matter
+// to BFS), moved to the device once per session.
+template <class E, class G>
+void Engine<E, G>::buildInEdges()
+{
+    const ull n = g_->num_nodes, m = g_->num_edges;
+    const HostCsr<E> &c = *g_->csr;
+    std::vector<ull> inPtr(n + 1, 0);
+    std::vector<uint32_t> cnt(n, 0), adj(m);
+    parallel_for(n, [&](uint64_t b, uint64_t e, int) {
+        for (ull v = b; v < e; v++)
+            for (ull k = c.ptr[v]; k < c.ptr[v + 1]; k++) __atomic_fetch_add(&cnt[c.edgeAt(k).end], 1u, __ATOMIC_RELAXED);
+    });
+    for (ull v = 0; v < n; v++) inPtr[v + 1] = inPtr[v] + cnt[v];
+    parallel_for(n, [&](uint64_t b, uint64_t e, int) {
+        for (ull v = b; v < e; v++) cnt[v] = 0;
+    });
+    parallel_for(n, [&](uint64_t b, uint64_t e, int) {
+        for (ull v = b; v < e; v++)
+            for (ull k = c.ptr[v]; k < c.ptr[v + 1]; k++) {
+                const uint32_t d = c.edgeAt(k).end;
+                adj[inPtr[d] + __atomic_fetch_add(&cnt[d], 1u, __ATOMIC_RELAXED)] = (uint32_t)v;
+            }
+    });
+    SUBWAY_CHECK(hipMalloc(&d_inPtr_, (n + 1) * sizeof(ull)));
+    SUBWAY_CHECK(hipMalloc(&d_inAdj_, (m ? m : 1) * sizeof(uint32_t)));
+    SUBWAY_CHECK(hipMemcpy(d_inPtr_, inPtr.data(), (n + 1) * sizeof(ull), hipMemcpyHostToDevice));
+    if (m) SUBWAY_CHECK(hipMemcpy(d_inAdj_, adj.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+}
+
+// Direction-optimizing BFS on the resident graph (Beamer, Asanovic and
+// Patterson, SC 2012), level by level: each step's frontier pass selects the
+// labelled vertices (all at level `cur`), clears their labels and counts
+// them and their out-edges; a top-down step then pushes along those edges
+// (the advance), a bottom-up step has every unvisited vertex look for an
+// in-neighbour at level `cur` instead.  Bottom-up when the frontier's edges
+// exceed 1/14 of the edges of unvisited vertices, back to top-down when the
+// frontier falls below 1/24 of the vertices (the published thresholds).
+// Both steps label the next frontier; levels equal the push-only fixpoint.
+template <class E, class G>
+void Engine<E, G>::bfsDirectionOptimizing(const Unit &un, int buf)
+{
+    if constexpr (!isPR) {
+        constexpr ull kAlpha = 14, kBeta = 24;
+        const ull n = g_->num_nodes, m = g_->num_edges;
+        uint8_t *la = (uint8_t *)g_->d_label1, *lb = (uint8_t *)g_->d_label2;
+        ull explored = 0;
+        bool pull = false;
+        for (uint32_t cur = 0;; cur++) {
+            // level 0: the source's label may sit in either array
+            frontier(1, la, cur == 0 ? lb : nullptr, un);
+            SUBWAY_CHECK(hipMemcpyAsync(h_ctl_, fr_.ctl, sizeof(ScanCtl), hipMemcpyDeviceToHost, sc_));
+            SUBWAY_CHECK(hipStreamSynchronize(sc_));
+            if (h_ctl_->error) {
+                h_err_[0] = h_ctl_->error;
+                checkWaits();
+            }
+            const ull nf = h_ctl_->count, mf = h_ctl_->total;
+            if (nf == 0) break;
+            const ull unexplored = m > explored ? m - explored : 0;
+            if (!pull && mf * kAlpha > unexplored) pull = true;
+            else if (pull && nf * kBeta < n) pull = false;
+            explored += mf;
+            if (pull) {
+                launch_fill_u32(reinterpret_cast<uint32_t *>(&fr_.ctl->pull_found), 0u, 4, sc_);
+                if (o_.profile) pAdv_.begin(sc_);
+                launch_bfs_pull(n, d_inPtr_, d_inAdj_, g_->d_outDegree, g_->d_value, cur, lb, fr_.ctl, sc_);
+                if (o_.profile) pAdv_.end(sc_);
+                st_.pullRounds++;
+                pullCounted_ = true;
+            } else {
+                advance(lb, 0, ~0ull, 0, buf);
+            }
+            std::swap(la, lb);
+            if (pullCounted_) {
+                SUBWAY_CHECK(hipMemcpyAsync(&h_ctl_->pull_found, &fr_.ctl->pull_found, sizeof(ull),
+                                            hipMemcpyDeviceToHost, sc_));
+                SUBWAY_CHECK(hipStreamSynchronize(sc_));
+                st_.pullFound += h_ctl_->pull_found;
+                pullCounted_ = false;
+            }
+        }
+    } else {
+        (void)un;
+        (void)buf;
+    }
+}
+
 template <class E, class G>
 void Engine<E, G>::compute(size_t idx)
 {
@@ -1225,7 +1326,8 @@ void Engine<E, G>::compute(size_t idx)
     } else if (resident_) {
         // rounds to the fixpoint (or the round cap) on the resident graph, in
         // both modes: the label fixpoints are unique, PR stays within its bound
-        asyncRounds(u, b);
+        if (d_inPtr_) bfsDirectionOptimizing(u, b);
+        else asyncRounds(u, b);
     } else if (u.hub) {
         // one round over the oversized vertex, streamed chunk by chunk: the
         // frontier kernel runs once (clearing its label, fixing its value),

@@ -11,7 +11,12 @@
 // OutEdge is {u32 end}; OutEdgeWeighted is {u32 end, u32 w8}.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -285,6 +290,17 @@ void HostCsr<E>::settleEdges(bool forceSplit)
     pinned = true;
 }
 
+template <class E>
+void HostCsr<E>::adoptPacked(uint32_t *p, uint32_t shift, uint32_t base)
+{
+    std::lock_guard<std::mutex> lock(packMu_);
+    packTried_ = true;
+    packed = p;
+    packShift = shift;
+    packBase = base;
+    SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_packed, packed, 0));
+}
+
 // ------------------------------------------------------------ binary I/O
 static void read_exact(FILE *f, void *dst, size_t bytes, const std::string &path)
 {
@@ -317,51 +333,191 @@ static ull file_size(FILE *f)
     return s;
 }
 
+// Positioned reads of [off, off + bytes) of the file into dst by host
+// threads, 64 MiB blocks claimed in order (several reads in flight keep a
+// fast device or the page cache busy; one stream of fread does not).
+static void pread_all(int fd, void *dst, ull bytes, ull off, const std::string &path)
+{
+    constexpr ull BLK = 64ull << 20;
+    const ull nb = (bytes + BLK - 1) / BLK;
+    std::atomic<ull> next{0};
+    std::atomic<int> bad{0};
+    const int T = (int)std::max<ull>(1, std::min<ull>((ull)std::min(host_threads(), 16), nb));
+    auto work = [&] {
+        for (ull b; (b = next.fetch_add(1)) < nb && !bad.load();) {
+            char *p = (char *)dst + b * BLK;
+            ull left = std::min(BLK, bytes - b * BLK), at = off + b * BLK;
+            while (left) {
+                const ssize_t got = ::pread(fd, p, (size_t)std::min<ull>(left, 1ull << 30), (off_t)at);
+                if (got <= 0) {
+                    bad.store(got < 0 ? errno : -1);
+                    break;
+                }
+                p += got;
+                at += (ull)got;
+                left -= (ull)got;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (bad.load()) throw Error(ERR_IO, "read of the graph file failed: " + path);
+}
+
+// A weighted edge region read straight into the pinned split arrays, block
+// by block through per-thread staging: no 8-byte list is ever held.  Returns
+// false (nothing kept) when a weight does not fit a byte.
+static bool read_split(int fd, HostCsr<OutEdgeWeighted> &csr, ull off, const std::string &path, uint32_t &minW,
+                       uint32_t &maxW)
+{
+    if (!csr.allocateSplit()) throw Error(ERR_NOMEM, "pinned allocation of the split edge arrays failed");
+    constexpr ull BLK = 8ull << 20;   // edges per block (64 MiB)
+    const ull m = csr.m, nb = (m + BLK - 1) / BLK;
+    std::atomic<ull> next{0};
+    std::atomic<int> bad{0};
+    std::atomic<uint32_t> mx{0}, mn{0xFFFFFFFFu}, maxEnd{0};
+    const int T = (int)std::max<ull>(1, std::min<ull>((ull)std::min(host_threads(), 16), nb));
+    auto work = [&] {
+        std::vector<OutEdgeWeighted> buf((size_t)std::min(BLK, m ? m : 1));
+        uint32_t lmx = 0, lmn = 0xFFFFFFFFu, lend = 0;
+        for (ull b; (b = next.fetch_add(1)) < nb && !bad.load();) {
+            const ull e0 = b * BLK, cnt = std::min(BLK, m - e0);
+            char *p = (char *)buf.data();
+            ull left = cnt * sizeof(OutEdgeWeighted), at = off + e0 * sizeof(OutEdgeWeighted);
+            while (left) {
+                const ssize_t got = ::pread(fd, p, (size_t)left, (off_t)at);
+                if (got <= 0) {
+                    bad.store(1);
+                    break;
+                }
+                p += got;
+                at += (ull)got;
+                left -= (ull)got;
+            }
+            for (ull i = 0; i < cnt; i++) {
+                const OutEdgeWeighted x = buf[(size_t)i];
+                csr.splitDst[e0 + i] = x.end;
+                csr.splitW[e0 + i] = (uint8_t)x.w8;
+                lmx = std::max(lmx, x.w8);
+                lmn = std::min(lmn, x.w8);
+                lend = std::max(lend, x.end);
+            }
+        }
+        for (uint32_t c = mx.load(); lmx > c && !mx.compare_exchange_weak(c, lmx);) {}
+        for (uint32_t c = mn.load(); lmn < c && !mn.compare_exchange_weak(c, lmn);) {}
+        for (uint32_t c = maxEnd.load(); lend > c && !maxEnd.compare_exchange_weak(c, lend);) {}
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (bad.load()) throw Error(ERR_IO, "truncated graph file: " + path);
+    if (m && maxEnd.load() >= csr.n) throw Error(ERR_FORMAT, "edge endpoint out of range: " + path);
+    maxW = m ? mx.load() : 0;
+    minW = m ? mn.load() : 0;
+    return maxW <= 0xFFu;
+}
+
 template <class E>
 static void read_binary(HostCsr<E> &csr, const std::string &path, bool wide, bool splitLayout)
 {
-    FILE *f = fopen(path.c_str(), "rb");
-    if (!f) throw Error(ERR_IO, "cannot open graph file: " + path + " (" + strerror(errno) + ")");
-    struct Closer { FILE *f; ~Closer() { fclose(f); } } closer{f};
-    ull size = file_size(f);
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) throw Error(ERR_IO, "cannot open graph file: " + path + " (" + strerror(errno) + ")");
+    struct Closer { int fd; ~Closer() { ::close(fd); } } closer{fd};
+    struct stat sb;
+    if (::fstat(fd, &sb) != 0) throw Error(ERR_IO, "cannot stat graph file: " + path);
+    const ull size = (ull)sb.st_size;
     ull n, m;
-    size_t hdr = wide ? 16 : 8;
+    const size_t hdr = wide ? 16 : 8;
     if (size < hdr) throw Error(ERR_IO, "truncated graph file header: " + path);
     if (wide) {
         uint64_t h[2];
-        read_exact(f, h, 16, path);
+        pread_all(fd, h, 16, 0, path);
         n = h[0];
         m = h[1];
     } else {
         uint32_t h[2];
-        read_exact(f, h, 8, path);
+        pread_all(fd, h, 8, 0, path);
         n = h[0];
         m = h[1];
     }
-    ull ptrBytes = n * (wide ? 8ull : 4ull);
-    ull expect = hdr + ptrBytes + m * sizeof(E);
+    const ull ptrBytes = n * (wide ? 8ull : 4ull);
+    const ull expect = hdr + ptrBytes + m * sizeof(E);
     if (size != expect)
         throw Error(ERR_FORMAT, "graph file size " + std::to_string(size) + " does not match header (n=" +
                                     std::to_string(n) + ", m=" + std::to_string(m) + ", expected " +
                                     std::to_string(expect) + " bytes): " + path);
-    // weighted lists of 16 GiB or more are read into ordinary memory and
-    // settled afterwards, when their weights are known
-    const bool settle = is_weighted_edge<E>() && (splitLayout || m * sizeof(E) >= (1ull << 34));
-    csr.allocate(n, m, !settle);
+    const ull edgeOff = hdr + ptrBytes;
+    // weighted lists of 16 GiB or more (or a requested split layout) go
+    // straight into the pinned split arrays when their weights fit a byte
+    bool settled = false;
+    if constexpr (is_weighted_edge<E>()) {
+        int devices = 0;
+        const bool gpu = hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
+        if (!gpu) (void)hipGetLastError();
+        if (gpu && (splitLayout || m * sizeof(E) >= (1ull << 34))) {
+            csr.allocate(n, m, false, /*withEdges=*/false);
+            uint32_t minW = 0, maxW = 0;
+            if (read_split(fd, csr, edgeOff, path, minW, maxW)) {
+                settled = true;
+                // ids and weight span in one word: the transfers use a packed
+                // copy built from the split arrays (4 bytes per edge)
+                uint32_t db = 1;
+                while (db < 32 && (1ull << db) < n) db++;
+                uint32_t base = minW >> 27 ? 0 : minW;
+                if (!splitLayout && db < 32 && ((uint64_t)(maxW - base) >> (32 - db)) == 0 && m) {
+                    uint32_t *p = nullptr;
+                    if (hipHostMalloc((void **)&p, (size_t)m * sizeof(uint32_t),
+                                      hipHostMallocMapped | hipHostMallocPortable) == hipSuccess) {
+                        parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+                            for (uint64_t i = b; i < e; i++)
+                                p[i] = csr.splitDst[i] | (((uint32_t)csr.splitW[i] - base) << db);
+                        });
+                        csr.adoptPacked(p, db, base);
+                    } else {
+                        (void)hipGetLastError();
+                    }
+                }
+            } else if (splitLayout) {
+                throw Error(ERR_ARG, "the split layout needs weights that fit a byte");
+            } else {
+                // weights wider than a byte: the 8-byte list after all
+                if (csr.splitDst) (void)hipHostFree(csr.splitDst);
+                if (csr.splitW) (void)hipHostFree(csr.splitW);
+                csr.splitDst = csr.d_splitDst = nullptr;
+                csr.splitW = csr.d_splitW = nullptr;
+                delete[] csr.ptr;
+                delete[] csr.degree;
+                csr.ptr = nullptr;
+                csr.degree = nullptr;
+            }
+        }
+    }
+    if (!settled) {
+        if (!csr.ptr) csr.allocate(n, m, true);
+        pread_all(fd, csr.edges, m * sizeof(E), edgeOff, path);
+        std::atomic<uint32_t> maxEnd{0};
+        parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+            uint32_t x = 0;
+            for (uint64_t i = b; i < e; i++) x = std::max(x, csr.edges[i].end);
+            for (uint32_t c = maxEnd.load(); x > c && !maxEnd.compare_exchange_weak(c, x);) {}
+        });
+        if (m && maxEnd.load() >= n) throw Error(ERR_FORMAT, "edge endpoint out of range: " + path);
+    }
     csr.large = wide;
     if (wide) {
-        read_exact(f, csr.ptr, n * 8, path);
+        pread_all(fd, csr.ptr, n * 8, hdr, path);
     } else {
         std::vector<uint32_t> p32(n);
-        read_exact(f, p32.data(), n * 4, path);
-        for (ull i = 0; i < n; i++) csr.ptr[i] = p32[i];
+        pread_all(fd, p32.data(), n * 4, hdr, path);
+        parallel_for(n, [&](uint64_t b, uint64_t e, int) {
+            for (uint64_t i = b; i < e; i++) csr.ptr[i] = p32[i];
+        });
     }
-    read_exact(f, csr.edges, m * sizeof(E), path);
     for (ull i = 0; i < n; i++)
         if (csr.ptr[i] > m) throw Error(ERR_FORMAT, "node pointer beyond edge count: " + path);
-    for (ull e = 0; e < m; e++)
-        if (csr.edges[e].end >= n) throw Error(ERR_FORMAT, "edge endpoint out of range: " + path);
-    if (settle) csr.settleEdges(splitLayout);
     csr.finalize();
 }
 

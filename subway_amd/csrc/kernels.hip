@@ -1845,6 +1845,36 @@ __global__ void finish_pr_kernel(u64 n, float *value, const double *sum, float *
     }
 }
 
+// Bottom-up BFS step (direction-optimizing BFS, Beamer, Asanovic and
+// Patterson, SC 2012): every unvisited vertex scans its in-edges and stops at
+// the first in-neighbour on the frontier (level == cur); found vertices get
+// level cur + 1 and the label that makes them the next frontier.  One thread
+// per vertex, counts reduced per wave.
+__global__ __launch_bounds__(256) void bfs_pull_kernel(u64 n, const u64 *inPtr, const u32 *inAdj, const u32 *outDeg,
+                                                       u32 *level, u32 cur, uint8_t *labOut, ScanCtl *ctl)
+{
+    u64 found = 0, edges = 0;
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (u64)gridDim.x * blockDim.x) {
+        if (level[v] != 4294967294u) continue;   // unvisited: DIST_INFINITY
+        const u64 e1 = inPtr[v + 1];
+        for (u64 e = inPtr[v]; e < e1; e++) {
+            if (level[inAdj[e]] == cur) {
+                level[v] = cur + 1;
+                labOut[v] = 1;
+                found++;
+                edges += outDeg[v];
+                break;
+            }
+        }
+    }
+    found = wave_reduce(found);
+    edges = wave_reduce(edges);
+    if (lane_id() == 0 && found) {
+        atomicAdd((unsigned long long *)&ctl->pull_found, (unsigned long long)found);
+        atomicAdd((unsigned long long *)&ctl->pull_edges, (unsigned long long)edges);
+    }
+}
+
 // Plain streaming copy (16 B per lane and iteration); used to calibrate
 // hardware byte counters against a kernel of known traffic.
 __global__ void copy_probe_kernel(const uint4 *src, uint4 *dst, u64 n16)
@@ -2106,6 +2136,15 @@ void launch_finish_pr(ull n, float *value, const double *sum, float *deltaOut, c
 {
     hipLaunchKernelGGL(finish_pr_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (u64)n, value, sum,
                        deltaOut, resid);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_bfs_pull(ull n, const ull *inPtr, const uint32_t *inAdj, const uint32_t *outDeg, uint32_t *level,
+                     uint32_t cur, uint8_t *labOut, ScanCtl *ctl, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(bfs_pull_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, st, (u64)n,
+                       (const u64 *)inPtr, inAdj, outDeg, level, cur, labOut, ctl);
     SUBWAY_CHECK(hipGetLastError());
 }
 

@@ -266,6 +266,12 @@ void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStr
 void launch_finish_pr(ull n, float *value, const double *sum, float *deltaOut, const double *resid,
                       hipStream_t st);
 
+// Bottom-up BFS step over every vertex: unvisited vertices with an
+// in-neighbour at level `cur` get level cur + 1 and labOut set; the number
+// found and their out-edges are added to ctl->pull_found / pull_edges.
+void launch_bfs_pull(ull n, const ull *inPtr, const uint32_t *inAdj, const uint32_t *outDeg, uint32_t *level,
+                     uint32_t cur, uint8_t *labOut, ScanCtl *ctl, hipStream_t st);
+
 // Streaming copy of `bytes` (multiple of 16) for counter calibration.
 void launch_copy_probe(const void *src, void *dst, ull bytes, hipStream_t st);
 

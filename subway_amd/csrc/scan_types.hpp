@@ -44,6 +44,10 @@ struct ScanCtl {
     u64 pair_entries;
     u64 pair_work;
     u64 pair_updates;
+    // bottom-up BFS steps: vertices found and their out-edges (the next
+    // frontier), cleared by the host before each step
+    u64 pull_found;
+    u64 pull_edges;
 };
 static_assert(offsetof(ScanCtl, lane_entries) == offsetof(ScanCtl, lane_work) + 8,
               "lane_work, lane_entries form one two-word block");
