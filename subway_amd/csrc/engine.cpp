@@ -242,8 +242,8 @@ public:
             if (resident_ && o_.algo == BFS && !o_.pushOnly) buildInEdges();
         }
         if (!g_->csr->pinned && fmt_ != EDGE_SPLIT && fmt_ != EDGE_PACKED)
-            throw Error(ERR_ARG, "this graph keeps no pinned 8-byte edge list: it moves only in the split or "
-                                 "packed format (unpacked_edges must be 0)");
+            throw Error(ERR_ARG, "this graph keeps no pinned 8-byte edge list: it moves only in the split format "
+                                 "or as packed words (unpacked_edges must be 0)");
         alloc_scan_scratch(fr_, ((ull)g_->num_nodes + FR_TILE - 1) / FR_TILE + 1);
         ull maxEntries = std::min<ull>((ull)g_->num_nodes, maxPart_) + 1;
         SUBWAY_CHECK(hipMalloc(&entries_, maxEntries * sizeof(Entry)));

@@ -4,6 +4,14 @@ kernels.
 
     python -m subway_amd.train_scheduler --out subway_amd/data/scheduler_mlp.f32
 
+The training set is the small-graph sweep (R-MAT scales 16-23, budgets of a
+whole, a half and a sixth of the graph) plus the benchmark configurations
+at their stated sizes (BFS-sync scale 16 with no cap; SSSP-/SSWP-async scale
+24 under 1 GiB; CC-async on the symmetrised scale 25 and PR-async scale 25,
+2 GiB; SSSP-/BFS-async scale 26 under 4 GiB).  A sweep can be split over
+several processes (--set, --only i:j, --dump) and trained from the dumps
+(--train-from).
+
 For every training configuration (algorithm, R-MAT graph, device budget) the
 async run is repeated under a grid of fixed schedules -- whole-range density
 threshold theta x inner-round cap -- with the learned policy's unit builder
@@ -37,10 +45,22 @@ THETA_PREF = (0.8, 0.5, 2.0, 0.25)
 CAP_PREF = (64, 8, 1, 4096)
 
 
-def configs(quick: bool):
-    """(algo, scale, seed, budget divisor) of the training set: the device
-    budget holds 1/div of the graph's edges (two buffers)."""
+# the benchmark configurations: (algo, scale, edge factor, seed, budget bytes, symmetrise)
+NORTH = [("bfs", 16, 16, 42, 0, False), ("sssp", 24, 16, 42, 1 << 30, False), ("sswp", 24, 16, 42, 1 << 30, False),
+         ("cc", 25, 16, 42, 2 << 30, True), ("pr", 25, 16, 42, 2 << 30, False),
+         ("sssp", 26, 16, 42, 4 << 30, False), ("bfs", 26, 16, 42, 4 << 30, False)]
+
+
+def configs(quick: bool, which: str = "small"):
+    """Training configurations as (algo, scale, edge factor, seed, budget
+    divisor, budget bytes, symmetrise): the small sweep's device budget holds
+    1/div of the graph's edges (two buffers); the benchmark configurations
+    give their budget in bytes (0: no cap)."""
     out = []
+    if which in ("north", "all"):
+        out += [(a, s, ef, seed, 0, b, sym) for a, s, ef, seed, b, sym in NORTH]
+    if which == "north":
+        return out
     for algo in ("sssp", "sswp", "bfs", "pr"):
         if quick:
             scales = (15, 16)
@@ -53,21 +73,21 @@ def configs(quick: bool):
         for scale in scales:
             for seed in ((1,) if quick or scale >= 20 else (1, 2)):
                 for div in ((2, 6) if quick else (1, 2, 6)):
-                    out.append((algo, scale, seed, div))
+                    out.append((algo, scale, 16, seed, div, 0, False))
     return out
 
 
-def make_graph(algo, scale, seed):
-    return sw.Graph.rmat(scale, 16, seed=seed, weighted=algo in ("sssp", "sswp"))
+def make_graph(algo, scale, seed, edge_factor=16, symmetrize=False):
+    return sw.Graph.rmat(scale, edge_factor, seed=seed, weighted=algo in ("sssp", "sswp"), symmetrize=symmetrize)
 
 
-def run_config(g, algo, div, reps, log, seed=0):
+def run_config(g, algo, div, reps, log, seed=0, budget_bytes=None):
     """Median time and the trace of each fixed schedule, and of the plain
     policy.  All sessions exist at once and are timed round-robin in a fresh
     random order each round, so warm-up and clock drift touch every schedule
     alike."""
     edge_bytes = 4    # weighted R-MAT at these scales packs into one word; unweighted edges are 4 B
-    budget = max(64 << 10, (g.m * edge_bytes) // div)
+    budget = budget_bytes if budget_bytes is not None else max(64 << 10, (g.m * edge_bytes) // div)
     src = g.max_degree_vertex if algo != "pr" else 0
     kw = dict(mode="async", budget=budget)
     if algo == "pr":
@@ -99,7 +119,8 @@ def run_config(g, algo, div, reps, log, seed=0):
             traces[k] = st.trace
     for s in sess.values():
         s.close()
-    res = {k: (float(np.median(times[k])), traces[k]) for k in keys if k != "plain"}
+    res = {k: (float(np.median(times[k])), [{"features": list(r["features"])} for r in traces[k]])
+           for k in keys if k != "plain"}
     plain_ms = float(np.median(times["plain"]))
     # each decision labelled by its own best (the other decision at its best)
     t_theta = {th: min(res[(th, c)][0] for c in CAPS) for th in THETAS}
@@ -135,6 +156,10 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--save-data", default="", help="also write the training samples to this .npz")
+    ap.add_argument("--set", default="all", choices=["small", "north", "all"])
+    ap.add_argument("--only", default="", help="i:j, a slice of the configuration list")
+    ap.add_argument("--dump", default="", help="write the sweep's results (times, traces) to this JSON and stop")
+    ap.add_argument("--train-from", nargs="*", default=[], help="train from sweep dumps instead of sweeping")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -142,12 +167,36 @@ def main(argv=None):
 
     t0 = time.time()
     results, summary = [], []
-    for algo, scale, seed, div in configs(args.quick):
-        g = make_graph(algo, scale, seed)
-        res, best, plain_ms = run_config(g, algo, div, args.reps, log, seed=len(results))
-        results.append((res, best))
-        summary.append({"algo": algo, "scale": scale, "seed": seed, "budget_divisor": div, "plain_ms": plain_ms,
-                        "best": list(best), "ms": {f"{k[0]}/{k[1]}": v[0] for k, v in res.items()}})
+    if args.train_from:
+        for path in args.train_from:
+            with open(path) as f:
+                for c in json.load(f):
+                    res = {tuple(k): (v[0], v[1]) for k, v in c["results"]}
+                    results.append((res, tuple(c["summary"]["best"])))
+                    summary.append(c["summary"])
+    else:
+        todo = configs(args.quick, args.set)
+        if args.only:
+            i, j = (int(x) if x else None for x in args.only.split(":"))
+            todo = todo[i:j]
+        dumped = []
+        for algo, scale, ef, seed, div, budget, sym in todo:
+            g = make_graph(algo, scale, seed, ef, sym)
+            big = g.m >= (1 << 28)
+            res, best, plain_ms = run_config(g, algo, div, 1 if big else args.reps, log, seed=len(results),
+                                             budget_bytes=budget if not div else None)
+            g.free()
+            results.append((res, best))
+            s = {"algo": algo, "scale": scale, "edge_factor": ef, "seed": seed, "budget_divisor": div,
+                 "budget_bytes": budget, "symmetrize": sym, "plain_ms": plain_ms, "best": list(best),
+                 "ms": {f"{k[0]}/{k[1]}": v[0] for k, v in res.items()}}
+            summary.append(s)
+            dumped.append({"summary": s, "results": [[list(k), [v[0], v[1]]] for k, v in res.items()]})
+        if args.dump:
+            with open(args.dump, "w") as f:
+                json.dump(dumped, f)
+            log(f"wrote {len(dumped)} configurations to {args.dump} in {time.time() - t0:.0f} s")
+            return
     x, y, mask = dataset(results)
     log(f"{len(x)} samples from {len(results)} configurations in {time.time() - t0:.0f} s")
     if args.save_data:
