@@ -108,6 +108,8 @@ def run_config(g, algo, div, reps, log, seed=0, budget_bytes=None):
         rng.shuffle(order)
         for k in order:
             vals, st = sess[k].run(src)
+            if st.ms_total > 1000:
+                log(f"    round {r}, schedule {k}: {st.ms_total:.0f} ms")   # long runs report as they go
             if r < 2:
                 # warm-up rounds; the values must not depend on the schedule
                 if ref is None:
