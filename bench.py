@@ -7,7 +7,10 @@ U[1,64], self-loops dropped), from the highest out-degree vertex, with the
 device edge budget capped at 16 GiB, under the learned scheduler and under
 the plain Subway policy.  The graph is built on the GPU (the host
 generator's CSR, byte for byte) into pinned host memory, where every run
-reads it: the path's job is to move it over the host link.
+reads it: the path's job is to move it over the host link.  By default the
+graph is then renumbered on the GPU by descending in-degree (--order
+natural keeps the generator's ids), a one-time step reported as
+config.reorder_s; sources and values stay in the original numbering.
 
 A step is one full run from the initial state to the fixpoint (first
 generate .. values in the session's host result array).  value = input
@@ -255,10 +258,39 @@ def roofline(stats, args):
     }
 
 
+def original_degrees(g):
+    """Out-degrees indexed by original vertex id (a renumbered graph's
+    arrays are in its internal numbering)."""
+    deg = g.out_degree()
+    o = g.order()
+    return deg[o[0]] if o is not None else deg
+
+
+def to_original(np, g, algo, vals):
+    """Values of a run on a renumbered graph's own CSR (internal ids) in the
+    original numbering; CC labels (smallest internal id of each component)
+    become the smallest original id, as the library reports them."""
+    o = g.order()
+    if o is None:
+        return vals
+    perm, iperm = o
+    out = vals[perm]
+    if algo == "cc":
+        # component key = internal label; canonical label = smallest original id with that key
+        low = np.full(g.n, np.iinfo(np.uint32).max, np.uint32)
+        np.minimum.at(low, out, np.arange(g.n, dtype=np.uint32))
+        out = low[out]
+    return out
+
+
 def cpu_baseline(np, g, args, src, gpu_vals):
     """OpenMP build of the same rules on the same CSR (the graph's own host
-    arrays), BASELINE protocol: one warm-up, median of 3."""
+    arrays, so a renumbered graph's internal numbering, its results mapped
+    back), BASELINE protocol: one warm-up, median of 3."""
     from tests import _oracle as O
+    o = g.order()
+    if o is not None:
+        src = int(o[0][src])
     if args.weighted and not args.symmetrize:
         try:
             ptr, adj, w = g.split_view()
@@ -278,6 +310,7 @@ def cpu_baseline(np, g, args, src, gpu_vals):
         if i:
             times.append(dt)
     med = statistics.median(times)
+    vals = to_original(np, g, args.algo, vals)
     if args.algo == "pr":
         a = gpu_vals.astype(np.float64)
         same = bool(np.max(np.abs(a - vals) / np.maximum(np.abs(a), 1e-30)) < 1e-3)
@@ -298,7 +331,7 @@ def cpu_baseline(np, g, args, src, gpu_vals):
 def batch_bench(sw, np, g, args, budget, peak):
     """K-source batched SSSP against K sequential single-source runs."""
     k = args.batch
-    deg = g.out_degree()
+    deg = original_degrees(g)
     cand = np.flatnonzero(deg >= 1)
     rng = np.random.default_rng(args.seed)
     sources = np.sort(rng.choice(cand, size=k, replace=False)).astype(np.uint32)
@@ -357,7 +390,7 @@ def batch_bench(sw, np, g, args, budget, peak):
                         f"{args.edge_factor}, sources uniform over out-degree >= 1 (seed {args.seed}), device edge "
                         f"budget {budget} B ({budget / GiB:g} GiB)",
             "vertices": g.n, "edges": g.m, "sources": k, "budget_bytes": budget, "buffers": args.buffers,
-            "transfer": args.transfer, "unroll": unroll,
+            "transfer": args.transfer, "unroll": unroll, "vertex_order": args.order,
         },
         "batched": {"ms_per_run": round(ms_batch, 3), "global_iterations": stats[-1].global_iterations,
                     "inner_rounds": stats[-1].inner_rounds, "rounds_launches": stats[-1].rounds_launches,
@@ -422,6 +455,9 @@ def main():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--batch", type=int)
+    ap.add_argument("--order", default="in-degree", choices=["in-degree", "natural"],
+                    help="vertex numbering the runs use: renumbered by descending in-degree on the GPU after the "
+                         "build (results still in the original numbering), or the generator's own")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="launch the plain copy kernel once on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -455,6 +491,12 @@ def main():
     gen_s = time.perf_counter() - t0
     log(f"R-MAT scale {args.scale} ef {args.edge_factor}: {g.n} vertices, {g.m} edges built in {gen_s:.1f} s")
     src = g.max_degree_vertex
+    reorder_s = None
+    if args.order == "in-degree":
+        t0 = time.perf_counter()
+        g.reorder()
+        reorder_s = time.perf_counter() - t0
+        log(f"renumbered by descending in-degree in {reorder_s:.1f} s")
 
     if args.probe_copy_mb:
         ms = sw.probe_copy(args.probe_copy_mb << 20)
@@ -496,7 +538,7 @@ def main():
     bound = None
     if args.algo in ("sssp", "bfs", "sswp"):
         reach = (vals != 4294967294) if args.algo != "sswp" else (vals != 0)
-        deg = g.out_degree()
+        deg = original_degrees(g)
         moved = int(deg[reach].sum(dtype=np.uint64))
         bound_ms = moved * edge_bytes / (peak * 1e9) * 1e3
         bound = {"edges_out_of_reached_vertices": moved, "fraction_of_edges": round(moved / g.m, 4),
@@ -532,6 +574,8 @@ def main():
             "budget_bytes": budget, "buffers": args.buffers, "transfer": args.transfer, "policy": head,
             "scheduler_network": "subway_amd/data/scheduler_mlp.f32",
             "graph_build_s": round(gen_s, 2),
+            "vertex_order": args.order,
+            "reorder_s": round(reorder_s, 2) if reorder_s is not None else None,
         },
         "variants": {k: v[0] for k, v in results.items()},
         **speed,

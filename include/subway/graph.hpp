@@ -81,7 +81,21 @@ public:
     }
     bool allocateSplit();          // pinned split arrays, filled by the caller; false if not allocated
     void finalize();               // degrees, maxDegree, device mapping
+    // Vertex renumbering (reorder_by_in_degree): every array above then
+    // describes the renumbered graph; perm[v] is the internal id of original
+    // vertex v and iperm[p] the original id of internal vertex p (n entries
+    // each, ordinary host memory).  Null while the graph keeps its own ids.
+    // Runs take sources and return values in the original numbering.
+    uint32_t *perm = nullptr;
+    uint32_t *iperm = nullptr;
 };
+
+// Renumbers the vertices in descending in-degree order (ties by ascending
+// id), on the device, overwriting the graph's host arrays in place and
+// setting perm / iperm.  Needs about twice the edge bytes of device memory.
+// Call it before any run or session holds the graph.
+template <class E>
+void reorder_by_in_degree(HostCsr<E> &csr);
 
 // splitLayout: a .bwcsr graph that moves only in the split format (weights
 // must fit a byte); weighted lists of 16 GiB or more choose it by themselves

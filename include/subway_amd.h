@@ -40,6 +40,8 @@ extern "C" {
 
 /* shared/globals.hpp:27 */
 #define SUBWAY_DIST_INFINITY 4294967294u
+/* vertex orders of subway_graph_reorder */
+#define SUBWAY_ORDER_IN_DEGREE 1
 
 typedef struct subway_graph subway_graph;
 typedef struct subway_session subway_session;
@@ -204,6 +206,19 @@ int32_t subway_graph_rmat_device(int32_t scale, int32_t edge_factor, double a, d
  * lives.  SUBWAY_ERR_ARG when the graph holds no split arrays. */
 int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_pointer, const uint32_t **ends,
                                 const uint8_t **weights);
+/* Renumbers the vertices (order SUBWAY_ORDER_IN_DEGREE: descending in-degree,
+ * ties by ascending id) on the GPU, in place: the most-targeted vertices'
+ * values then share cache lines.  Runs, sessions and batches on the graph
+ * still take sources and return values in the original numbering; the
+ * accessors (export, split_view, save, subway_generate / partition / gather)
+ * see the renumbered graph.  Call it before any session is created on the
+ * graph (SUBWAY_ERR_ARG otherwise).  Needs about twice the graph's edge bytes
+ * of free device memory.  No counterpart in the reference. */
+int32_t subway_graph_reorder(subway_graph *g, int32_t order);
+/* perm[v] = internal id of original vertex v, iperm[p] = original id of
+ * internal vertex p (n entries each, valid while the graph lives); both NULL
+ * for a graph that keeps its own numbering. */
+int32_t subway_graph_order(const subway_graph *g, const uint32_t **perm, const uint32_t **iperm);
 /* tools/converter.cpp:88-93,157-162 file layouts; extension picks the variant */
 int32_t subway_graph_save(const subway_graph *g, const char *path);
 int32_t subway_graph_info(const subway_graph *g, uint64_t *n, uint64_t *m, int32_t *weighted,

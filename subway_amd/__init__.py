@@ -223,6 +223,40 @@ class Graph:
         check(lib.subway_graph_rmat(*args))
         return cls(h)
 
+    def reorder(self, order: str = "in-degree") -> None:
+        """Renumbers the vertices in descending in-degree order (ties by
+        ascending id) on the GPU, in place (subway_graph_reorder), so that
+        the most-targeted vertices' values share cache lines.  Runs, sessions
+        and batches keep taking sources and returning values in the original
+        numbering; csr(), split_view(), node_pointer(), out_degree() and
+        save() then describe the renumbered graph, and order() gives the
+        maps.  Call it before creating any session on the graph."""
+        if order != "in-degree":
+            raise ValueError(f"order must be 'in-degree', not {order!r}")
+        check(lib.subway_graph_reorder(self._h, 1))
+
+    def order(self):
+        """(perm, iperm): read-only u32[n] views with perm[v] the internal id
+        of original vertex v and iperm[p] the original id of internal vertex
+        p, or None for a graph that keeps its own numbering.  Each view keeps
+        the graph alive."""
+        p, ip = _u32p(), _u32p()
+        check(lib.subway_graph_order(self._h, ctypes.byref(p), ctypes.byref(ip)))
+        if not p:
+            return None
+        out = []
+        for ptr in (p, ip):
+            buf = (ctypes.c_uint32 * self.n).from_address(ctypes.addressof(ptr.contents))
+            buf._graph = self
+            a = np.frombuffer(buf, dtype=np.uint32)
+            a.flags.writeable = False
+            out.append(a)
+        return tuple(out)
+
+    @property
+    def reordered(self) -> bool:
+        return self.order() is not None
+
     # --- accessors
     def save(self, path: str) -> None:
         check(lib.subway_graph_save(self._h, path.encode()))

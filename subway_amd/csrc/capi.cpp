@@ -186,6 +186,30 @@ int32_t subway_graph_rmat_device(int32_t scale, int32_t edge_factor, double a, d
     });
 }
 
+int32_t subway_graph_reorder(subway_graph *g, int32_t order)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        if (order != SUBWAY_ORDER_IN_DEGREE) throw Error(ERR_ARG, "unknown vertex order");
+        // sessions keep their own references to the CSR (and device state built from it)
+        const long users = g->weighted ? g->w.use_count() : g->u.use_count();
+        if (users > 1) throw Error(ERR_ARG, "renumber a graph before any session is created on it");
+        if (g->weighted) reorder_by_in_degree(*g->w);
+        else reorder_by_in_degree(*g->u);
+    });
+}
+
+int32_t subway_graph_order(const subway_graph *g, const uint32_t **perm, const uint32_t **iperm)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        const uint32_t *p = g->weighted ? g->w->perm : g->u->perm;
+        const uint32_t *ip = g->weighted ? g->w->iperm : g->u->iperm;
+        if (perm) *perm = p;
+        if (iperm) *iperm = ip;
+    });
+}
+
 int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_pointer, const uint32_t **ends,
                                 const uint8_t **weights)
 {
@@ -219,11 +243,14 @@ int32_t subway_graph_info(const subway_graph *g, uint64_t *n, uint64_t *m, int32
         uint md = g->weighted ? g->w->maxDegree : g->u->maxDegree;
         if (max_degree) *max_degree = md;
         if (max_degree_vertex) {
-            uint64_t at = 0;
+            // the smallest original id among the vertices of largest out-degree
+            const uint32_t *ip = g->weighted ? g->w->iperm : g->u->iperm;
+            uint64_t at = ~0ull;
             for (uint64_t i = 0; i < g->n(); i++)
                 if (deg[i] == md) {
-                    at = i;
-                    break;
+                    const uint64_t v = ip ? ip[i] : i;
+                    if (v < at) at = v;
+                    if (!ip) break;
                 }
             *max_degree_vertex = (uint32_t)at;
         }

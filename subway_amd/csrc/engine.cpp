@@ -258,6 +258,18 @@ public:
             SUBWAY_CHECK(hipMalloc(&d_sum_, (size_t)g_->num_nodes * sizeof(double)));
             SUBWAY_CHECK(hipMalloc(&d_resid_, (size_t)g_->num_nodes * sizeof(double)));
         }
+        if (g_->csr->perm) {
+            const ull n = g_->num_nodes;
+            SUBWAY_CHECK(hipMalloc(&d_perm_, n * sizeof(uint32_t)));
+            SUBWAY_CHECK(hipMemcpy(d_perm_, g_->csr->perm, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+            if (o_.algo == CC) {
+                SUBWAY_CHECK(hipMalloc(&d_iperm_, n * sizeof(uint32_t)));
+                SUBWAY_CHECK(hipMemcpy(d_iperm_, g_->csr->iperm, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+            }
+            // batched rows (256 bytes a vertex) go out in chunks of 2^20 vertices
+            unpermRows_ = o_.batch ? std::min<ull>(n, 1ull << 20) : n;
+            SUBWAY_CHECK(hipMalloc(&d_unperm_, unpermRows_ * (o_.batch ? BATCH_LANES : 1) * sizeof(uint32_t)));
+        }
         gen_->scanScratch().spinLimit = fr_.spinLimit;
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
@@ -357,6 +369,9 @@ public:
         if (h_err_) (void)hipHostFree(h_err_);
         if (d_sum_) (void)hipFree(d_sum_);
         if (d_resid_) (void)hipFree(d_resid_);
+        if (d_perm_) (void)hipFree(d_perm_);
+        if (d_iperm_) (void)hipFree(d_iperm_);
+        if (d_unperm_) (void)hipFree(d_unperm_);
         if (d_inPtr_) (void)hipFree(d_inPtr_);
         if (d_inAdj_) (void)hipFree(d_inAdj_);
         if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
@@ -392,6 +407,11 @@ private:
     uint32_t *chunkMap_ = nullptr;
     ScanCtl *h_ctl_ = nullptr;
     uint32_t *h_err_ = nullptr;   // pinned: error words of the frontier, generator and rounds control blocks
+    // renumbered graph: perm (original -> internal), iperm (CC's initial
+    // values), and the result in the original order before its copy out
+    uint32_t *d_perm_ = nullptr, *d_iperm_ = nullptr, *d_unperm_ = nullptr;
+    ull unpermRows_ = 0;          // vertices d_unperm_ holds (batched rows are chunked)
+    void copyValuesOut(const void *dValues, void *hValues, uint32_t width);
     double *d_sum_ = nullptr;     // PR: FP64 running sum of each vertex's absorbed residual
     double *d_resid_ = nullptr;   // PR: FP64 residual (delta) while a run is under way
     // BFS on a resident graph: the in-edges (CSC) for bottom-up steps
@@ -505,7 +525,7 @@ void Engine<E, G>::initState()
         launch_init_pr_state(n, d_sum_, d_resid_, o_.initPR, sc_);
     } else {
         launch_init_label_state(o_.algo, o_.mode == ASYNC, n, g_->d_value, (uint8_t *)g_->d_label1,
-                                (uint8_t *)g_->d_label2, src_, sc_);
+                                (uint8_t *)g_->d_label2, src_, d_iperm_, sc_);
     }
 }
 
@@ -1460,13 +1480,35 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
     return st_;
 }
 
+// Device values (width u32 per vertex) into the host array, in the original
+// vertex order: a plain copy, or on a renumbered graph a gather through perm
+// into d_unperm_ first, in chunks of unpermRows_ vertices.
+template <class E, class G>
+void Engine<E, G>::copyValuesOut(const void *dValues, void *hValues, uint32_t width)
+{
+    const ull n = g_->num_nodes;
+    if (!d_perm_) {
+        SUBWAY_CHECK(hipMemcpyAsync(hValues, dValues, (size_t)n * width * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    sc_));
+        return;
+    }
+    for (ull v0 = 0; v0 < n; v0 += unpermRows_) {
+        const ull cnt = std::min<ull>(unpermRows_, n - v0);
+        if (v0) SUBWAY_CHECK(hipStreamSynchronize(sc_));   // d_unperm_ is free again
+        launch_unpermute((const uint32_t *)dValues, d_perm_, v0, cnt, width, d_unperm_, sc_);
+        SUBWAY_CHECK(hipMemcpyAsync((uint32_t *)hValues + v0 * width, d_unperm_, (size_t)cnt * width * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, sc_));
+    }
+}
+
 template <class E, class G>
 RunStats Engine<E, G>::run(uint source)
 {
     if (batchK_ || o_.batch) throw Error(ERR_ARG, "this engine runs batched queries (runBatch)");
     if (o_.algo != PR && o_.algo != CC && source >= g_->num_nodes)
         throw Error(ERR_ARG, "source vertex " + std::to_string(source) + " out of range");
-    src_ = source;
+    // sources are original ids; a renumbered graph runs on internal ones
+    src_ = (g_->csr->perm && source < g_->num_nodes) ? g_->csr->perm[source] : source;
     resetRun();
     initState();
     const uint n = g_->num_nodes;
@@ -1489,11 +1531,8 @@ RunStats Engine<E, G>::run(uint source)
         }
     }
     loop(t0);
-    if constexpr (isPR) {
-        launch_finish_pr(n, g_->d_value, d_sum_, g_->d_delta, d_resid_, sc_);
-        SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, sc_));
-    } else
-        SUBWAY_CHECK(hipMemcpyAsync(g_->value, g_->d_value, (size_t)n * sizeof(uint), hipMemcpyDeviceToHost, sc_));
+    if constexpr (isPR) launch_finish_pr(n, g_->d_value, d_sum_, g_->d_delta, d_resid_, sc_);
+    copyValuesOut(g_->d_value, g_->value, 1);
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     finishStats();
@@ -1514,7 +1553,7 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
         if (sources[i] >= n) throw Error(ERR_ARG, "batch source " + std::to_string(sources[i]) + " out of range");
     batchK_ = k;
     resetRun();
-    std::memcpy(h_sources_, sources, (size_t)k * sizeof(uint32_t));
+    for (int i = 0; i < k; i++) h_sources_[i] = g_->csr->perm ? g_->csr->perm[sources[i]] : sources[i];
     SUBWAY_CHECK(hipMemcpyAsync(d_sources_, h_sources_, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, sc_));
     launch_init_batch_state(o_.algo, n, d_dist_, d_m1_, d_m2_, 1, sc_);
     startStats();
@@ -1530,8 +1569,7 @@ RunStats Engine<E, G>::runBatch(const uint *sources, int k)
     launch_fill_u32((uint32_t *)d_m1_, 0, 2ull * n, sc_);
     launch_set_batch_sources(o_.algo, d_sources_, k, d_dist_, d_m1_, sc_);
     loop(t0);
-    SUBWAY_CHECK(hipMemcpyAsync(h_dist_, d_dist_, (size_t)n * BATCH_LANES * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                sc_));
+    copyValuesOut(d_dist_, h_dist_, BATCH_LANES);
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     batchK_ = 0;

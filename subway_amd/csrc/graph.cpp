@@ -56,6 +56,8 @@ HostCsr<E>::~HostCsr()
     if (packed) (void)hipHostFree(packed);
     if (splitDst) (void)hipHostFree(splitDst);
     if (splitW) (void)hipHostFree(splitW);
+    delete[] perm;
+    delete[] iperm;
 }
 
 template <class E>
@@ -302,18 +304,6 @@ void HostCsr<E>::adoptPacked(uint32_t *p, uint32_t shift, uint32_t base)
 }
 
 // ------------------------------------------------------------ binary I/O
-static void read_exact(FILE *f, void *dst, size_t bytes, const std::string &path)
-{
-    char *p = (char *)dst;
-    while (bytes) {
-        size_t chunk = std::min(bytes, (size_t)1 << 30);
-        size_t got = fread(p, 1, chunk, f);
-        if (got != chunk) throw Error(ERR_IO, "truncated graph file: " + path);
-        p += got;
-        bytes -= got;
-    }
-}
-
 static void write_exact(FILE *f, const void *src, size_t bytes, const std::string &path)
 {
     const char *p = (const char *)src;
@@ -323,14 +313,6 @@ static void write_exact(FILE *f, const void *src, size_t bytes, const std::strin
         p += chunk;
         bytes -= chunk;
     }
-}
-
-static ull file_size(FILE *f)
-{
-    fseeko(f, 0, SEEK_END);
-    ull s = (ull)ftello(f);
-    fseeko(f, 0, SEEK_SET);
-    return s;
 }
 
 // Positioned reads of [off, off + bytes) of the file into dst by host

@@ -1808,12 +1808,15 @@ __global__ void fill_u32_kernel(u32 *p, u32 v, u64 n)
 
 // Initial vertex state of the label algorithms, written on the device (the
 // reference fills host arrays and copies them, e.g. subway/bfs-sync.cu:28-42).
+// On a renumbered graph CC starts from the original ids (ids[i], the
+// original id of vertex i), so each component still ends at its smallest
+// original id.
 __global__ void init_label_state_kernel(int algo, int async, u64 n, u32 *value, uint8_t *l1, uint8_t *l2,
-                                        u32 source)
+                                        u32 source, const u32 *ids)
 {
     const u32 INF = 4294967294u;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
-        u32 v = (algo == ALGO_CC) ? (u32)i : (algo == ALGO_SSWP ? 0u : INF);
+        u32 v = (algo == ALGO_CC) ? (ids ? ids[i] : (u32)i) : (algo == ALGO_SSWP ? 0u : INF);
         uint8_t a = async ? 1 : 0;
         uint8_t b = (!async && algo == ALGO_CC) ? 1 : 0;
         if (algo != ALGO_CC && i == source) {
@@ -1833,6 +1836,18 @@ __global__ void init_pr_state_kernel(u64 n, double *sum, double *delta, double i
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
         sum[i] = 0.0;
         delta[i] = init;
+    }
+}
+
+// Values of a renumbered graph in the original order: out[v] = in[perm[v]],
+// rows of `width` u32 (1 for one value per vertex, 64 for batched rows),
+// vertices [v0, v0 + count) into out[0 ..).
+__global__ void unpermute_kernel(const u32 *in, const u32 *perm, u64 v0, u64 count, u32 width, u32 *out)
+{
+    const u64 total = count * width;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
+        const u64 v = i / width, c = i - v * width;
+        out[i] = in[(u64)perm[v0 + v] * width + c];
     }
 }
 
@@ -2118,10 +2133,19 @@ void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 }
 
 void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
-                             uint32_t source, hipStream_t st)
+                             uint32_t source, const uint32_t *ids, hipStream_t st)
 {
     hipLaunchKernelGGL(init_label_state_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, algo, async,
-                       (u64)n, value, l1, l2, source);
+                       (u64)n, value, l1, l2, source, ids);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull count, uint32_t width, uint32_t *out,
+                      hipStream_t st)
+{
+    if (count == 0) return;
+    hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(count * width, 256, 16384)), dim3(256), 0, st, in, perm,
+                       (u64)v0, (u64)count, width, out);
     SUBWAY_CHECK(hipGetLastError());
 }
 

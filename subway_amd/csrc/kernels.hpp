@@ -256,8 +256,14 @@ void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packCode, const
 void launch_rounds_batch(int algo, int edgeFmt, int unroll, const BatchLaunch &l, int grid, hipStream_t st);
 
 // Initial vertex state (values and both label arrays / PR value and delta).
+// ids (may be null): CC's initial value of vertex i, the original id of a
+// renumbered graph's vertex i.
 void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_t *l1, uint8_t *l2,
-                             uint32_t source, hipStream_t st);
+                             uint32_t source, const uint32_t *ids, hipStream_t st);
+// out[i * width + c] = in[perm[v0 + i] * width + c] for i < count: values (or
+// batched rows) of a renumbered graph in the original order.
+void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull count, uint32_t width, uint32_t *out,
+                      hipStream_t st);
 // PR: the running sum and the residual are FP64 (an FP32 residual at a hub
 // rounds away shares below half its ulp).
 void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStream_t st);

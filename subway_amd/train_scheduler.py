@@ -10,7 +10,8 @@ at their stated sizes (BFS-sync scale 16 with no cap; SSSP-/SSWP-async scale
 24 under 1 GiB; CC-async on the symmetrised scale 25 and PR-async scale 25,
 2 GiB; SSSP-/BFS-async scale 26 under 4 GiB).  A sweep can be split over
 several processes (--set, --only i:j, --dump) and trained from the dumps
-(--train-from).
+(--train-from).  The graphs are renumbered by descending in-degree, as the
+benchmark's are (--order natural keeps the generator's ids).
 
 For every training configuration (algorithm, R-MAT graph, device budget) the
 async run is repeated under a grid of fixed schedules -- whole-range density
@@ -77,8 +78,14 @@ def configs(quick: bool, which: str = "small"):
     return out
 
 
-def make_graph(algo, scale, seed, edge_factor=16, symmetrize=False):
-    return sw.Graph.rmat(scale, edge_factor, seed=seed, weighted=algo in ("sssp", "sswp"), symmetrize=symmetrize)
+def make_graph(algo, scale, seed, edge_factor=16, symmetrize=False, order="in-degree"):
+    """The R-MAT graph of a configuration, renumbered by descending in-degree
+    unless order is "natural" (the benchmark renumbers its graphs, and the
+    schedule's id ranges see the numbering)."""
+    g = sw.Graph.rmat(scale, edge_factor, seed=seed, weighted=algo in ("sssp", "sswp"), symmetrize=symmetrize)
+    if order == "in-degree":
+        g.reorder()
+    return g
 
 
 def run_config(g, algo, div, reps, log, seed=0, budget_bytes=None):
@@ -162,6 +169,8 @@ def main(argv=None):
     ap.add_argument("--only", default="", help="i:j, a slice of the configuration list")
     ap.add_argument("--dump", default="", help="write the sweep's results (times, traces) to this JSON and stop")
     ap.add_argument("--train-from", nargs="*", default=[], help="train from sweep dumps instead of sweeping")
+    ap.add_argument("--order", default="in-degree", choices=["in-degree", "natural"],
+                    help="vertex numbering of the sweep's graphs")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -183,14 +192,15 @@ def main(argv=None):
             todo = todo[i:j]
         dumped = []
         for algo, scale, ef, seed, div, budget, sym in todo:
-            g = make_graph(algo, scale, seed, ef, sym)
+            g = make_graph(algo, scale, seed, ef, sym, args.order)
             big = g.m >= (1 << 28)
             res, best, plain_ms = run_config(g, algo, div, 1 if big else args.reps, log, seed=len(results),
                                              budget_bytes=budget if not div else None)
             g.free()
             results.append((res, best))
             s = {"algo": algo, "scale": scale, "edge_factor": ef, "seed": seed, "budget_divisor": div,
-                 "budget_bytes": budget, "symmetrize": sym, "plain_ms": plain_ms, "best": list(best),
+                 "budget_bytes": budget, "symmetrize": sym, "order": args.order, "plain_ms": plain_ms,
+                 "best": list(best),
                  "ms": {f"{k[0]}/{k[1]}": v[0] for k, v in res.items()}}
             summary.append(s)
             dumped.append({"summary": s, "results": [[list(k), [v[0], v[1]]] for k, v in res.items()]})
