@@ -183,14 +183,16 @@ def test_c4_batched_64_sources_scale26(sw, gpu):
 
 @pytest.mark.timeout(600)
 def test_large_file_loads_into_pinned_layouts(sw, gpu, tmp_path):
-    """A 17 GB .bwcsr64 file (weighted R-MAT scale 26, edge factor 32: 2.1 G
-    edges, an 8-byte list of 16 GiB or more) is read by parallel positioned
+    """An 18 GB .bwcsr64 file (weighted R-MAT scale 26, edge factor 33: 2.2 G
+    edges, an 8-byte list of 17.7 GB, above the 16 GiB that picks the split
+    layout; edge factor 32 drops just under it once self-loops go) is read by parallel positioned
     reads straight into the pinned split arrays, never into an 8-byte list,
     and since its ids and weight span fit one word a packed copy is built from
     them: the loaded graph equals the generated one and runs with 4-byte
     transfers to the same distances."""
     import shutil
-    g = sw.Graph.rmat(26, 32, seed=42, weighted=True)
+    g = sw.Graph.rmat(26, 33, seed=42, weighted=True)
+    assert g.m * 8 >= 16 * GiB
     need = 16 + 8 * (g.n + g.m)
     if shutil.disk_usage(str(tmp_path)).free < 2 * need:
         pytest.skip(f"needs {2 * need / 1e9:.0f} GB of free disk for the file")
@@ -200,7 +202,7 @@ def test_large_file_loads_into_pinned_layouts(sw, gpu, tmp_path):
     t1 = time.perf_counter()
     h = sw.Graph.load(str(path))
     t2 = time.perf_counter()
-    log(f"scale 26 ef 32: {g.m} edges, save {t1 - t0:.1f} s, load {t2 - t1:.1f} s "
+    log(f"scale 26 ef 33: {g.m} edges, save {t1 - t0:.1f} s, load {t2 - t1:.1f} s "
         f"({path.stat().st_size / (t2 - t1) / 1e9:.1f} GB/s)")
     assert (h.n, h.m, h.max_degree, h.max_degree_vertex) == (g.n, g.m, g.max_degree, g.max_degree_vertex)
     assert np.array_equal(h.node_pointer(), g.node_pointer())
