@@ -56,6 +56,7 @@ struct RunOptions {
     int noResident = 0;     // 1: move active edges every iteration even when the whole graph fits one buffer
     uint residentMinBytes = 0;   // learned schedule: least active edge bytes for an in-place visit (0: 8 MiB)
     int pushOnly = 0;       // BFS on a resident graph: 1 = top-down steps only (no direction switching)
+    int noBlocks = 0;       // 1: whole id ranges move as plain edges even when the graph holds its block layout
 };
 
 struct IterRecord {
@@ -109,6 +110,9 @@ struct RunStats {
     ull h2dBytes = 0;          // edge bytes moved host -> device
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads
+    ull h2dBlockedBytes = 0;   // ... of which in the block layout
+    ull blockedEdges = 0;      // edges of the units moved in the block layout
+    uint blockedUnits = 0;     // units moved in the block layout
     uint globalIterations = 0;
     ull innerRounds = 0;       // frontier launches that selected work
     ull partitions = 0;

@@ -88,6 +88,16 @@ public:
     // Runs take sources and return values in the original numbering.
     uint32_t *perm = nullptr;
     uint32_t *iperm = nullptr;
+    // Block-compressed rows (the layout of subway_amd/csrc/blocks.hpp), built
+    // on the device by build_blocks(): blk holds blkWords pinned words,
+    // blkPtr[v] (n + 1 entries, ordinary memory) the word where row v starts,
+    // blkCode the weight code (bits | base << 6).  rowsSorted: every row of
+    // every host layout is in ascending destination order.
+    uint32_t *blk = nullptr;
+    ull *blkPtr = nullptr;
+    ull blkWords = 0;
+    uint32_t blkCode = 0;
+    bool rowsSorted = false;
 };
 
 // Renumbers the vertices in descending in-degree order (ties by ascending
@@ -96,6 +106,16 @@ public:
 // Call it before any run or session holds the graph.
 template <class E>
 void reorder_by_in_degree(HostCsr<E> &csr);
+
+// Sorts every row by destination, on the device, rewriting each host layout
+// the graph holds in place (results of every application are unchanged:
+// only the order of a vertex's out-edges moves), then builds the
+// block-compressed copy (blk, blkPtr, blkCode).  Needs about 1.5 times the
+// graph's edge bytes of free device memory plus 13.5 GiB of sort scratch
+// (less for graphs under 2^29 edges).  Call it before any run or session
+// holds the graph.
+template <class E>
+void build_blocks(HostCsr<E> &csr);
 
 // splitLayout: a .bwcsr graph that moves only in the split format (weights
 // must fit a byte); weighted lists of 16 GiB or more choose it by themselves

@@ -83,7 +83,8 @@ typedef struct subway_options {
                                edges in the resident ranges; 0 = 8 MiB */
     int32_t push_only;      /* BFS on a resident graph: 0 (default) switches between top-down push steps and
                                bottom-up pull steps by frontier size (direction-optimizing BFS); 1 pushes only */
-    int32_t reserved[1];
+    int32_t no_blocks;      /* 1: a graph with its block layout (subway_graph_build_blocks) still moves its
+                               plain edges; 0 (default) moves whole id ranges in the block layout */
 } subway_options;
 
 typedef struct subway_stats {
@@ -128,8 +129,11 @@ typedef struct subway_stats {
     /* direction-optimizing BFS on a resident graph: bottom-up steps taken and
      * the vertices they found */
     uint32_t pull_rounds;
-    uint32_t pad0;
+    /* block layout: units moved in it, their bytes and their edges */
+    uint32_t blocked_units;
     uint64_t pull_found;
+    uint64_t h2d_blocked_bytes;
+    uint64_t blocked_edges;
 } subway_stats;
 
 typedef struct subway_iter_record {
@@ -215,6 +219,23 @@ int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_poi
  * graph (SUBWAY_ERR_ARG otherwise).  Needs about twice the graph's edge bytes
  * of free device memory.  No counterpart in the reference. */
 int32_t subway_graph_reorder(subway_graph *g, int32_t order);
+/* Sorts every row by destination and builds the block-compressed copy of
+ * the rows (32-edge blocks: an anchor destination, offsets from it at the
+ * block's bit width, weights at the graph's weight width; layout in
+ * subway_amd/csrc/blocks.hpp), on the GPU, in place.  Sessions on the graph
+ * then move whole id ranges in that layout (fewer bytes over the host link)
+ * unless options.no_blocks is set; every result is unchanged (a vertex's
+ * out-edges keep their multiset).  The accessors (export, split_view, save)
+ * see the sorted rows.  Call it before any session is created on the graph
+ * (SUBWAY_ERR_ARG otherwise); needs about 1.5 times the graph's edge bytes
+ * plus 13.5 GiB of free device memory.  No counterpart in the reference (its
+ * transfers are the 8-byte edges, shared/subgraph_generator.cu:63-97). */
+int32_t subway_graph_build_blocks(subway_graph *g);
+/* The block layout: row_ptr[n+1] (word offset of each row), the words
+ * (pinned, num_words of them) and the weight code (bits | base << 6); valid
+ * while the graph lives.  SUBWAY_ERR_ARG when the graph has none. */
+int32_t subway_graph_blocks_view(const subway_graph *g, const uint64_t **row_ptr, const uint32_t **words,
+                                 uint64_t *num_words, uint32_t *weight_code);
 /* perm[v] = internal id of original vertex v, iperm[p] = original id of
  * internal vertex p (n entries each, valid while the graph lives); both NULL
  * for a graph that keeps its own numbering. */

@@ -52,7 +52,7 @@ class _Options(ctypes.Structure):
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
         ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
         ("no_resident", ctypes.c_int32), ("resident_min_bytes", ctypes.c_uint32), ("push_only", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("no_blocks", ctypes.c_int32),
     ]
 
 
@@ -72,7 +72,8 @@ class _Stats(ctypes.Structure):
         ("lane_work", ctypes.c_uint64), ("lane_entries", ctypes.c_uint64),
         ("pair_work_edges", ctypes.c_uint64), ("pair_work_entries", ctypes.c_uint64), ("pair_updates", ctypes.c_uint64),
         ("resident_visits", ctypes.c_uint32), ("resident_overlaps", ctypes.c_uint32),
-        ("pull_rounds", ctypes.c_uint32), ("pad0", ctypes.c_uint32), ("pull_found", ctypes.c_uint64),
+        ("pull_rounds", ctypes.c_uint32), ("blocked_units", ctypes.c_uint32), ("pull_found", ctypes.c_uint64),
+        ("h2d_blocked_bytes", ctypes.c_uint64), ("blocked_edges", ctypes.c_uint64),
     ]
 
 
@@ -235,6 +236,34 @@ class Graph:
             raise ValueError(f"order must be 'in-degree', not {order!r}")
         check(lib.subway_graph_reorder(self._h, 1))
 
+    def build_blocks(self) -> None:
+        """Sorts every row by destination and builds the block-compressed
+        copy of the rows on the GPU, in place (subway_graph_build_blocks).
+        Sessions then move whole id ranges in that layout unless created
+        with blocks=False; every result is unchanged.  csr(), split_view()
+        and save() then see the sorted rows.  Call it before creating any
+        session on the graph."""
+        check(lib.subway_graph_build_blocks(self._h))
+
+    def blocks_view(self):
+        """(row_ptr u64[n+1], words u32[num_words], weight_bits, weight_base)
+        of the block layout, read-only zero-copy views that keep the graph
+        alive, or None when the graph has no block layout."""
+        rp, w, nw, code = _u64p(), _u32p(), ctypes.c_uint64(), ctypes.c_uint32()
+        rc = lib.subway_graph_blocks_view(self._h, ctypes.byref(rp), ctypes.byref(w), ctypes.byref(nw),
+                                          ctypes.byref(code))
+        if rc != 0:
+            return None
+        out = []
+        for ptr, count, ct, dt in ((rp, self.n + 1, ctypes.c_uint64, np.uint64),
+                                   (w, nw.value, ctypes.c_uint32, np.uint32)):
+            buf = (ct * max(count, 1)).from_address(ctypes.addressof(ptr.contents))
+            buf._graph = self
+            a = np.frombuffer(buf, dtype=dt)[:count]
+            a.flags.writeable = False
+            out.append(a)
+        return out[0], out[1], code.value & 63, code.value >> 6
+
     def order(self):
         """(perm, iperm): read-only u32[n] views with perm[v] the internal id
         of original vertex v and iperm[p] the original id of internal vertex
@@ -329,8 +358,10 @@ class RunStats:
     resident_visits: int = 0
     resident_overlaps: int = 0
     pull_rounds: int = 0
-    pad0: int = 0
+    blocked_units: int = 0
     pull_found: int = 0
+    h2d_blocked_bytes: int = 0
+    blocked_edges: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -349,7 +380,7 @@ def _set_policy(h, params):
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
              profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
-             join_window=0, resident=True, resident_min_bytes=0, push_only=False):
+             join_window=0, resident=True, resident_min_bytes=0, push_only=False, blocks=True):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -374,6 +405,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.no_resident = 0 if resident else 1
     o.resident_min_bytes = resident_min_bytes
     o.push_only = int(push_only)
+    o.no_blocks = 0 if blocks else 1
     return o
 
 
@@ -399,16 +431,18 @@ class Session:
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
                  packed: bool = True, params=None, join_window: int = 0, resident: bool = True,
-                 resident_min_bytes: int = 0, push_only: bool = False):
+                 resident_min_bytes: int = 0, push_only: bool = False, blocks: bool = True):
         """``resident``: a graph whose edges fit one buffer stays there, and
         the learned schedule visits in place the ranges whose whole edge slice
         a buffer still holds, when they hold ``resident_min_bytes`` (0: 8 MiB)
         of active edges or more.  ``push_only``: BFS on a resident graph
-        takes top-down steps only (no bottom-up steps for dense frontiers)."""
+        takes top-down steps only (no bottom-up steps for dense frontiers).
+        ``blocks``: on a graph with its block layout (Graph.build_blocks),
+        whole id ranges move in that layout (False: as plain edges)."""
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                      profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
-                     resident_min_bytes, push_only)
+                     resident_min_bytes, push_only, blocks)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -507,7 +541,7 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
         threshold: float = 0.01, init_pr: float = 0.15, transfer: str = "auto", policy: str = "plain",
         verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
         rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0, packed: bool = True,
-        params=None, join_window: int = 0, resident: bool = True, push_only: bool = False):
+        params=None, join_window: int = 0, resident: bool = True, push_only: bool = False, blocks: bool = True):
     """Run one application ("sssp", mode "async" = sssp-async) to its fixpoint.
 
     Returns (values, RunStats); values are uint32 (BFS/CC/SSSP/SSWP) or
@@ -518,20 +552,21 @@ def run(graph: Graph, algo: str, mode: str = "sync", source: int = 0, budget: in
     ``resident``: a graph whose edges fit one device buffer moves once per
     run and stays there (False: the active edges move every iteration).
     ``push_only``: BFS on a resident graph without bottom-up steps.
+    ``blocks``: whole id ranges of a graph with its block layout move in it.
     """
     if params is not None:
         s = Session(graph, algo, mode=mode, budget=budget, buffers=buffers, threshold=threshold, init_pr=init_pr,
                     transfer=transfer, policy=policy, verbose=verbose, advance_grid=advance_grid, profile=profile,
                     rounds=rounds, rounds_grid=rounds_grid, max_rounds=max_rounds, theta=theta,
                     spin_limit=spin_limit, packed=packed, params=params, join_window=join_window,
-                    resident=resident, push_only=push_only)
+                    resident=resident, push_only=push_only, blocks=blocks)
         try:
             return s.run(source)
         finally:
             s.close()
     o = _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                  profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
-                 push_only=push_only)
+                 push_only=push_only, blocks=blocks)
     vals = np.empty(graph.n, dtype=np.float32 if algo == "pr" else np.uint32)
     st = _Stats()
     rec = (_IterRecord * _TRACE_CAP)()

@@ -199,6 +199,31 @@ int32_t subway_graph_reorder(subway_graph *g, int32_t order)
     });
 }
 
+int32_t subway_graph_build_blocks(subway_graph *g)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        const long users = g->weighted ? g->w.use_count() : g->u.use_count();
+        if (users > 1) throw Error(ERR_ARG, "build the block layout before any session is created on the graph");
+        if (g->weighted) build_blocks(*g->w);
+        else build_blocks(*g->u);
+    });
+}
+
+int32_t subway_graph_blocks_view(const subway_graph *g, const uint64_t **row_ptr, const uint32_t **words,
+                                 uint64_t *num_words, uint32_t *weight_code)
+{
+    return guard([&] {
+        if (!g) throw Error(ERR_ARG, "null graph");
+        const uint32_t *w = g->weighted ? g->w->blk : g->u->blk;
+        if (!w) throw Error(ERR_ARG, "the graph holds no block layout");
+        if (row_ptr) *row_ptr = reinterpret_cast<const uint64_t *>(g->weighted ? g->w->blkPtr : g->u->blkPtr);
+        if (words) *words = w;
+        if (num_words) *num_words = g->weighted ? g->w->blkWords : g->u->blkWords;
+        if (weight_code) *weight_code = g->weighted ? g->w->blkCode : g->u->blkCode;
+    });
+}
+
 int32_t subway_graph_order(const subway_graph *g, const uint32_t **perm, const uint32_t **iperm)
 {
     return guard([&] {
@@ -309,6 +334,7 @@ static RunOptions to_options(const subway_options *o)
     r.noResident = o->no_resident;
     r.residentMinBytes = o->resident_min_bytes;
     r.pushOnly = o->push_only;
+    r.noBlocks = o->no_blocks;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -377,6 +403,9 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->resident_overlaps = s.residentOverlaps;
     stats->pull_rounds = s.pullRounds;
     stats->pull_found = s.pullFound;
+    stats->blocked_units = s.blockedUnits;
+    stats->h2d_blocked_bytes = s.h2dBlockedBytes;
+    stats->blocked_edges = s.blockedEdges;
 }
 
 extern "C" {

@@ -109,6 +109,26 @@ __device__ __forceinline__ u32 wave_reduce(u32 x)
     return x;
 }
 
+__device__ __forceinline__ u32 wave_reduce_min(u32 x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const u32 y = __shfl_xor(x, o, WAVE);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ u32 wave_reduce_max(u32 x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const u32 y = __shfl_xor(x, o, WAVE);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
 // --------------------------------------------- epoch-tagged tile descriptor
 // A tile publishes two self-contained 64-bit words, each epoch(22) |
 // state(2) | value(40): one carries the selected count, the other the weight

@@ -122,6 +122,10 @@ struct Unit {
     // a DMA of every edge of the ids [idLo, idHi) (whole ranges, fully active
     // contiguous runs): the buffer then holds that id range's whole slice
     uint idLo, idHi;
+    // ... moved in the block layout: words [blkLo, blkLo + blkWords) of the
+    // graph's block copy (the rows of idLo .. idHi - 1)
+    bool blocked;
+    ull blkLo, blkWords;
 };
 
 // One compute and one copy stream per device, shared by every engine of the
@@ -236,6 +240,16 @@ public:
                 }
             }
         }
+        bufBytes_ = (ull)sub_->max_partition_size * sizeof(E);
+        if (g_->csr->blk && !o_.noBlocks && !o_.batch && g_->num_edges > 0) {
+            // whole id ranges move in the block layout; the frontier finds a
+            // row's words through the graph's row offsets
+            blkOk_ = true;
+            blkCode_ = g_->csr->blkCode;
+            const size_t bytes = ((size_t)g_->num_nodes + 1) * sizeof(ull);
+            SUBWAY_CHECK(hipMalloc(&d_blk_, bytes));
+            SUBWAY_CHECK(hipMemcpy(d_blk_, g_->csr->blkPtr, bytes, hipMemcpyHostToDevice));
+        }
         // a graph that fits one buffer moves once per run and stays there
         resident_ = !o_.noResident && !o_.batch && g_->num_edges > 0 && g_->num_edges <= maxPart_;
         if constexpr (!isPR) {
@@ -283,6 +297,7 @@ public:
         }
         if (o_.persistent) {
             int mx = rounds_max_grid(o_.algo, fmt_);
+            if (blkOk_) mx = std::min(mx, rounds_max_grid(o_.algo, EDGE_BLOCKS));
             roundsGrid_ = std::min(mx, o_.roundsGrid > 0 ? o_.roundsGrid : prop.multiProcessorCount * 4);
             if (roundsGrid_ > 0) {
                 SUBWAY_CHECK(hipMalloc(&pc_, sizeof(dev::PersistCtl)));
@@ -374,6 +389,7 @@ public:
         if (d_unperm_) (void)hipFree(d_unperm_);
         if (d_inPtr_) (void)hipFree(d_inPtr_);
         if (d_inAdj_) (void)hipFree(d_inAdj_);
+        if (d_blk_) (void)hipFree(d_blk_);
         if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
@@ -425,6 +441,16 @@ private:
     int fmt_ = (int)sizeof(E);          // EdgeFormat of the device buffers and transfers
     int eb_ = (int)sizeof(E);           // bytes per transferred / buffered edge
     uint32_t packCode_ = 0;            // EDGE_PACKED: shift | weight base << 5
+    // block layout: row word offsets on the device, the weight code, and the
+    // bytes of a device buffer; bufBlocked_ = the buffer holds a block slice
+    ull *d_blk_ = nullptr;
+    bool blkOk_ = false;
+    uint32_t blkCode_ = 0;
+    ull bufBytes_ = 0;
+    bool bufBlocked_[2] = {false, false};
+    void markBlocked(Unit &u) const;
+    int fmtOf(const Unit &u) const { return u.blocked ? (int)EDGE_BLOCKS : fmt_; }
+    uint32_t codeOf(const Unit &u) const { return u.blocked ? blkCode_ : packCode_; }
     int advGrid_ = 0;
     RunStats st_;
     std::vector<Unit> units_;
@@ -456,7 +482,7 @@ private:
     void issue(size_t u);
     void compute(size_t u);
     void frontier(int first, uint8_t *labA, uint8_t *labB, const Unit &un);
-    void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf);
+    void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf, const Unit &un);
     void asyncRounds(const Unit &un, int buf);
     // gathers the unit's active adjacency, elements of esz bytes from src,
     // into buffer b's staging area at byte stageOffset
@@ -695,6 +721,23 @@ void Engine<E, G>::pushHubUnits(uint part, uint v, ull fromNode, ull fromEdge, u
     }
 }
 
+// A unit that moves every edge of the ids [idLo, idHi) by DMA moves them in
+// the block layout when the graph holds it, the rows fit a buffer with two
+// words of slack (the decoder reads a field's second word unconditionally)
+// and the blocks are smaller than the plain edges.
+template <class E, class G>
+void Engine<E, G>::markBlocked(Unit &u) const
+{
+    u.blocked = false;
+    if (!blkOk_ || !u.dma || u.hub || u.count == 0 || u.idHi <= u.idLo) return;
+    const ull lo = g_->csr->blkPtr[u.idLo], words = g_->csr->blkPtr[u.idHi] - lo;
+    if ((words + 2) * 4 > bufBytes_ || words + 2 >= (1ull << 32)) return;
+    if (words * 4 >= u.count * (ull)eb_) return;
+    u.blocked = true;
+    u.blkLo = lo;
+    u.blkWords = words;
+}
+
 // Learned policy, per global iteration, after the policy kernel: the
 // network's heads give the next iteration's active-edge fraction, the
 // whole-range density threshold and the inner-round cap.  The plain policy is
@@ -809,6 +852,7 @@ void Engine<E, G>::buildUnitsLearned()
             u.idBase = lo;
             u.idLo = lo;
             u.idHi = hi;
+            markBlocked(u);
             hasRes[b] = true;
             st_.residentVisits++;
         }
@@ -846,6 +890,7 @@ void Engine<E, G>::buildUnitsLearned()
             u.idHi = bounds_[gLastRange + 1];
         }
         setItems(u, gLo, gHi - gLo, gBase);
+        markBlocked(u);
         units_.push_back(u);
         open = false;
     };
@@ -864,6 +909,7 @@ void Engine<E, G>::buildUnitsLearned()
         u.idBase = wB0;
         u.idLo = wB0;
         u.idHi = wB1;
+        markBlocked(u);
         units_.push_back(u);
         wOpen = false;
     };
@@ -951,6 +997,9 @@ void Engine<E, G>::buildUnits()
         u.ptr = g_->d_nodePointer;
         u.fromEdge = 0;
         u.idBase = 0;
+        u.idLo = 0;
+        u.idHi = g_->num_nodes;
+        markBlocked(u);
         units_.push_back(u);
         return;
     }
@@ -992,6 +1041,8 @@ void Engine<E, G>::buildUnits()
         if (contiguous && o_.transfer != TRANSFER_ZEROCOPY) {
             u.dma = true;
             u.hostEdge = g_->nodePointer[p.firstVertex[i]];
+            u.idLo = p.firstVertex[i];
+            u.idHi = p.lastVertex[i] + 1;
         } else {
             u.dma = false;
         }
@@ -1000,6 +1051,7 @@ void Engine<E, G>::buildUnits()
         u.kend = p.fromNode[i] + cnt;
         u.edgeBase = p.fromEdge[i];
         setItems(u, p.fromNode[i], cnt, p.fromEdge[i]);
+        markBlocked(u);
         units_.push_back(u);
     }
 }
@@ -1015,12 +1067,14 @@ void Engine<E, G>::issue(size_t idx)
         return;
     }
     if (resident_) residentLoaded_ = true;
-    if (u.dma && u.count && bufCount_[b] == u.count && bufHostEdge_[b] == u.hostEdge) {
+    if (u.dma && u.count && bufCount_[b] == u.count && bufHostEdge_[b] == u.hostEdge &&
+        bufBlocked_[b] == u.blocked) {
         SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));   // the slice is still in the buffer
         return;
     }
     bufCount_[b] = u.dma ? u.count : 0;
     bufHostEdge_[b] = u.hostEdge;
+    bufBlocked_[b] = u.blocked;
     const bool slice = u.dma && u.count && u.idHi > u.idLo;
     bufIdLo_[b] = slice ? u.idLo : 0u;
     bufIdHi_[b] = slice ? u.idHi : 0u;
@@ -1031,7 +1085,15 @@ void Engine<E, G>::issue(size_t idx)
         const bool split = fmt_ == EDGE_SPLIT;
         // split format: destinations at the buffer's start, weights at byte 4 * capacity
         char *dstW = split ? dst + 4ull * maxPart_ : nullptr;
-        if (u.dma) {
+        if (u.blocked) {
+            const size_t bb = (size_t)u.blkWords * 4;
+            SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->blk + u.blkLo, bb, hipMemcpyHostToDevice, sx_));
+            st_.h2dDmaBytes += bb;
+            st_.h2dBlockedBytes += bb;
+            st_.blockedEdges += u.count;
+            st_.blockedUnits++;
+            st_.h2dBytes += bb;
+        } else if (u.dma) {
             if (split) {
                 SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->splitDst + u.hostEdge, 4ull * u.count,
                                             hipMemcpyHostToDevice, sx_));
@@ -1069,7 +1131,7 @@ void Engine<E, G>::issue(size_t idx)
             }
             st_.h2dGatherBytes += bytes;
         }
-        st_.h2dBytes += bytes;
+        if (!u.blocked) st_.h2dBytes += bytes;
         if (o_.profile) pXfer_.end(sx_);
     }
     SUBWAY_CHECK(hipEventRecord(evLoaded_[b], sx_));
@@ -1141,18 +1203,22 @@ void Engine<E, G>::frontier(int first, uint8_t *labA, uint8_t *labB, const Unit 
     a.entries = entries_;
     a.chunkMap = chunkMap_;
     a.first = first;
+    if (un.blocked) {
+        a.blk = d_blk_;
+        a.blkBase = g_->csr->blkPtr[un.idLo];
+    }
     if (o_.profile) pFr_.begin(sc_);
     launch_frontier(a, fr_, sc_);
     if (o_.profile) pFr_.end(sc_);
 }
 
 template <class E, class G>
-void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf)
+void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf, const Unit &un)
 {
     AdvanceArgs a{};
     a.algo = o_.algo;
-    a.edgeFmt = fmt_;
-    a.packCode = packCode_;
+    a.edgeFmt = fmtOf(un);
+    a.packCode = codeOf(un);
     a.entries = entries_;
     a.chunkMap = chunkMap_;
     a.edges = sub_->d_buffers[buf];
@@ -1184,7 +1250,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         // would add its membership and barrier costs for nothing); the labels
         // it sets are picked up by the next generate
         frontier(1, l1, l2, un);
-        advance(l2, 0, ~0ull, 0, buf);
+        advance(l2, 0, ~0ull, 0, buf, un);
         return;
     }
     if (roundsGrid_ > 0) {
@@ -1214,9 +1280,13 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.maxRounds = (uint32_t)std::max(1, roundsCap_);
         l.roundBase = 0;
         if (o_.profile) pRounds_.begin(sc_);
-        l.packCode = packCode_;
+        l.packCode = codeOf(un);
+        if (un.blocked) {
+            l.blk = d_blk_;
+            l.blkBase = g_->csr->blkPtr[un.idLo];
+        }
         l.joinTicks = o_.joinWindow;
-        launch_rounds(o_.algo, fmt_, l, roundsGrid_, sc_);
+        launch_rounds(o_.algo, fmtOf(un), l, roundsGrid_, sc_);
         if (o_.profile) pRounds_.end(sc_);
         return;
     }
@@ -1228,7 +1298,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
             const bool odd = (r & 1) != 0;
             if (r == 1) frontier(1, l1, l2, un);
             else frontier(0, odd ? l1 : l2, nullptr, un);
-            advance(odd ? l2 : l1, 0, ~0ull, 0, buf);
+            advance(odd ? l2 : l1, 0, ~0ull, 0, buf, un);
         }
         SUBWAY_CHECK(hipMemcpyAsync(&h_ctl_->stop, &fr_.ctl->stop, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
         SUBWAY_CHECK(hipMemcpyAsync(&h_err_[0], &fr_.ctl->error, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
@@ -1312,7 +1382,7 @@ void Engine<E, G>::bfsDirectionOptimizing(const Unit &un, int buf)
                 st_.pullRounds++;
                 pullCounted_ = true;
             } else {
-                advance(lb, 0, ~0ull, 0, buf);
+                advance(lb, 0, ~0ull, 0, buf, un);
             }
             std::swap(la, lb);
             if (pullCounted_) {
@@ -1358,11 +1428,11 @@ void Engine<E, G>::compute(size_t idx)
             else frontier(1, l2, nullptr, u);
         }
         const ull w0 = u.chunk * maxPart_;
-        advance(l2, w0, w0 + u.count, w0, b);
+        advance(l2, w0, w0 + u.count, w0, b, u);
     } else if (!async) {
         if (isPR) frontier(1, nullptr, nullptr, u);
         else frontier(1, l2, nullptr, u);
-        advance(l2, 0, ~0ull, 0, b);
+        advance(l2, 0, ~0ull, 0, b, u);
     } else {
         asyncRounds(u, b);
     }

@@ -58,6 +58,8 @@ HostCsr<E>::~HostCsr()
     if (splitW) (void)hipHostFree(splitW);
     delete[] perm;
     delete[] iperm;
+    if (blk) (void)hipHostFree(blk);
+    delete[] blkPtr;
 }
 
 template <class E>

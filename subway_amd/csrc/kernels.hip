@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 #include "device.hpp"
 #include "mlp_device.hpp"
+#include "blocks.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -482,9 +483,9 @@ __device__ __forceinline__ void write_entry(Entry *entries, u32 *chunkMap, u64 k
     if constexpr (COH) {
         u64 *q = reinterpret_cast<u64 *>(entries + k);
         mst<true>(q, (u64)loc | ((u64)(u32)pre << 32));
-        mst<true>(q + 1, (u64)val);
+        mst<true>(q + 1, (u64)val | ((u64)(u32)deg << 32));
     } else {
-        *reinterpret_cast<uint4 *>(entries + k) = make_uint4(loc, (u32)pre, val, 0u);
+        *reinterpret_cast<uint4 *>(entries + k) = make_uint4(loc, (u32)pre, val, (u32)deg);
     }
     // chunk j starts inside [pre, pre + deg): record the entry that holds it
     u64 j = (pre + CHUNK - 1) / CHUNK;
@@ -492,9 +493,9 @@ __device__ __forceinline__ void write_entry(Entry *entries, u32 *chunkMap, u64 k
     for (; j <= jend; j++) mst<COH>(&chunkMap[j], (u32)k);
 }
 
-// Entry fields (loc, prefix, val) of entries[k].
+// Entry fields (loc, prefix, val, degree) of entries[k].
 template <bool COH>
-__device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc, u32 &pre, u32 &val)
+__device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc, u32 &pre, u32 &val, u32 &deg)
 {
     if constexpr (COH) {
         const u64 *q = reinterpret_cast<const u64 *>(entries + k);
@@ -502,12 +503,20 @@ __device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc
         loc = (u32)a;
         pre = (u32)(a >> 32);
         val = (u32)b;
+        deg = (u32)(b >> 32);
     } else {
         const uint4 e = *reinterpret_cast<const uint4 *>(entries + k);
         loc = e.x;
         pre = e.y;
         val = e.z;
+        deg = e.w;
     }
+}
+template <bool COH>
+__device__ __forceinline__ void read_entry(const Entry *entries, u64 k, u32 &loc, u32 &pre, u32 &val)
+{
+    u32 deg;
+    read_entry<COH>(entries, k, loc, pre, val, deg);
 }
 
 // Labelled vertices of the partition push their current value; their labels
@@ -525,6 +534,8 @@ struct FrontierLabelOp {
     Entry *entries;
     u32 *chunkMap;
     u32 idBase;    // items are ids idBase + i when nodes is null (a whole id range)
+    const u64 *blk;   // block layout in the buffer: row word offsets (null: plain edges)
+    u64 blkBase;      // ... of the buffer's first row
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -542,7 +553,7 @@ struct FrontierLabelOp {
                     if (p1 > p0) {
                         mask |= 1u << j;
                         w[j] = p1 - p0;
-                        it[j].loc = (u32)(p0 - fromEdge);
+                        it[j].loc = blk ? (u32)(blk[id] - blkBase) : (u32)(p0 - fromEdge);
                         it[j].val = mld<COH>(&value[id]);
                     }
                 }
@@ -576,6 +587,8 @@ struct FrontierPrOp {
     Entry *entries;
     u32 *chunkMap;
     u32 idBase;
+    const u64 *blk;   // block layout in the buffer: row word offsets (null: plain edges)
+    u64 blkBase;
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -593,7 +606,7 @@ struct FrontierPrOp {
                         const float share = (float)(d / (double)(p1 - p0) * 0.85);
                         mask |= 1u << j;
                         w[j] = p1 - p0;
-                        it[j].loc = (u32)(p0 - fromEdge);
+                        it[j].loc = blk ? (u32)(blk[id] - blkBase) : (u32)(p0 - fromEdge);
                         it[j].val = __float_as_uint(share);
                     }
                 }
@@ -841,9 +854,11 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
                                              u32 *value, uint8_t *lab, double *delta, u64 count, u64 W, u64 w0,
                                              u64 wend, u64 shift, u32 psh, u32 *err, u32 *claimCtr, u32 *s_next)
 {
+    constexpr bool BLOCKED = std::is_same<E, BlockEdge>::value;
     __shared__ u32 s_pre[ADV_CHUNK + 1];
     __shared__ u32 s_loc[ADV_CHUNK + 1];
     __shared__ u32 s_val[ADV_CHUNK + 1];
+    __shared__ u32 s_deg[BLOCKED ? ADV_CHUNK + 1 : 1];
     const int tid = threadIdx.x;
     const u64 cbeg = j * ADV_CHUNK;
     const u64 pbeg = cbeg > w0 ? cbeg : w0;
@@ -855,7 +870,10 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     // never dereferenced)
     const bool bad = k0 > k1 || k1 >= count || k1 - k0 > (u64)ADV_CHUNK;
     const int nk = bad ? 0 : (int)(k1 - k0 + 1);
-    for (int t = tid; t < nk; t += ADV_BLOCK) read_entry<COH>(entries, k0 + t, s_loc[t], s_pre[t], s_val[t]);
+    for (int t = tid; t < nk; t += ADV_BLOCK) {
+        if constexpr (BLOCKED) read_entry<COH>(entries, k0 + t, s_loc[t], s_pre[t], s_val[t], s_deg[t]);
+        else read_entry<COH>(entries, k0 + t, s_loc[t], s_pre[t], s_val[t]);
+    }
     if (__syncthreads_or(bad)) {
         if (tid == 0) {
             (void)atomicCAS(err, 0u, WAIT_DATA);
@@ -873,35 +891,99 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     // for the memory operation right there) is needed.
     constexpr int PPT = ADV_CHUNK / ADV_BLOCK;
     static_assert(ADV_CHUNK % ADV_BLOCK == 0, "a chunk is a whole number of positions per thread");
-    typename Codec<E>::Raw raw[PPT];
     u32 sv[PPT];
     bool ok[PPT];
-#pragma unroll
-    for (int u = 0; u < PPT; u++) {
-        const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
-        ok[u] = p < pend;
-        if (ok[u]) {
-            const u32 pp = (u32)p;
-            int lo = 0, hi = nk - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= pp) lo = mid;
-                else hi = mid - 1;
-            }
-            const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-            raw[u] = Codec<E>::load(edges, eidx, psh);
-            sv[u] = s_val[lo];
-        }
-    }
     u32 dst[PPT], cv[PPT];
+    if constexpr (BLOCKED) {
+        // block layout (blocks.hpp): the descriptor of the position's block,
+        // then the words its offset and weight fields sit in (both words of
+        // each field are read; a field that ends in the first one ignores the
+        // second, and a buffer keeps two words of slack past its rows)
+        const u32 *W = reinterpret_cast<const u32 *>(edges);
+        const u32 wbits = psh & 63u, wbase = psh >> BLK_WFIELD;
+        constexpr bool WEIGHTS = ALGO == ALGO_SSSP || ALGO == ALGO_SSWP;
+        // per position: pay = first payload word of the row, jc = index in
+        // the block | edges of the block << 8, then the descriptor (da, dc)
+        u32 pay[PPT], jc[PPT], da[PPT], dc[PPT];
 #pragma unroll
-    for (int u = 0; u < PPT; u++)
-        if (ok[u]) {
-            const auto e = Codec<E>::decode(raw[u], psh);
-            dst[u] = e.end;
-            if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
-            else cv[u] = Rule<ALGO>::cand(sv[u], edge_weight(e));
+        for (int u = 0; u < PPT; u++) {
+            const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
+            ok[u] = p < pend;
+            if (ok[u]) {
+                const u32 pp = (u32)p;
+                int lo = 0, hi = nk - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= pp) lo = mid;
+                    else hi = mid - 1;
+                }
+                const u32 row = s_loc[lo], i = pp - s_pre[lo], d = s_deg[lo];
+                sv[u] = s_val[lo];
+                const u32 k = i >> BLK_SHIFT, left = d - (k << BLK_SHIFT);
+                pay[u] = row + 2u * ((d + BLK_EDGES - 1u) >> BLK_SHIFT);
+                jc[u] = (i & (BLK_EDGES - 1u)) | ((left < (u32)BLK_EDGES ? left : (u32)BLK_EDGES) << 8);
+                da[u] = W[row + 2u * k];
+                dc[u] = W[row + 2u * k + 1u];
+            }
         }
+        // the offset field's two words (and the weight field's); sh packs the
+        // two bit shifts and the offset width
+        u32 f0[PPT], f1[PPT], g0[PPT], g1[PPT], sh[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (ok[u]) {
+                const u32 jj = jc[u] & 0xFFu, cnt = jc[u] >> 8, wd = dc[u] & 63u;
+                const u32 P = pay[u] + (dc[u] >> BLK_WFIELD);
+                const u32 obit = jj ? (jj - 1u) * wd : 0u;
+                f0[u] = W[P + (obit >> 5)];
+                f1[u] = W[P + (obit >> 5) + 1u];
+                u32 gsh = 0;
+                if constexpr (WEIGHTS) {
+                    const u32 gbit = (cnt - 1u) * wd + jj * wbits;
+                    g0[u] = W[P + (gbit >> 5)];
+                    g1[u] = W[P + (gbit >> 5) + 1u];
+                    gsh = gbit & 31u;
+                }
+                sh[u] = (obit & 31u) | (gsh << 8) | ((jj ? wd : 0u) << 16);
+            }
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (ok[u]) {
+                dst[u] = da[u] + blk_extract(f0[u], f1[u], sh[u] & 31u, sh[u] >> 16);
+                if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
+                else if constexpr (WEIGHTS)
+                    cv[u] = Rule<ALGO>::cand(sv[u], wbase + blk_extract(g0[u], g1[u], (sh[u] >> 8) & 31u, wbits));
+                else cv[u] = Rule<ALGO>::cand(sv[u], 1u);
+            }
+        (void)shift;
+    } else {
+        typename Codec<E>::Raw raw[PPT];
+#pragma unroll
+        for (int u = 0; u < PPT; u++) {
+            const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
+            ok[u] = p < pend;
+            if (ok[u]) {
+                const u32 pp = (u32)p;
+                int lo = 0, hi = nk - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= pp) lo = mid;
+                    else hi = mid - 1;
+                }
+                const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
+                raw[u] = Codec<E>::load(edges, eidx, psh);
+                sv[u] = s_val[lo];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PPT; u++)
+            if (ok[u]) {
+                const auto e = Codec<E>::decode(raw[u], psh);
+                dst[u] = e.end;
+                if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
+                else cv[u] = Rule<ALGO>::cand(sv[u], edge_weight(e));
+            }
+    }
     u32 updates = 0;
     if constexpr (ALGO == ALGO_PR) {
         // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
@@ -1136,6 +1218,8 @@ struct RoundsArgs {
     u32 spinLimit;         // poll limit of every wait of the launch
     u32 joinTicks;         // membership window
     u32 packCode;         // EDGE_PACKED: destination bits
+    const u64 *blk;        // EDGE_BLOCKS: row word offsets (null otherwise)
+    u64 blkBase;           // ... of the buffer's first row
 };
 
 // All async inner rounds of one resident partition in one persistent
@@ -1172,13 +1256,13 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
             if (t >= numTiles) break;
             if constexpr (ALGO == ALGO_PR) {
                 FrontierPrOp<true> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc,
-                                      a.entries, a.chunkMap, a.idBase};
+                                      a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase};
                 if (!scan_tile<FrontierPrOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                  wt))
                     break;
             } else {
                 FrontierLabelOp<true> op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
-                                   a.entries, a.chunkMap, a.idBase};
+                                   a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase};
                 if (!scan_tile<FrontierLabelOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td,
                                                                     a.sc, wt))
                     break;
@@ -2079,12 +2163,13 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
     check_tiles(s, tiles, st);
     if (a.algo == ALGO_PR) {
         FrontierPrOp<false> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc, a.entries,
-                               a.chunkMap, a.idBase};
+                               a.chunkMap, a.idBase, (const u64 *)a.blk, (u64)a.blkBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                            dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first,
                            s.spinLimit);
     } else {
-        FrontierLabelOp<false> op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap, a.idBase};
+        FrontierLabelOp<false> op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap,
+                                  a.idBase, (const u64 *)a.blk, (u64)a.blkBase};
         hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp<false>, SCAN_BLOCK, FR_IPT>),
                            dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td,
                            a.first, s.spinLimit);
@@ -2101,8 +2186,16 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 }
 
 // Launch `L<ALGO, storage>` for the buffer's edge format; SSSP and SSWP need
-// a weighted format.
+// a weighted format.  The batched kernels take the plain formats only.
 #define SUBWAY_BY_FORMAT(L, ALGO, fmt, ...)                                                            \
+    do {                                                                                               \
+        if ((fmt) == EDGE_BLOCKS) {                                                                    \
+            L<ALGO, BlockEdge>(__VA_ARGS__);                                                           \
+        } else {                                                                                       \
+            SUBWAY_BY_PLAIN_FORMAT(L, ALGO, fmt, __VA_ARGS__);                                         \
+        }                                                                                              \
+    } while (0)
+#define SUBWAY_BY_PLAIN_FORMAT(L, ALGO, fmt, ...)                                                      \
     do {                                                                                               \
         if ((fmt) == EDGE_PACKED) {                                                                    \
             L<ALGO, PackedEdge>(__VA_ARGS__);                                                          \
@@ -2249,6 +2342,8 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.roundBase = l.roundBase;
     a.idBase = l.idBase;
     a.packCode = l.packCode;
+    a.blk = (const u64 *)l.blk;
+    a.blkBase = (u64)l.blkBase;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
     // an ordinary launch: the membership protocol (join_launch) makes the
@@ -2320,9 +2415,9 @@ void launch_advance_batch_window(int algo, int edgeFmt, uint32_t packCode, const
     if (grid <= 0) throw Error(ERR_ARG, "grid must be positive");
     const BatchWindow w{entries, entryMask, chunkMap, edges, dist, maskOut, w0, w1, shift, packCode, ctl, grid};
     switch (algo) {
-        case ALGO_BFS: SUBWAY_BY_FORMAT(adv_batch, ALGO_BFS, edgeFmt, w, st); break;
-        case ALGO_SSSP: SUBWAY_BY_FORMAT(adv_batch, ALGO_SSSP, edgeFmt, w, st); break;
-        case ALGO_SSWP: SUBWAY_BY_FORMAT(adv_batch, ALGO_SSWP, edgeFmt, w, st); break;
+        case ALGO_BFS: SUBWAY_BY_PLAIN_FORMAT(adv_batch, ALGO_BFS, edgeFmt, w, st); break;
+        case ALGO_SSSP: SUBWAY_BY_PLAIN_FORMAT(adv_batch, ALGO_SSSP, edgeFmt, w, st); break;
+        case ALGO_SSWP: SUBWAY_BY_PLAIN_FORMAT(adv_batch, ALGO_SSWP, edgeFmt, w, st); break;
         default: throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
     }
     SUBWAY_CHECK(hipGetLastError());
@@ -2352,9 +2447,9 @@ static const void *batch_fn_for(int algo, int fmt, int unroll)
 {
     const void *f = nullptr;
     switch (algo) {
-        case ALGO_BFS: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_BFS, fmt, &f, unroll); break;
-        case ALGO_SSSP: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_SSSP, fmt, &f, unroll); break;
-        case ALGO_SSWP: SUBWAY_BY_FORMAT(batch_fn_u, ALGO_SSWP, fmt, &f, unroll); break;
+        case ALGO_BFS: SUBWAY_BY_PLAIN_FORMAT(batch_fn_u, ALGO_BFS, fmt, &f, unroll); break;
+        case ALGO_SSSP: SUBWAY_BY_PLAIN_FORMAT(batch_fn_u, ALGO_SSSP, fmt, &f, unroll); break;
+        case ALGO_SSWP: SUBWAY_BY_PLAIN_FORMAT(batch_fn_u, ALGO_SSWP, fmt, &f, unroll); break;
         default: throw Error(ERR_ARG, "batched runs support BFS, SSSP and SSWP");
     }
     return f;

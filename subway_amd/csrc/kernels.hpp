@@ -43,11 +43,15 @@ constexpr int GATHER_CHUNK = 4096;
 // kernels take the decode word packCode = shift | base << 5.  EDGE_SPLIT
 // keeps a buffer of capacity C edges as C u32 destinations followed, at byte
 // 4 C, by C u8 weights (5 bytes per edge); kernels take packCode = C.
+// EDGE_BLOCKS holds the block-compressed rows of an id range (blocks.hpp);
+// an entry's edge_local is then its row's word offset in the buffer, and
+// kernels take packCode = the weight code (bits | base << 6).
 enum EdgeFormat : int {
     EDGE_U32 = 4,      // OutEdge
     EDGE_W64 = 8,      // OutEdgeWeighted
     EDGE_PACKED = 5,   // PackedEdge
     EDGE_SPLIT = 6,    // SplitEdge
+    EDGE_BLOCKS = 7,   // BlockEdge
 };
 struct PackedEdge {
     uint32_t bits;
@@ -55,12 +59,15 @@ struct PackedEdge {
 struct SplitEdge {
     uint32_t dst;      // element of the destination array; the weights follow the array
 };
+struct BlockEdge {
+    uint32_t word;     // word of the block layout
+};
 
 struct Entry {
     uint32_t edge_local;
     uint32_t prefix;
     uint32_t val;
-    uint32_t pad;
+    uint32_t degree;   // out-degree of the vertex (the block layout decodes from it)
 };
 
 // One partition row as produced by the device partitioner.
@@ -172,6 +179,8 @@ struct FrontierArgs {
     uint32_t *chunkMap;
     int first;                       // 1: ignore the stop flag of the previous launch
     uint32_t idBase;                 // nodes == null: the items are ids idBase .. idBase + n - 1
+    const ull *blk;                  // EDGE_BLOCKS buffers: row word offsets of the graph (else null)
+    ull blkBase;                     // ... of the buffer's first row
 };
 void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
 
@@ -213,6 +222,8 @@ struct RoundsLaunch {
     uint32_t idBase;       // nodes == null: a whole id range starting here
     uint32_t joinTicks;    // membership window of the launch (100 MHz ticks); 0 = 500 us
     uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
+    const ull *blk;        // EDGE_BLOCKS buffers: row word offsets of the graph (else null)
+    ull blkBase;           // ... of the buffer's first row
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
 // below full occupancy.
