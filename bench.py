@@ -11,6 +11,10 @@ reads it: the path's job is to move it over the host link.  By default the
 graph is then renumbered on the GPU by descending in-degree (--order
 natural keeps the generator's ids), a one-time step reported as
 config.reorder_s; sources and values stay in the original numbering.
+Then (--blocks 1, the default) every row is sorted by destination and the
+block-compressed copy of the rows is built on the GPU (config.blocks_s, its
+bytes per edge in config.block_bytes_per_edge): whole id ranges then move
+in that layout; every policy variant uses it.
 
 A step is one full run from the initial state to the fixpoint (first
 generate .. values in the session's host result array).  value = input
@@ -169,6 +173,8 @@ def measure(sw, np, g, args, src, kw, names, params):
             "inner_rounds": ss[-1].inner_rounds,
             "in_place_visits": ss[-1].resident_visits,
             "h2d_gb_per_step": round(h2d / 1e9, 4),
+            "h2d_blocked_gb_per_step": round(float(np.mean([s.h2d_blocked_bytes for s in ss])) / 1e9, 4),
+            "blocked_units": ss[-1].blocked_units,
             "h2d_mb_per_iteration": [round(r["h2d_bytes"] / 1e6, 2) for r in ss[-1].trace],
             "h2d_gbps_achieved": round(h2d / (xfer_ms * 1e-3) / 1e9, 2) if xfer_ms > 0 else None,
             "ms_breakdown": {"rounds": round(float(np.mean([s.ms_rounds for s in ss])), 3),
@@ -215,8 +221,14 @@ def roofline(stats, args):
     reported apart (items_scanned, at the bytes each scanned item reads) and
     not counted.  Partition visits run either as one persistent rounds_kernel
     launch or as stand-alone frontier + advance launch pairs; the kind with
-    more kernel time is reported, priced with its own work only."""
+    more kernel time is reported, priced with its own work only.  Units moved
+    in the block layout hold an edge in fewer bytes: when they carry most of
+    the bytes moved, s(E) is their bytes per edge."""
     eb = stats[-1].edge_bytes
+    blk_b = sum(s.h2d_blocked_bytes for s in stats)
+    blk_e = sum(s.blocked_edges for s in stats)
+    if blk_e and 2 * blk_b >= sum(s.h2d_bytes for s in stats):
+        eb = blk_b / blk_e
     rounds_ms = sum(s.ms_rounds for s in stats)
     rounds_launches = sum(s.rounds_launches for s in stats)
     pair_ms = sum(s.ms_advance + s.ms_frontier for s in stats)
@@ -237,7 +249,8 @@ def roofline(stats, args):
         alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
     items = sum(s.work_items for s in stats)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    key = f"{args.algo}-{args.mode} rmat{args.scale}ef{args.edge_factor} budget{args.budget} learned"
+    key = (f"{args.algo}-{args.mode} rmat{args.scale}ef{args.edge_factor} budget{args.budget} learned"
+           + (" blocks" if args.blocks else ""))
     traffic = profiled_traffic(kern, key)
     return {
         "bound": "hbm",
@@ -253,7 +266,7 @@ def roofline(stats, args):
         "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
         "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
         "items_scanned_per_launch": int(items / max(k_launches, 1)),
-        "edge_bytes": eb,
+        "edge_bytes": round(eb, 4),
         "profile_key": key,
     }
 
@@ -458,6 +471,9 @@ def main():
     ap.add_argument("--order", default="in-degree", choices=["in-degree", "natural"],
                     help="vertex numbering the runs use: renumbered by descending in-degree on the GPU after the "
                          "build (results still in the original numbering), or the generator's own")
+    ap.add_argument("--blocks", type=int, default=1,
+                    help="1: build the block-compressed rows after the build (and renumbering) and move whole id "
+                         "ranges in them; 0: plain edges only")
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="launch the plain copy kernel once on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -497,6 +513,18 @@ def main():
         g.reorder()
         reorder_s = time.perf_counter() - t0
         log(f"renumbered by descending in-degree in {reorder_s:.1f} s")
+    blocks_s = block_bpe = None
+    if args.blocks and not args.batch:
+        t0 = time.perf_counter()
+        # a graph at the size where pinned memory runs short holds only its
+        # split arrays pinned; they move to ordinary memory before the block
+        # words are pinned (compacted units are then gathered on the host)
+        large = g.m * 5 >= 16 * GiB
+        g.build_blocks(unpin_split=large and args.weighted)
+        blocks_s = time.perf_counter() - t0
+        _, words, _, _ = g.blocks_view()
+        block_bpe = 4.0 * len(words) / max(g.m, 1)
+        log(f"block layout built in {blocks_s:.1f} s: {block_bpe:.3f} bytes per edge")
 
     if args.probe_copy_mb:
         ms = sw.probe_copy(args.probe_copy_mb << 20)
@@ -540,9 +568,18 @@ def main():
         reach = (vals != 4294967294) if args.algo != "sswp" else (vals != 0)
         deg = original_degrees(g)
         moved = int(deg[reach].sum(dtype=np.uint64))
-        bound_ms = moved * edge_bytes / (peak * 1e9) * 1e3
+        moved_bytes = moved * edge_bytes
+        bv = g.blocks_view() if args.blocks else None
+        if bv is not None:
+            # the rows of the reached vertices in the block layout
+            rowb = np.diff(bv[0]).astype(np.int64) * 4
+            o = g.order()
+            if o is not None:
+                rowb = rowb[o[0]]
+            moved_bytes = min(moved_bytes, int(rowb[reach].sum()))
+        bound_ms = moved_bytes / (peak * 1e9) * 1e3
         bound = {"edges_out_of_reached_vertices": moved, "fraction_of_edges": round(moved / g.m, 4),
-                 "edge_bytes": edge_bytes, "gb": round(moved * edge_bytes / 1e9, 3),
+                 "edge_bytes": edge_bytes, "gb": round(moved_bytes / 1e9, 3),
                  "ms_at_pinned_peak": round(bound_ms, 2),
                  "best_possible_speedup_over_plain": (round(results["plain"][0]["ms_per_step"] / bound_ms, 3)
                                                       if "plain" in results else None)}
@@ -576,6 +613,9 @@ def main():
             "graph_build_s": round(gen_s, 2),
             "vertex_order": args.order,
             "reorder_s": round(reorder_s, 2) if reorder_s is not None else None,
+            "blocks": bool(blocks_s is not None),
+            "blocks_s": round(blocks_s, 2) if blocks_s is not None else None,
+            "block_bytes_per_edge": round(block_bpe, 4) if block_bpe is not None else None,
         },
         "variants": {k: v[0] for k, v in results.items()},
         **speed,

@@ -50,7 +50,12 @@ struct HostCsr {
     // bytes per edge instead of 8), built on first use.
     uint32_t *splitDst = nullptr, *d_splitDst = nullptr;
     uint8_t *splitW = nullptr, *d_splitW = nullptr;
+    bool splitPinned = true;        // the split arrays are pinned and mapped (else ordinary memory, d_* null)
     bool ensureSplit();             // builds the split copy once; false when it does not apply
+    // Moves pinned split arrays into ordinary memory (the process's pinned
+    // memory is limited; a graph whose whole ranges move in the block layout
+    // then gathers its compacted units on the host from these arrays).
+    void demoteSplit();
     // After an unpinned read: split arrays, or the list moved into pinned memory
     void settleEdges(bool forceSplit);
     // Takes a pinned packed copy built elsewhere (a reader that never holds
@@ -113,9 +118,10 @@ void reorder_by_in_degree(HostCsr<E> &csr);
 // block-compressed copy (blk, blkPtr, blkCode).  Needs about 1.5 times the
 // graph's edge bytes of free device memory plus 13.5 GiB of sort scratch
 // (less for graphs under 2^29 edges).  Call it before any run or session
-// holds the graph.
+// holds the graph.  unpinSplit: move pinned split arrays to ordinary memory
+// before the block words are pinned (done anyway when pinning fails).
 template <class E>
-void build_blocks(HostCsr<E> &csr);
+void build_blocks(HostCsr<E> &csr, bool unpinSplit = false);
 
 // splitLayout: a .bwcsr graph that moves only in the split format (weights
 // must fit a byte); weighted lists of 16 GiB or more choose it by themselves

@@ -228,9 +228,14 @@ int32_t subway_graph_reorder(subway_graph *g, int32_t order);
  * out-edges keep their multiset).  The accessors (export, split_view, save)
  * see the sorted rows.  Call it before any session is created on the graph
  * (SUBWAY_ERR_ARG otherwise); needs about 1.5 times the graph's edge bytes
- * plus 13.5 GiB of free device memory.  No counterpart in the reference (its
- * transfers are the 8-byte edges, shared/subgraph_generator.cu:63-97). */
-int32_t subway_graph_build_blocks(subway_graph *g);
+ * plus 13.5 GiB of free device memory.  A process pins a limited amount of
+ * host memory: when the words cannot be pinned beside pinned split arrays,
+ * those arrays move to ordinary memory first (flag SUBWAY_BLOCKS_UNPIN_SPLIT
+ * asks for that up front); compacted units are then gathered by host
+ * threads.  No counterpart in the reference (its transfers are the 8-byte
+ * edges, shared/subgraph_generator.cu:63-97). */
+#define SUBWAY_BLOCKS_UNPIN_SPLIT 1
+int32_t subway_graph_build_blocks(subway_graph *g, int32_t flags);
 /* The block layout: row_ptr[n+1] (word offset of each row), the words
  * (pinned, num_words of them) and the weight code (bits | base << 6); valid
  * while the graph lives.  SUBWAY_ERR_ARG when the graph has none. */

@@ -236,14 +236,16 @@ class Graph:
             raise ValueError(f"order must be 'in-degree', not {order!r}")
         check(lib.subway_graph_reorder(self._h, 1))
 
-    def build_blocks(self) -> None:
+    def build_blocks(self, unpin_split: bool = False) -> None:
         """Sorts every row by destination and builds the block-compressed
         copy of the rows on the GPU, in place (subway_graph_build_blocks).
         Sessions then move whole id ranges in that layout unless created
         with blocks=False; every result is unchanged.  csr(), split_view()
-        and save() then see the sorted rows.  Call it before creating any
-        session on the graph."""
-        check(lib.subway_graph_build_blocks(self._h))
+        and save() then see the sorted rows.  unpin_split: move the split
+        arrays to ordinary memory (done by itself when the words cannot be
+        pinned beside them); compacted units are then gathered by host
+        threads.  Call it before creating any session on the graph."""
+        check(lib.subway_graph_build_blocks(self._h, 1 if unpin_split else 0))
 
     def blocks_view(self):
         """(row_ptr u64[n+1], words u32[num_words], weight_bits, weight_base)

@@ -54,7 +54,7 @@ _SIGS = {
     "subway_graph_split_view": (_i32, [_vp, ctypes.POINTER(_u64p), ctypes.POINTER(_u32p), ctypes.POINTER(_u8p)]),
     "subway_graph_reorder": (_i32, [_vp, _i32]),
     "subway_graph_order": (_i32, [_vp, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]),
-    "subway_graph_build_blocks": (_i32, [_vp]),
+    "subway_graph_build_blocks": (_i32, [_vp, _i32]),
     "subway_graph_blocks_view": (_i32, [_vp, ctypes.POINTER(_u64p), ctypes.POINTER(_u32p), _u64p, _u32p]),
     "subway_graph_save": (_i32, [_vp, ctypes.c_char_p]),
     "subway_graph_info": (_i32, [_vp, _u64p, _u64p, _i32p, _u32p, _u32p]),

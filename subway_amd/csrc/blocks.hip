@@ -30,6 +30,7 @@
 
 #include "blocks.hpp"
 #include "device.hpp"
+#include "host_util.hpp"
 #include "radix_sort.hpp"
 #include "subway/globals.hpp"
 #include "subway/gpu_error_check.hpp"
@@ -49,6 +50,58 @@ inline unsigned grid_of(u64 items)
 {
     return (unsigned)std::max<u64>(1, std::min<u64>((items + BB - 1) / BB, 1u << 16));
 }
+
+// Copies between the device and host edge arrays.  Pinned arrays are copied
+// directly; arrays in ordinary memory go through a pinned bounce buffer in
+// chunks, so that the runtime never has to lock existing host pages itself
+// (registering existing memory is refused on some hosts).
+class HostCopy {
+public:
+    explicit HostCopy(hipStream_t st) : st_(st) {}
+    ~HostCopy()
+    {
+        if (buf_) (void)hipHostFree(buf_);
+    }
+    void h2d(void *dev, const void *host, u64 bytes, bool pinned)
+    {
+        if (!bytes) return;
+        if (pinned) {
+            SUBWAY_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st_));
+            return;
+        }
+        bounce();
+        for (u64 a = 0; a < bytes; a += kBounce) {
+            const u64 n = std::min<u64>(kBounce, bytes - a);
+            parallel_for(n, [&](u64 b, u64 e, int) { std::memcpy(buf_ + b, (const char *)host + a + b, e - b); });
+            SUBWAY_CHECK(hipMemcpyAsync((char *)dev + a, buf_, n, hipMemcpyHostToDevice, st_));
+            SUBWAY_CHECK(hipStreamSynchronize(st_));
+        }
+    }
+    void d2h(void *host, const void *dev, u64 bytes, bool pinned)
+    {
+        if (!bytes) return;
+        if (pinned) {
+            SUBWAY_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st_));
+            return;
+        }
+        bounce();
+        for (u64 a = 0; a < bytes; a += kBounce) {
+            const u64 n = std::min<u64>(kBounce, bytes - a);
+            SUBWAY_CHECK(hipMemcpyAsync(buf_, (const char *)dev + a, n, hipMemcpyDeviceToHost, st_));
+            SUBWAY_CHECK(hipStreamSynchronize(st_));
+            parallel_for(n, [&](u64 b, u64 e, int) { std::memcpy((char *)host + a + b, buf_ + b, e - b); });
+        }
+    }
+
+private:
+    static constexpr u64 kBounce = 64ull << 20;
+    hipStream_t st_;
+    char *buf_ = nullptr;
+    void bounce()
+    {
+        if (!buf_) SUBWAY_CHECK(hipHostMalloc((void **)&buf_, kBounce, 0));
+    }
+};
 
 // last r in [lo, hi] with a[r] <= q (a[lo] <= q holds)
 __device__ __forceinline__ u64 last_le(const u64 *a, u64 lo, u64 hi, u64 q)
@@ -301,12 +354,11 @@ void write_back(HostCsr<E> &csr, const u32 *dst, const W *w, hipStream_t st)
     constexpr bool weighted = sizeof(E) == 8;
     const u64 m = csr.m;
     if (m == 0) return;
+    HostCopy hc(st);
     if constexpr (weighted) {
         if (csr.splitDst) {
-            SUBWAY_CHECK(hipMemcpyAsync(csr.splitDst, dst, m * sizeof(u32), hipMemcpyDeviceToHost, st));
-            if (sizeof(W) == 1) {
-                SUBWAY_CHECK(hipMemcpyAsync(csr.splitW, w, m, hipMemcpyDeviceToHost, st));
-            }
+            hc.d2h(csr.splitDst, dst, m * sizeof(u32), csr.splitPinned);
+            if (sizeof(W) == 1) hc.d2h(csr.splitW, w, m, csr.splitPinned);
         }
         const u64 CH = std::min<u64>(m, 1ull << 27);
         DevArr<uint2> stage(CH);
@@ -315,12 +367,12 @@ void write_back(HostCsr<E> &csr, const u32 *dst, const W *w, hipStream_t st)
             if (csr.splitDst && sizeof(W) != 1) {
                 hipLaunchKernelGGL((narrow_kernel<W>), dim3(grid_of(cnt)), dim3(BB), 0, st, w, e0, cnt,
                                    reinterpret_cast<uint8_t *>(stage.p));
-                SUBWAY_CHECK(hipMemcpyAsync(csr.splitW + e0, stage.p, cnt, hipMemcpyDeviceToHost, st));
+                hc.d2h(csr.splitW + e0, stage.p, cnt, csr.splitPinned);
                 SUBWAY_CHECK(hipStreamSynchronize(st));
             }
             if (csr.edges) {
                 hipLaunchKernelGGL((zip_kernel<W>), dim3(grid_of(cnt)), dim3(BB), 0, st, dst, w, e0, cnt, stage.p);
-                SUBWAY_CHECK(hipMemcpyAsync(csr.edges + e0, stage.p, cnt * sizeof(E), hipMemcpyDeviceToHost, st));
+                hc.d2h(csr.edges + e0, stage.p, cnt * sizeof(E), csr.pinned);
                 SUBWAY_CHECK(hipStreamSynchronize(st));
             }
             if (csr.packed) {
@@ -333,17 +385,18 @@ void write_back(HostCsr<E> &csr, const u32 *dst, const W *w, hipStream_t st)
         }
     } else {
         (void)w;
-        SUBWAY_CHECK(hipMemcpyAsync(csr.edges, dst, m * sizeof(u32), hipMemcpyDeviceToHost, st));
+        hc.d2h(csr.edges, dst, m * sizeof(u32), csr.pinned);
     }
     SUBWAY_CHECK(hipStreamSynchronize(st));
 }
 
 template <class E, class W>
-void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream_t st)
+void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, bool unpinSplit, hipStream_t st)
 {
     const u64 n = csr.n, m = csr.m;
+    HostCopy hc(st);
     DevArr<u64> dptr(n + 1);
-    SUBWAY_CHECK(hipMemcpyAsync(dptr.p, csr.ptr, (n + 1) * sizeof(u64), hipMemcpyHostToDevice, st));
+    hc.h2d(dptr.p, csr.ptr, (n + 1) * sizeof(u64), false);
 
     // 1. weight code
     u32 wbits = 0, wbase = 0;
@@ -373,8 +426,14 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream
     DevArr<u64> bidx(n + 1);
     scan_counts(nb.p, n + 1, bidx.p, st);
     nb.reset();
+    // the build's stream is non-blocking: every read-back goes through it
+    // and waits for it (a plain hipMemcpy would not wait for its kernels)
+    auto readBack = [&](void *host, const void *dev, size_t bytes) {
+        SUBWAY_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+        SUBWAY_CHECK(hipStreamSynchronize(st));
+    };
     u64 B = 0;
-    SUBWAY_CHECK(hipMemcpy(&B, bidx.p + n, sizeof(u64), hipMemcpyDeviceToHost));
+    readBack(&B, bidx.p + n, sizeof(u64));
     DevArr<u64> rowPtr(n + 1);
     u64 T = 0;
     DevArr<u32> words;
@@ -390,7 +449,7 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream
         pw.reset();
         hipLaunchKernelGGL(row_ptr_kernel, dim3(grid_of(n + 1)), dim3(BB), 0, st, bidx.p, pay.p, n, rowPtr.p);
         SUBWAY_CHECK(hipGetLastError());
-        SUBWAY_CHECK(hipMemcpy(&T, rowPtr.p + n, sizeof(u64), hipMemcpyDeviceToHost));
+        readBack(&T, rowPtr.p + n, sizeof(u64));
 
         // 5. descriptors and payloads
         words.alloc(T);
@@ -400,15 +459,32 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream
                            dst.p, hasW ? (const W *)w.p : (const W *)nullptr, wbits, wbase, words.p, err.p);
         SUBWAY_CHECK(hipGetLastError());
         u32 e = 0;
-        SUBWAY_CHECK(hipMemcpy(&e, err.p, sizeof(u32), hipMemcpyDeviceToHost));
+        readBack(&e, err.p, sizeof(u32));
         if (e) throw Error(ERR_ARG, "a row is too large for the block layout (2^26 payload words per row)");
     }
 
-    // 6. to the host
+    // 6. to the host; when the process cannot pin the words beside pinned
+    // split arrays (a host pins about 70 GB per process), the split arrays
+    // move to ordinary memory first: the blocks then carry the whole ranges
+    // and compacted units are gathered on the host
+    (void)unpinSplit;
     u32 *hw = nullptr;
-    if (hipHostMalloc((void **)&hw, (size_t)(T ? T : 1) * sizeof(u32), hipHostMallocPortable) != hipSuccess) {
+    if (hipHostMalloc((void **)&hw, (size_t)(T ? T : 1) * sizeof(u32), hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
         (void)hipGetLastError();
-        throw Error(ERR_NOMEM, "pinned host allocation of " + std::to_string(T * 4) + " bytes of blocks failed");
+        hw = nullptr;
+        if (csr.splitDst && csr.splitPinned) {
+            csr.demoteSplit();
+            // the runtime releases freed pinned ranges once the device is idle;
+            // an allocation made before that gave host pointers that copies
+            // then refused ("invalid argument")
+            SUBWAY_CHECK(hipDeviceSynchronize());
+            if (hipHostMalloc((void **)&hw, (size_t)(T ? T : 1) * sizeof(u32), hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
+                (void)hipGetLastError();
+                hw = nullptr;
+            }
+        }
+        if (!hw)
+            throw Error(ERR_NOMEM, "pinned host allocation of " + std::to_string(T * 4) + " bytes of blocks failed");
     }
     ull *hp = nullptr;
     try {
@@ -417,14 +493,14 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream
         (void)hipHostFree(hw);
         throw Error(ERR_NOMEM, "host allocation of the block row offsets failed");
     }
-    hipError_t e = hipSuccess;
-    if (T) e = hipMemcpyAsync(hw, words.p, T * sizeof(u32), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hp, rowPtr.p, (n + 1) * sizeof(u64), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
+    try {
+        hc.d2h(hw, words.p, T * sizeof(u32), true);
+        hc.d2h(hp, rowPtr.p, (n + 1) * sizeof(u64), false);
+        SUBWAY_CHECK(hipStreamSynchronize(st));
+    } catch (...) {
         (void)hipHostFree(hw);
         delete[] hp;
-        SUBWAY_CHECK(e);
+        throw;
     }
     csr.blk = hw;
     csr.blkPtr = hp;
@@ -435,7 +511,7 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, hipStream
 }  // namespace
 
 template <class E>
-void build_blocks(HostCsr<E> &csr)
+void build_blocks(HostCsr<E> &csr, bool unpinSplit)
 {
     constexpr bool weighted = sizeof(E) == 8;
     if (csr.blk) throw Error(ERR_ARG, "the graph already holds its block layout");
@@ -448,34 +524,42 @@ void build_blocks(HostCsr<E> &csr)
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{st};
+    // split arrays asked to leave pinned memory leave it first: the uploads and
+    // write-backs below then go through the bounce buffer, and the device has
+    // long been idle when the block words are pinned
+    if (unpinSplit && csr.splitDst && csr.splitPinned) {
+        csr.demoteSplit();
+        SUBWAY_CHECK(hipDeviceSynchronize());
+    }
     DevArr<u32> dst(m);
+    HostCopy hc(st);
     if constexpr (weighted) {
         if (csr.splitDst) {
             DevArr<uint8_t> w(m);
-            if (m) {
-                SUBWAY_CHECK(hipMemcpyAsync(dst.p, csr.splitDst, m * sizeof(u32), hipMemcpyHostToDevice, st));
-                SUBWAY_CHECK(hipMemcpyAsync(w.p, csr.splitW, m, hipMemcpyHostToDevice, st));
-            }
-            build<E, uint8_t>(csr, dst, w, true, st);
+            hc.h2d(dst.p, csr.splitDst, m * sizeof(u32), csr.splitPinned);
+            hc.h2d(w.p, csr.splitW, m, csr.splitPinned);
+            SUBWAY_CHECK(hipStreamSynchronize(st));
+            build<E, uint8_t>(csr, dst, w, true, unpinSplit, st);
         } else {
             DevArr<u32> w(m);
             if (m) {
                 DevArr<uint2> list(m);
-                SUBWAY_CHECK(hipMemcpyAsync(list.p, csr.edges, m * sizeof(E), hipMemcpyHostToDevice, st));
+                hc.h2d(list.p, csr.edges, m * sizeof(E), csr.pinned);
                 hipLaunchKernelGGL(unzip_kernel, dim3(grid_of(m)), dim3(BB), 0, st, list.p, m, dst.p, w.p);
                 SUBWAY_CHECK(hipGetLastError());
                 SUBWAY_CHECK(hipStreamSynchronize(st));
             }
-            build<E, u32>(csr, dst, w, true, st);
+            build<E, u32>(csr, dst, w, true, unpinSplit, st);
         }
     } else {
-        if (m) SUBWAY_CHECK(hipMemcpyAsync(dst.p, csr.edges, m * sizeof(u32), hipMemcpyHostToDevice, st));
+        hc.h2d(dst.p, csr.edges, m * sizeof(u32), csr.pinned);
+        SUBWAY_CHECK(hipStreamSynchronize(st));
         DevArr<uint8_t> none;
-        build<E, uint8_t>(csr, dst, none, false, st);
+        build<E, uint8_t>(csr, dst, none, false, unpinSplit, st);
     }
 }
 
-template void build_blocks<OutEdge>(HostCsr<OutEdge> &);
-template void build_blocks<OutEdgeWeighted>(HostCsr<OutEdgeWeighted> &);
+template void build_blocks<OutEdge>(HostCsr<OutEdge> &, bool);
+template void build_blocks<OutEdgeWeighted>(HostCsr<OutEdgeWeighted> &, bool);
 
 }  // namespace subway

@@ -199,14 +199,16 @@ int32_t subway_graph_reorder(subway_graph *g, int32_t order)
     });
 }
 
-int32_t subway_graph_build_blocks(subway_graph *g)
+int32_t subway_graph_build_blocks(subway_graph *g, int32_t flags)
 {
     return guard([&] {
         if (!g) throw Error(ERR_ARG, "null graph");
+        if (flags & ~SUBWAY_BLOCKS_UNPIN_SPLIT) throw Error(ERR_ARG, "unknown block layout flags");
         const long users = g->weighted ? g->w.use_count() : g->u.use_count();
         if (users > 1) throw Error(ERR_ARG, "build the block layout before any session is created on the graph");
-        if (g->weighted) build_blocks(*g->w);
-        else build_blocks(*g->u);
+        const bool unpin = (flags & SUBWAY_BLOCKS_UNPIN_SPLIT) != 0;
+        if (g->weighted) build_blocks(*g->w, unpin);
+        else build_blocks(*g->u, unpin);
     });
 }
 

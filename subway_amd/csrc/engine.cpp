@@ -241,6 +241,7 @@ public:
             }
         }
         bufBytes_ = (ull)sub_->max_partition_size * sizeof(E);
+        edgesMapped_ = fmt_ == EDGE_SPLIT ? g_->csr->d_splitDst != nullptr : true;
         if (g_->csr->blk && !o_.noBlocks && !o_.batch && g_->num_edges > 0) {
             // whole id ranges move in the block layout; the frontier finds a
             // row's words through the graph's row offsets
@@ -249,6 +250,7 @@ public:
             const size_t bytes = ((size_t)g_->num_nodes + 1) * sizeof(ull);
             SUBWAY_CHECK(hipMalloc(&d_blk_, bytes));
             SUBWAY_CHECK(hipMemcpy(d_blk_, g_->csr->blkPtr, bytes, hipMemcpyHostToDevice));
+            SUBWAY_CHECK(hipDeviceSynchronize());   // a pageable copy may return before its DMA ends
         }
         // a graph that fits one buffer moves once per run and stays there
         resident_ = !o_.noResident && !o_.batch && g_->num_edges > 0 && g_->num_edges <= maxPart_;
@@ -396,6 +398,7 @@ public:
             (void)hipEventDestroy(evLoaded_[b]);
             (void)hipEventDestroy(evFree_[b]);
             (void)hipEventDestroy(evStaged_[b]);
+            if (stage_[b]) (void)hipHostFree(stage_[b]);
         }
         if (h_ids_) (void)hipHostFree(h_ids_);
         (void)hipEventDestroy(evX0_);
@@ -448,6 +451,10 @@ private:
     uint32_t blkCode_ = 0;
     ull bufBytes_ = 0;
     bool bufBlocked_[2] = {false, false};
+    // the GPU can read the host edges of the transfer format (mapped pinned
+    // memory); otherwise compacted units are gathered by host threads
+    bool edgesMapped_ = true;
+    bool blkDiag_ = false;
     void markBlocked(Unit &u) const;
     int fmtOf(const Unit &u) const { return u.blocked ? (int)EDGE_BLOCKS : fmt_; }
     uint32_t codeOf(const Unit &u) const { return u.blocked ? blkCode_ : packCode_; }
@@ -484,9 +491,16 @@ private:
     void frontier(int first, uint8_t *labA, uint8_t *labB, const Unit &un);
     void advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf, const Unit &un);
     void asyncRounds(const Unit &un, int buf);
-    // gathers the unit's active adjacency, elements of esz bytes from src,
-    // into buffer b's staging area at byte stageOffset
-    void hostGather(const Unit &u, int b, const char *src, size_t esz, size_t stageOffset);
+    // gathers the unit's active adjacency, elements of esz bytes from src, into
+    // the device array dstDev, through the two pinned staging slots
+    void hostGather(const Unit &u, const char *src, size_t esz, char *dstDev);
+    // copies a contiguous host range in ordinary memory to the device through the slots
+    void stagedCopy(const char *host, char *dstDev, size_t bytes);
+    char *stageSlot(int &s);
+    static constexpr size_t kStageBytes = 64ull << 20;
+    char *stage_[2] = {nullptr, nullptr};
+    int stageSlot_ = 0;
+    std::vector<ull> gatherPre_;
     // host edge array in the transfer format (pinned)
     const char *hostEdges() const
     {
@@ -1087,14 +1101,36 @@ void Engine<E, G>::issue(size_t idx)
         char *dstW = split ? dst + 4ull * maxPart_ : nullptr;
         if (u.blocked) {
             const size_t bb = (size_t)u.blkWords * 4;
-            SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->blk + u.blkLo, bb, hipMemcpyHostToDevice, sx_));
+            const hipError_t ce = hipMemcpyAsync(dst, g_->csr->blk + u.blkLo, bb, hipMemcpyHostToDevice, sx_);
+            if (ce != hipSuccess) {
+                (void)hipGetLastError();
+                if (!blkDiag_) {
+                    blkDiag_ = true;
+                    size_t sz = 0;
+                    const hipError_t ie = hipMemPtrGetInfo(g_->csr->blk, &sz);
+                    hipPointerAttribute_t at{};
+                    const hipError_t pe = hipPointerGetAttributes(&at, g_->csr->blk + u.blkLo);
+                    std::fprintf(stderr,
+                                 "subway: block copy refused (%s): words [%llu, +%llu) of %llu; allocation %s size %zu; "
+                                 "pointer %s type %d host %p device %p\n",
+                                 hipGetErrorString(ce), (unsigned long long)u.blkLo, (unsigned long long)u.blkWords,
+                                 (unsigned long long)g_->csr->blkWords, hipGetErrorString(ie), sz,
+                                 hipGetErrorString(pe), (int)at.type, at.hostPointer, at.devicePointer);
+                    (void)hipGetLastError();
+                }
+                SUBWAY_CHECK(ce);
+            }
             st_.h2dDmaBytes += bb;
             st_.h2dBlockedBytes += bb;
             st_.blockedEdges += u.count;
             st_.blockedUnits++;
             st_.h2dBytes += bb;
         } else if (u.dma) {
-            if (split) {
+            if (split && !edgesMapped_) {
+                // split arrays in ordinary memory: through the pinned staging slots
+                stagedCopy((const char *)(g_->csr->splitDst + u.hostEdge), dst, 4ull * u.count);
+                stagedCopy((const char *)(g_->csr->splitW + u.hostEdge), dstW, u.count);
+            } else if (split) {
                 SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->splitDst + u.hostEdge, 4ull * u.count,
                                             hipMemcpyHostToDevice, sx_));
                 SUBWAY_CHECK(hipMemcpyAsync(dstW, g_->csr->splitW + u.hostEdge, u.count, hipMemcpyHostToDevice,
@@ -1104,18 +1140,13 @@ void Engine<E, G>::issue(size_t idx)
                                             hipMemcpyHostToDevice, sx_));
             }
             st_.h2dDmaBytes += bytes;
-        } else if (o_.transfer == TRANSFER_HOST) {
-            const char *stage = (const char *)sub_->stagingBuffer(b);
+        } else if (o_.transfer == TRANSFER_HOST || !edgesMapped_) {
             if (split) {
-                hostGather(u, b, (const char *)g_->csr->splitDst, 4, 0);
-                hostGather(u, b, (const char *)g_->csr->splitW, 1, 4ull * u.count);
-                SUBWAY_CHECK(hipMemcpyAsync(dst, stage, 4ull * u.count, hipMemcpyHostToDevice, sx_));
-                SUBWAY_CHECK(hipMemcpyAsync(dstW, stage + 4ull * u.count, u.count, hipMemcpyHostToDevice, sx_));
+                hostGather(u, (const char *)g_->csr->splitDst, 4, dst);
+                hostGather(u, (const char *)g_->csr->splitW, 1, dstW);
             } else {
-                hostGather(u, b, hostEdges(), (size_t)eb_, 0);
-                SUBWAY_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, sx_));
+                hostGather(u, hostEdges(), (size_t)eb_, dst);
             }
-            SUBWAY_CHECK(hipEventRecord(evStaged_[b], sx_));
             st_.h2dDmaBytes += bytes;
         } else {
             if (split) {
@@ -1139,11 +1170,13 @@ void Engine<E, G>::issue(size_t idx)
 
 // Host-thread gather (the reference's dynamic<E>, subgraph_generator.cu:63-97,
 // without its tail bug): the partition's active ids are mirrored once per
-// global iteration; chunks of the active range are claimed by host threads,
-// whose destination offsets come from a parallel prefix of chunk edge sums;
-// the result lands in the pinned staging buffer of device buffer b.
+// global iteration; host threads compute the unit's row offsets (chunk sums,
+// then a prefix inside each chunk), then copy the rows chunk by chunk into
+// one of two pinned staging slots of kStageBytes, each chunk followed by its
+// DMA to dstDev on the copy stream: the threads fill one slot while the other
+// one's copy runs, and the pinned memory a session holds stays small.
 template <class E, class G>
-void Engine<E, G>::hostGather(const Unit &u, int b, const char *src, size_t esz, size_t stageOffset)
+void Engine<E, G>::hostGather(const Unit &u, const char *src, size_t esz, char *dstDev)
 {
     if (!hostIdsValid_) {
         const ull A = sub_->numActiveNodes;
@@ -1151,14 +1184,13 @@ void Engine<E, G>::hostGather(const Unit &u, int b, const char *src, size_t esz,
         SUBWAY_CHECK(hipStreamSynchronize(sc_));
         hostIdsValid_ = true;
     }
-    // the staging buffer is reused only after the copy out of it finished
-    SUBWAY_CHECK(hipEventSynchronize(evStaged_[b]));
-    char *stage = (char *)sub_->stagingBuffer(b) + stageOffset;
     const uint *ids = h_ids_ + u.kbeg;
     const ull cnt = u.kend - u.kbeg;
     const ull *np = g_->nodePointer;
     const int T = std::max(1, std::min(host_threads(), 64));
-    const ull nch = std::min<ull>(cnt, (ull)T * 8);
+    std::vector<ull> &pre = gatherPre_;
+    pre.resize(cnt + 1);
+    const ull nch = std::max<ull>(1, std::min<ull>(cnt, (ull)T * 8));
     std::vector<ull> chunkEdges(nch + 1, 0);
     parallel_for(nch, [&](ull c0, ull c1, int) {
         for (ull c = c0; c < c1; c++) {
@@ -1173,12 +1205,58 @@ void Engine<E, G>::hostGather(const Unit &u, int b, const char *src, size_t esz,
         for (ull c = c0; c < c1; c++) {
             ull v0 = cnt * c / nch, v1 = cnt * (c + 1) / nch, at = chunkEdges[c];
             for (ull i = v0; i < v1; i++) {
-                const ull a = np[ids[i]], d = np[ids[i] + 1] - a;
-                std::memcpy(stage + at * esz, src + a * esz, d * esz);
-                at += d;
+                pre[i] = at;
+                at += np[ids[i] + 1] - np[ids[i]];
             }
         }
     }, T);
+    pre[cnt] = u.count;
+    const ull per = kStageBytes / esz;
+    for (ull a = 0; a < u.count; a += per) {
+        const ull e1 = std::min<ull>(u.count, a + per);
+        int s = 0;
+        char *stage = stageSlot(s);
+        parallel_for(e1 - a, [&](ull x0, ull x1, int) {
+            ull e = a + x0;
+            const ull end = a + x1;
+            // the row holding edge e: the last one that starts at or before it
+            ull k = (ull)(std::upper_bound(pre.begin(), pre.begin() + cnt + 1, e) - pre.begin()) - 1;
+            while (e < end) {
+                const ull take = std::min(pre[k + 1] - e, end - e);
+                if (take) std::memcpy(stage + (e - a) * esz, src + (np[ids[k]] + (e - pre[k])) * esz, take * esz);
+                e += take;
+                k++;
+            }
+        }, T);
+        SUBWAY_CHECK(hipMemcpyAsync(dstDev + a * esz, stage, (e1 - a) * esz, hipMemcpyHostToDevice, sx_));
+        SUBWAY_CHECK(hipEventRecord(evStaged_[s], sx_));
+    }
+}
+
+// The next of the two pinned staging slots, once its previous copy finished.
+template <class E, class G>
+char *Engine<E, G>::stageSlot(int &s)
+{
+    if (!stage_[0]) {
+        for (int k = 0; k < 2; k++) SUBWAY_CHECK(hipHostMalloc((void **)&stage_[k], kStageBytes, 0));
+    }
+    s = stageSlot_;
+    stageSlot_ ^= 1;
+    SUBWAY_CHECK(hipEventSynchronize(evStaged_[s]));
+    return stage_[s];
+}
+
+template <class E, class G>
+void Engine<E, G>::stagedCopy(const char *host, char *dstDev, size_t bytes)
+{
+    for (size_t a = 0; a < bytes; a += kStageBytes) {
+        const size_t n = std::min<size_t>(kStageBytes, bytes - a);
+        int s = 0;
+        char *stage = stageSlot(s);
+        parallel_for(n, [&](ull b, ull e, int) { std::memcpy(stage + b, host + a + b, e - b); });
+        SUBWAY_CHECK(hipMemcpyAsync(dstDev + a, stage, n, hipMemcpyHostToDevice, sx_));
+        SUBWAY_CHECK(hipEventRecord(evStaged_[s], sx_));
+    }
 }
 
 template <class E, class G>

@@ -54,8 +54,13 @@ HostCsr<E>::~HostCsr()
         else std::free(edges);
     }
     if (packed) (void)hipHostFree(packed);
-    if (splitDst) (void)hipHostFree(splitDst);
-    if (splitW) (void)hipHostFree(splitW);
+    if (splitPinned) {
+        if (splitDst) (void)hipHostFree(splitDst);
+        if (splitW) (void)hipHostFree(splitW);
+    } else {
+        std::free(splitDst);
+        std::free(splitW);
+    }
     delete[] perm;
     delete[] iperm;
     if (blk) (void)hipHostFree(blk);
@@ -150,6 +155,32 @@ bool HostCsr<E>::ensureSplit()
         SUBWAY_CHECK(hipHostGetDevicePointer((void **)&d_splitW, splitW, 0));
         return true;
     }
+}
+
+template <class E>
+void HostCsr<E>::demoteSplit()
+{
+    std::lock_guard<std::mutex> lock(packMu_);
+    if (!splitDst || !splitPinned) return;
+    const size_t mm = (size_t)(m ? m : 1);
+    uint32_t *d = static_cast<uint32_t *>(std::malloc(mm * sizeof(uint32_t)));
+    uint8_t *w = static_cast<uint8_t *>(std::malloc(mm));
+    if (!d || !w) {
+        std::free(d);
+        std::free(w);
+        throw Error(ERR_NOMEM, "host allocation for the split edge arrays failed");
+    }
+    parallel_for(m, [&](uint64_t b, uint64_t e, int) {
+        std::memcpy(d + b, splitDst + b, (e - b) * sizeof(uint32_t));
+        std::memcpy(w + b, splitW + b, e - b);
+    });
+    (void)hipHostFree(splitDst);
+    (void)hipHostFree(splitW);
+    splitDst = d;
+    splitW = w;
+    d_splitDst = nullptr;
+    d_splitW = nullptr;
+    splitPinned = false;
 }
 
 template <class E>

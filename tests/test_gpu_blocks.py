@@ -193,3 +193,35 @@ def test_blocks_on_renumbered_scale18_match_plain(sw, gpu):
     ptr, adj, w = sw.Graph.rmat(18, 16, seed=42, weighted=True).csr()
     assert np.array_equal(vb, O.sssp(ptr, adj, w, src))
     assert sb.blocked_units > 0 and sb.h2d_bytes < 0.8 * sp.h2d_bytes
+
+
+@pytest.mark.parametrize("algo,mode", [("sssp", "async"), ("sswp", "async"), ("sssp", "sync"), ("bfs", "async")])
+def test_unpinned_split_arrays_gather_on_the_host(sw, gpu, algo, mode):
+    """A split-layout graph whose split arrays moved to ordinary memory (as a
+    graph at the pinned-memory limit does by itself): whole ranges move as
+    blocks, compacted units and hub chunks through host gathers / pageable
+    DMA, with the oracle's answers."""
+    rng = np.random.default_rng(zlib.crc32(f"{algo}{mode}".encode()))
+    n, m = 20000, 160000
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    src = np.concatenate([src, np.full(30000, 17, np.uint32)])
+    dst = np.concatenate([dst, rng.integers(0, n, 30000).astype(np.uint32)])
+    w = rng.integers(1, 65, len(src)).astype(np.uint32)
+    g0 = sw.Graph.from_edges(n, src, dst, w)
+    ptr, adj, ww = g0.csr()
+    exp = O.expected(algo, ptr, adj, ww, 17)
+    path = None
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "g.bwcsr")
+        g0.save(path)
+        g = sw.Graph.load(path, split_layout=True)
+    g.split_view()                          # the split arrays exist
+    g.build_blocks(unpin_split=True)
+    for transfer in ("auto", "host"):
+        v, st = sw.run(g, algo, mode=mode, source=17, budget=64 << 10, transfer=transfer)
+        check(algo, v, exp)
+        assert st.edge_bytes == 5
+        if mode == "async":
+            assert st.blocked_units > 0
