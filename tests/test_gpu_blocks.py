@@ -225,3 +225,28 @@ def test_unpinned_split_arrays_gather_on_the_host(sw, gpu, algo, mode):
         assert st.edge_bytes == 5
         if mode == "async":
             assert st.blocked_units > 0
+
+
+def test_row_offsets_end_at_the_word_count(sw, gpu):
+    """Regression: the word count is read back only after the kernel that
+    computes the row offsets has finished (a read that did not wait for the
+    build's stream sized the words short).  Scale 22: offsets are monotone,
+    the last equals the word count, and decoding sample rows gives the
+    sorted CSR."""
+    g = sw.Graph.rmat(22, 16, seed=42, weighted=True)
+    g.reorder()
+    g.build_blocks()
+    rp, words, wbits, wbase = g.blocks_view()
+    assert int(rp[-1]) == len(words)
+    assert np.all(np.diff(rp.astype(np.int64)) >= 0)
+    ptr, ends, w = g.csr()
+    deg = np.diff(ptr.astype(np.int64))
+    rows = np.concatenate([np.argsort(-deg)[:3], np.arange(g.n - 5, g.n), [g.n // 2, g.n // 3]])
+    for v in rows:
+        v = int(v)
+        a, b = int(ptr[v]), int(ptr[v + 1])
+        r0, r1 = int(rp[v]), int(rp[v + 1])
+        sub_ptr = np.array([0, b - a], np.uint64)
+        sub_rp = np.array([0, r1 - r0], np.uint64)
+        de, dw = B.decode(sub_rp, words[r0:r1], sub_ptr, wbits, wbase, True)
+        assert np.array_equal(de, ends[a:b]) and np.array_equal(dw, w[a:b]), v
