@@ -219,39 +219,39 @@ def roofline(stats, args):
     residual read and clear 16 + running-sum RMW 16) and s(E) + 16 B per
     edge (FP64 atomic add RMW).  The frontier's scan of unlabelled items is
     reported apart (items_scanned, at the bytes each scanned item reads) and
-    not counted.  Partition visits run either as one persistent rounds_kernel
-    launch or as stand-alone frontier + advance launch pairs; the kind with
-    more kernel time is reported, priced with its own work only.  Units moved
-    in the block layout hold an edge in fewer bytes: when they carry most of
-    the bytes moved, s(E) is their bytes per edge."""
+    not counted.  Three kinds of launches are told apart, each priced with
+    its own time and work: persistent launches over units in the block
+    layout (rounds_kernel<BlockEdge>, s(E) = the blocks' bytes per edge),
+    persistent launches over plain units (rounds_kernel, s(E) = the plain
+    edge size), and stand-alone frontier + advance launch pairs; the kind
+    with the most kernel time is reported."""
+    S = lambda f: sum(getattr(s, f) for s in stats)
     eb = stats[-1].edge_bytes
-    blk_b = sum(s.h2d_blocked_bytes for s in stats)
-    blk_e = sum(s.blocked_edges for s in stats)
-    if blk_e and 2 * blk_b >= sum(s.h2d_bytes for s in stats):
-        eb = blk_b / blk_e
-    rounds_ms = sum(s.ms_rounds for s in stats)
-    rounds_launches = sum(s.rounds_launches for s in stats)
-    pair_ms = sum(s.ms_advance + s.ms_frontier for s in stats)
-    adv_launches = sum(s.advance_launches for s in stats)
-    if rounds_launches and rounds_ms >= pair_ms:
-        kern, k_ms, k_launches = "rounds_kernel", rounds_ms, rounds_launches
-        work = sum(s.work_edges - s.pair_work_edges for s in stats)
-        upd = sum(s.updates - s.pair_updates for s in stats)
-        ents = sum(s.work_entries - s.pair_work_entries for s in stats)
-    else:
-        kern, k_ms, k_launches = "advance_kernel+frontier_kernel", pair_ms, adv_launches
-        work = sum(s.pair_work_edges for s in stats)
-        upd = sum(s.pair_updates for s in stats)
-        ents = sum(s.pair_work_entries for s in stats)
+    blk_e = S("blocked_edges")
+    eb_blk = S("h2d_blocked_bytes") / blk_e if blk_e else eb
+    blk_ms, blk_n = S("ms_rounds_blocked"), S("rounds_launches_blocked")
+    pair_w, pair_n_ent, pair_upd = S("pair_work_edges"), S("pair_work_entries"), S("pair_updates")
+    blk_w, blk_ent, blk_upd = S("blk_work_edges"), S("blk_work_entries"), S("blk_updates")
+    kinds = {
+        "rounds_kernel<BlockEdge>": (blk_ms, blk_n, blk_w, blk_ent, blk_upd, eb_blk),
+        "rounds_kernel": (S("ms_rounds") - blk_ms, S("rounds_launches") - blk_n,
+                          S("work_edges") - pair_w - blk_w, S("work_entries") - pair_n_ent - blk_ent,
+                          S("updates") - pair_upd - blk_upd, eb),
+        "advance_kernel+frontier_kernel": (S("ms_advance") + S("ms_frontier"), S("advance_launches"), pair_w,
+                                           pair_n_ent, pair_upd, eb),
+    }
+    kern = max(kinds, key=lambda k: kinds[k][0] if kinds[k][1] else -1.0)
+    k_ms, k_launches, work, ents, upd, seb = kinds[kern]
     if args.algo == "pr":
-        alg_bytes = 48 * ents + (eb + 16) * work
+        alg_bytes = 48 * ents + (seb + 16) * work
     else:
-        alg_bytes = 22 * ents + (eb + 4) * work + 9 * upd
-    items = sum(s.work_items for s in stats)
+        alg_bytes = 22 * ents + (seb + 4) * work + 9 * upd
+    items = S("work_items")
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     key = (f"{args.algo}-{args.mode} rmat{args.scale}ef{args.edge_factor} budget{args.budget} learned"
            + (" blocks" if args.blocks else ""))
     traffic = profiled_traffic(kern, key)
+    per_launch = alg_bytes / max(k_launches, 1)
     return {
         "bound": "hbm",
         "kernel": kern,
@@ -260,14 +260,14 @@ def roofline(stats, args):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "traffic_over_algorithmic": (round(traffic / (alg_bytes / max(k_launches, 1)), 2)
-                                     if traffic and alg_bytes else None),
+        "traffic_over_algorithmic": round(traffic / per_launch, 2) if traffic and alg_bytes else None,
         "launches": k_launches,
         "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
-        "algorithmic_bytes_per_launch": int(alg_bytes / max(k_launches, 1)),
-        "items_scanned_per_launch": int(items / max(k_launches, 1)),
-        "edge_bytes": round(eb, 4),
+        "algorithmic_bytes_per_launch": int(per_launch),
+        "items_scanned_per_launch": int(items / max(S("rounds_launches"), 1)),
+        "edge_bytes": round(seb, 4),
         "profile_key": key,
+        "by_kind_ms": {k: round(v[0], 3) for k, v in kinds.items()},
     }
 
 

@@ -111,6 +111,11 @@ struct RunStats {
     ull h2dDmaBytes = 0;       // ... of which by DMA
     ull h2dGatherBytes = 0;    // ... of which by GPU gather reads
     ull h2dBlockedBytes = 0;   // ... of which in the block layout
+    double msRoundsBlocked = 0;   // persistent rounds launches over block units: time (profile only), count,
+    ull roundsLaunchesBlocked = 0;  // and their share of workEdges, workEntries and updates
+    ull blkWorkEdges = 0;
+    ull blkWorkEntries = 0;
+    ull blkUpdates = 0;
     ull blockedEdges = 0;      // edges of the units moved in the block layout
     uint blockedUnits = 0;     // units moved in the block layout
     uint globalIterations = 0;

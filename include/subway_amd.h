@@ -134,6 +134,13 @@ typedef struct subway_stats {
     uint64_t pull_found;
     uint64_t h2d_blocked_bytes;
     uint64_t blocked_edges;
+    /* persistent rounds launches over block units: their time (profile = 1),
+     * count, and share of work_edges, work_entries and updates */
+    double ms_rounds_blocked;
+    uint64_t rounds_launches_blocked;
+    uint64_t blk_work_edges;
+    uint64_t blk_work_entries;
+    uint64_t blk_updates;
 } subway_stats;
 
 typedef struct subway_iter_record {

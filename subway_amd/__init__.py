@@ -74,6 +74,8 @@ class _Stats(ctypes.Structure):
         ("resident_visits", ctypes.c_uint32), ("resident_overlaps", ctypes.c_uint32),
         ("pull_rounds", ctypes.c_uint32), ("blocked_units", ctypes.c_uint32), ("pull_found", ctypes.c_uint64),
         ("h2d_blocked_bytes", ctypes.c_uint64), ("blocked_edges", ctypes.c_uint64),
+        ("ms_rounds_blocked", ctypes.c_double), ("rounds_launches_blocked", ctypes.c_uint64),
+        ("blk_work_edges", ctypes.c_uint64), ("blk_work_entries", ctypes.c_uint64), ("blk_updates", ctypes.c_uint64),
     ]
 
 
@@ -364,6 +366,11 @@ class RunStats:
     pull_found: int = 0
     h2d_blocked_bytes: int = 0
     blocked_edges: int = 0
+    ms_rounds_blocked: float = 0.0
+    rounds_launches_blocked: int = 0
+    blk_work_edges: int = 0
+    blk_work_entries: int = 0
+    blk_updates: int = 0
     trace: list = field(default_factory=list)
 
 

@@ -408,6 +408,11 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->blocked_units = s.blockedUnits;
     stats->h2d_blocked_bytes = s.h2dBlockedBytes;
     stats->blocked_edges = s.blockedEdges;
+    stats->ms_rounds_blocked = s.msRoundsBlocked;
+    stats->rounds_launches_blocked = s.roundsLaunchesBlocked;
+    stats->blk_work_edges = s.blkWorkEdges;
+    stats->blk_work_entries = s.blkWorkEntries;
+    stats->blk_updates = s.blkUpdates;
 }
 
 extern "C" {

@@ -461,7 +461,7 @@ private:
     int advGrid_ = 0;
     RunStats st_;
     std::vector<Unit> units_;
-    EventPairs pAdv_, pFr_, pXfer_, pGen_, pRounds_;
+    EventPairs pAdv_, pFr_, pXfer_, pGen_, pRounds_, pRoundsBlk_;   // pRoundsBlk_: rounds launches on block units
     dev::PersistCtl *pc_ = nullptr;
     int roundsGrid_ = 0;    // 0: persistent rounds kernel unavailable
     // batched state
@@ -884,10 +884,20 @@ void Engine<E, G>::buildUnitsLearned()
     ull gLo = 0, gHi = 0, gEdges = 0, gBase = 0;
     bool gFull = true;
     uint gFirstRange = 0, gLastRange = 0;
-    // open whole group: vertex ids [wB0, wB1)
+    // open whole group: vertex ids [wB0, wB1), at most wCap edges
     bool wOpen = false;
     uint wB0 = 0, wB1 = 0;
-    ull wEdges = 0;
+    ull wEdges = 0, wCap = 0;
+    // edges of the whole ranges from range i on (the ranges this iteration
+    // moves whole), for the geometric tail of whole units
+    std::vector<ull> wholeLeft(P + 1, 0);
+    for (size_t i = P; i-- > 0;) {
+        const ull lo = h_rstats_[4 * i], hi = h_rstats_[4 * i + 1], ea = h_rstats_[4 * i + 2];
+        const ull re = np[bounds_[i + 1]] - np[bounds_[i]];
+        const bool hub = (bounds_[i + 1] - bounds_[i] == 1) && re > maxPart_;
+        const bool whole = !done[i] && hi > lo && !hub && re > 0 && (double)ea >= theta_ * (double)re;
+        wholeLeft[i] = wholeLeft[i + 1] + (whole ? re : 0);
+    }
     auto flush = [&]() {
         if (!open) return;
         Unit u{};
@@ -955,7 +965,7 @@ void Engine<E, G>::buildUnitsLearned()
         }
         if (re > 0 && (double)ea >= theta_ * (double)re) {
             flush();
-            if (wOpen && wB1 == b0 && wEdges + re <= maxPart_) {
+            if (wOpen && wB1 == b0 && wEdges + re <= wCap) {
                 wB1 = b1;
                 wEdges += re;
             } else {
@@ -964,6 +974,7 @@ void Engine<E, G>::buildUnitsLearned()
                 wB0 = b0;
                 wB1 = b1;
                 wEdges = re;
+                wCap = std::min<ull>(maxPart_, whole_unit_cap(maxPart_, re, wholeLeft[i]));
             }
             continue;
         }
@@ -1357,7 +1368,8 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         l.pc = pc_;
         l.maxRounds = (uint32_t)std::max(1, roundsCap_);
         l.roundBase = 0;
-        if (o_.profile) pRounds_.begin(sc_);
+        EventPairs &ev = un.blocked ? pRoundsBlk_ : pRounds_;
+        if (o_.profile) ev.begin(sc_);
         l.packCode = codeOf(un);
         if (un.blocked) {
             l.blk = d_blk_;
@@ -1365,7 +1377,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         }
         l.joinTicks = o_.joinWindow;
         launch_rounds(o_.algo, fmtOf(un), l, roundsGrid_, sc_);
-        if (o_.profile) pRounds_.end(sc_);
+        if (o_.profile) ev.end(sc_);
         return;
     }
     ull r = 0;
@@ -1591,6 +1603,7 @@ void Engine<E, G>::resetRun()
     pXfer_.reset();
     pGen_.reset();
     pRounds_.reset();
+    pRoundsBlk_.reset();
 }
 
 // The global-iteration loop shared by single-source and batched runs; the
@@ -1747,6 +1760,9 @@ void Engine<E, G>::finishStats()
     st_.pairWorkEdges = c1.pair_work - ctl0_.pair_work;
     st_.pairWorkEntries = c1.pair_entries - ctl0_.pair_entries;
     st_.pairUpdates = c1.pair_updates - ctl0_.pair_updates;
+    st_.blkWorkEdges = c1.blk_work - ctl0_.blk_work;
+    st_.blkWorkEntries = c1.blk_entries - ctl0_.blk_entries;
+    st_.blkUpdates = c1.blk_updates - ctl0_.blk_updates;
     st_.edgeBytes = (uint)eb_;
     if (pc_) {
         uint32_t mm = 0;
@@ -1759,7 +1775,9 @@ void Engine<E, G>::finishStats()
         st_.msFrontier = pFr_.total(st_.frontierLaunches);
         st_.msTransfer = pXfer_.total(st_.transferOps);
         st_.msGenerate = pGen_.total(c);
-        st_.msRounds = pRounds_.total(st_.roundsLaunches);
+        st_.msRoundsBlocked = pRoundsBlk_.total(st_.roundsLaunchesBlocked);
+        st_.msRounds = pRounds_.total(st_.roundsLaunches) + st_.msRoundsBlocked;
+        st_.roundsLaunches += st_.roundsLaunchesBlocked;
     }
 }
 

@@ -459,7 +459,7 @@ struct GenPrOp {
 // increments).  Stand-alone scans (COH false) also count into the pair_*
 // words, so the persistent launches' share can be told apart.
 template <bool COH>
-__device__ __forceinline__ void stat_counts(ScanCtl *ctl, u64 cnt, u64 sum)
+__device__ __forceinline__ void stat_counts(ScanCtl *ctl, u64 cnt, u64 sum, bool blocked = false)
 {
     st_relaxed(&ctl->stop, cnt == 0 ? 1u : 0u);
     if (cnt) atomicAdd((unsigned long long *)&ctl->rounds, 1ull);
@@ -468,6 +468,9 @@ __device__ __forceinline__ void stat_counts(ScanCtl *ctl, u64 cnt, u64 sum)
     if constexpr (!COH) {
         atomicAdd((unsigned long long *)&ctl->pair_entries, (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctl->pair_work, (unsigned long long)sum);
+    } else if (blocked) {
+        atomicAdd((unsigned long long *)&ctl->blk_entries, (unsigned long long)cnt);
+        atomicAdd((unsigned long long *)&ctl->blk_work, (unsigned long long)sum);
     }
 }
 
@@ -568,7 +571,7 @@ struct FrontierLabelOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        stat_counts<COH>(ctl, cnt, sum);
+        stat_counts<COH>(ctl, cnt, sum, blk != nullptr);
     }
 };
 
@@ -621,7 +624,7 @@ struct FrontierPrOp {
     {
         st_relaxed(&ctl->count, cnt);
         st_relaxed(&ctl->total, sum);
-        stat_counts<COH>(ctl, cnt, sum);
+        stat_counts<COH>(ctl, cnt, sum, blk != nullptr);
     }
 };
 
@@ -1292,7 +1295,10 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
-    if (lane_id() == 0 && wsum) atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
+    if (lane_id() == 0 && wsum) {
+        atomicAdd((unsigned long long *)&a.sc->updates, (unsigned long long)wsum);
+        if (a.blk) atomicAdd((unsigned long long *)&a.sc->blk_updates, (unsigned long long)wsum);
+    }
     if (me == 0 && tid == 0) {
         st_relaxed(&a.sc->epoch, epoch0 + r + conv);
         a.pc->rounds_done = r;

@@ -48,6 +48,11 @@ struct ScanCtl {
     // frontier), cleared by the host before each step
     u64 pull_found;
     u64 pull_edges;
+    // the part of entry_sum, work_sum and updates that persistent rounds
+    // launches over units in the block layout contributed
+    u64 blk_entries;
+    u64 blk_work;
+    u64 blk_updates;
 };
 static_assert(offsetof(ScanCtl, lane_entries) == offsetof(ScanCtl, lane_work) + 8,
               "lane_work, lane_entries form one two-word block");

@@ -50,6 +50,21 @@ ResidentChoice choose_resident_visits(const uint32_t *bounds, size_t P, ActiveEd
     return c;
 }
 
+// Edge cap of a unit of whole id ranges that opens while `remaining` edges
+// of whole ranges (this one included) are still to move in the iteration:
+// a full buffer while plenty remains, otherwise half of what remains but not
+// below minEdges, and never less than the range it opens with.  Units then
+// shrink geometrically towards the end of the iteration, so the rounds of
+// the last unit, which no transfer overlaps, cover about one range instead
+// of a whole buffer; minEdges keeps units whose rounds would take less than
+// a launch costs from being split.
+constexpr unsigned long long kTailMinEdges = 1ull << 25;
+inline unsigned long long whole_unit_cap(unsigned long long bufferEdges, unsigned long long firstRange,
+                                         unsigned long long remaining, unsigned long long minEdges = kTailMinEdges)
+{
+    return std::max(firstRange, std::min(bufferEdges, std::max(remaining / 2, minEdges)));
+}
+
 }  // namespace subway
 
 #endif  // SUBWAY_SCHEDULE_HPP

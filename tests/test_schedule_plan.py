@@ -64,3 +64,43 @@ def test_minimum_volume_and_alignment(checker):
 def test_one_buffer(checker):
     visit, ov, _ = checker(BOUNDS, [5] * 8, [(0, 40), (0, 80)], nb=1)
     assert visit == (True, False) and ov == 0
+
+
+@pytest.fixture(scope="module")
+def tail_cap(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("tail") / "tail_cap"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "subway_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "tail_cap.cpp"), "-o", str(exe)], check=True)
+
+    def run(rows):
+        line = "\n".join(" ".join(map(str, r)) for r in rows)
+        out = subprocess.run([str(exe)], input=line + "\n", capture_output=True, text=True, check=True).stdout
+        return [int(x) for x in out.split()]
+    return run
+
+
+def test_whole_units_shrink_towards_the_end(tail_cap):
+    """Units of whole ranges, built greedily with the cap whole_unit_cap gives
+    at each unit's start: full buffers while plenty remains, then halves, and
+    the last unit is one range (8 ranges of 10 per buffer of 80, 60 ranges)."""
+    B, R, n = 80, 10, 60
+    caps, units, left = [], [], n * R
+    i = 0
+    while i < n:
+        cap = tail_cap([(B, R, left, 1)])[0]
+        size = 0
+        while i < n and size + R <= cap:
+            size += R
+            i += 1
+        units.append(size)
+        left -= size
+    assert units == [80, 80, 80, 80, 80, 80, 60, 30, 10, 10, 10]
+    assert sum(units) == n * R
+
+
+def test_tail_floor_keeps_small_units_whole(tail_cap):
+    # below the floor nothing is split; the opening range always fits
+    assert tail_cap([(80, 10, 100, 50)]) == [50]
+    assert tail_cap([(80, 10, 30, 50)]) == [50]
+    assert tail_cap([(80, 90, 30, 1)]) == [90]
+    assert tail_cap([(80, 10, 1000, 1)]) == [80]
