@@ -10,8 +10,9 @@ at their stated sizes (BFS-sync scale 16 with no cap; SSSP-/SSWP-async scale
 24 under 1 GiB; CC-async on the symmetrised scale 25 and PR-async scale 25,
 2 GiB; SSSP-/BFS-async scale 26 under 4 GiB).  A sweep can be split over
 several processes (--set, --only i:j, --dump) and trained from the dumps
-(--train-from).  The graphs are renumbered by descending in-degree, as the
-benchmark's are (--order natural keeps the generator's ids).
+(--train-from).  The graphs are renumbered by descending in-degree and
+carry their block-compressed rows, as the benchmark's do (--order natural
+keeps the generator's ids, --blocks 0 the plain transfers).
 
 For every training configuration (algorithm, R-MAT graph, device budget) the
 async run is repeated under a grid of fixed schedules -- whole-range density
@@ -78,13 +79,17 @@ def configs(quick: bool, which: str = "small"):
     return out
 
 
-def make_graph(algo, scale, seed, edge_factor=16, symmetrize=False, order="in-degree"):
+def make_graph(algo, scale, seed, edge_factor=16, symmetrize=False, order="in-degree", blocks=True):
     """The R-MAT graph of a configuration, renumbered by descending in-degree
     unless order is "natural" (the benchmark renumbers its graphs, and the
-    schedule's id ranges see the numbering)."""
+    schedule's id ranges see the numbering), with its block-compressed rows
+    unless blocks is False (the benchmark moves whole ranges in them, which
+    changes what moving a range whole costs)."""
     g = sw.Graph.rmat(scale, edge_factor, seed=seed, weighted=algo in ("sssp", "sswp"), symmetrize=symmetrize)
     if order == "in-degree":
         g.reorder()
+    if blocks:
+        g.build_blocks()
     return g
 
 
@@ -171,6 +176,7 @@ def main(argv=None):
     ap.add_argument("--train-from", nargs="*", default=[], help="train from sweep dumps instead of sweeping")
     ap.add_argument("--order", default="in-degree", choices=["in-degree", "natural"],
                     help="vertex numbering of the sweep's graphs")
+    ap.add_argument("--blocks", type=int, default=1, help="1: the sweep's graphs move whole ranges as blocks")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -192,14 +198,15 @@ def main(argv=None):
             todo = todo[i:j]
         dumped = []
         for algo, scale, ef, seed, div, budget, sym in todo:
-            g = make_graph(algo, scale, seed, ef, sym, args.order)
+            g = make_graph(algo, scale, seed, ef, sym, args.order, bool(args.blocks))
             big = g.m >= (1 << 28)
             res, best, plain_ms = run_config(g, algo, div, 1 if big else args.reps, log, seed=len(results),
                                              budget_bytes=budget if not div else None)
             g.free()
             results.append((res, best))
             s = {"algo": algo, "scale": scale, "edge_factor": ef, "seed": seed, "budget_divisor": div,
-                 "budget_bytes": budget, "symmetrize": sym, "order": args.order, "plain_ms": plain_ms,
+                 "budget_bytes": budget, "symmetrize": sym, "order": args.order, "blocks": bool(args.blocks),
+                 "plain_ms": plain_ms,
                  "best": list(best),
                  "ms": {f"{k[0]}/{k[1]}": v[0] for k, v in res.items()}}
             summary.append(s)
