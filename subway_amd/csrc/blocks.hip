@@ -13,7 +13,7 @@
 //   3. each host layout the graph holds (split arrays, 8-byte list, packed
 //      words, 4-byte list) is rewritten from the sorted rows;
 //   4. blocks per row (ceil(degree / 32)) and their exclusive scan; per
-//      block its offset width and payload words, their exclusive scan; the
+//      block its gap width and payload words, their exclusive scan; the
 //      row offsets rowPtr[v] = 2 * (blocks before v) + (payload words before
 //      v);
 //   5. one thread per block writes its two descriptor words and its payload
@@ -227,6 +227,17 @@ __device__ __forceinline__ BlockSpot block_spot(const u64 *ptr, const u64 *bidx,
     return s;
 }
 
+// bits of the widest gap between consecutive (sorted) destinations of a block
+__device__ __forceinline__ u32 block_width(const u32 *d, u32 cnt)
+{
+    u32 mx = 0;
+    for (u32 j = 1; j < cnt; j++) {
+        const u32 g = d[j] - d[j - 1];
+        mx = g > mx ? g : mx;
+    }
+    return blk_bits(mx);
+}
+
 __global__ __launch_bounds__(BB) void block_words_kernel(const u64 *ptr, const u64 *bidx, u64 n, u64 B,
                                                          const u32 *dst, u32 wbits, u32 *pw)
 {
@@ -240,8 +251,7 @@ __global__ __launch_bounds__(BB) void block_words_kernel(const u64 *ptr, const u
     __syncthreads();
     for (u64 b = b0 + threadIdx.x; b < b1; b += BB) {
         const BlockSpot s = block_spot(ptr, bidx, b, s_p0, s_p1);
-        const u32 w = blk_bits(dst[s.e + s.cnt - 1] - dst[s.e]);
-        pw[b] = blk_payload_words(s.cnt, w, wbits);
+        pw[b] = blk_payload_words(s.cnt, block_width(dst + s.e, s.cnt), wbits);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) pw[B] = 0;
 }
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(BB) void encode_kernel(const u64 *ptr, const u64 *b
     for (u64 b = b0 + threadIdx.x; b < b1; b += BB) {
         const BlockSpot s = block_spot(ptr, bidx, b, s_p0, s_p1);
         const u32 anchor = dst[s.e];
-        const u32 wd = blk_bits(dst[s.e + s.cnt - 1] - anchor);
+        const u32 wd = block_width(dst + s.e, s.cnt);
         const u64 woff = pay[b] - pay[bidx[s.v]];
         if (woff + (pay[b + 1] - pay[b]) > (u64)BLK_MAX_ROW_WORDS) {
             atomicOr(err, 1u);
@@ -293,7 +303,7 @@ __global__ __launch_bounds__(BB) void encode_kernel(const u64 *ptr, const u64 *b
                 used -= 32;
             }
         };
-        for (u32 j = 1; j < s.cnt; j++) put(dst[s.e + j] - anchor, wd);
+        for (u32 j = 1; j < s.cnt; j++) put(dst[s.e + j] - dst[s.e + j - 1], wd);
         if (w)
             for (u32 j = 0; j < s.cnt; j++) put((u32)w[s.e + j] - wbase, wbits);
         if (used) out[at] = (u32)acc;

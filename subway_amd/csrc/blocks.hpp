@@ -3,13 +3,13 @@
 //
 // Every row (a vertex's out-edges) is sorted by destination and cut into
 // blocks of BLK_EDGES edges.  A block stores its first (smallest)
-// destination as an anchor and every other destination as its offset from
-// the anchor, in w bits, w = bits(last - anchor); weights follow at the
+// destination as an anchor and every other destination as the gap to the
+// one before it, in w bits, w = bits(widest gap); weights follow at the
 // graph's weight width.  Rows of one id range are contiguous, so a range
-// moves as one DMA of fewer bytes than its plain edges (on a renumbered
-// R-MAT graph about 2.8-3.1 bytes per weighted edge against 5 in the split
-// layout), and any edge decodes from its row's offset, its index in the row
-// and the row's degree alone: no scan over earlier edges.
+// moves as one DMA of fewer bytes than its plain edges, and any edge decodes
+// from its row's offset, its index in the row and the row's degree plus the
+// gaps before it in its own block (a prefix sum over at most 31 gaps, which
+// a wave computes with a segmented scan): no scan over earlier blocks.
 //
 // Layout, in 32-bit words.  Row v (degree d > 0, nb = ceil(d / BLK_EDGES)
 // blocks) starts at word rowPtr[v]:
@@ -19,7 +19,8 @@
 //   words [2 nb, ...)         payloads; block k's starts woff_k words after
 //                             the first payload word of the row
 // Payload of block k (cnt edges, cnt = min(BLK_EDGES, d - k BLK_EDGES)):
-//   (cnt - 1) offsets of w_k bits (edges 1 .. cnt - 1 of the block), then
+//   (cnt - 1) gaps of w_k bits (gap j = destination j - destination j - 1,
+//   j = 1 .. cnt - 1; destination j = anchor + gaps 1 .. j), then
 //   cnt weights of wbits bits (weight - wbase), least significant bit first
 //   in little-endian words, padded to a whole word.
 // Rows of degree 0 take no words.  rowPtr has n + 1 entries (u64), the last

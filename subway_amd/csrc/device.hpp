@@ -109,6 +109,30 @@ __device__ __forceinline__ u32 wave_reduce(u32 x)
     return x;
 }
 
+// Inclusive scan of x over the lanes [segStart, lane] of a wave64
+// (segStart = first lane of the calling lane's segment, 0 for a plain scan),
+// with DPP moves: four row shifts within 16-lane rows, then the last lane of
+// row 0 / 2 broadcast into row 1 / 3, then lane 31 into rows 2 and 3.  A lane
+// adds a moved value only when its source lane lies in its own segment.
+__device__ __forceinline__ u32 wave_seg_incl_scan(u32 x, int lane, int segStart)
+{
+    const int r = lane & 15;
+    u32 y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    if (r >= 1 && lane - 1 >= segStart) x += y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    if (r >= 2 && lane - 2 >= segStart) x += y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    if (r >= 4 && lane - 4 >= segStart) x += y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);   // row_shr:8
+    if (r >= 8 && lane - 8 >= segStart) x += y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);   // row_bcast:15 into rows 1, 3
+    if (((lane >> 4) & 1) && (lane & ~15) - 1 >= segStart) x += y;
+    y = (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);   // row_bcast:31 into rows 2, 3
+    if (lane >= 32 && 31 >= segStart) x += y;
+    return x;
+}
+
 __device__ __forceinline__ u32 wave_reduce_min(u32 x)
 {
 #pragma unroll

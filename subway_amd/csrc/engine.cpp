@@ -1321,6 +1321,7 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf, 
     a.w1 = w1;
     a.shift = shift;
     a.grid = advGrid_;
+    a.nv = g_->num_nodes;
     if (o_.profile) pAdv_.begin(sc_);
     launch_advance(a, fr_.ctl, sc_);
     if (o_.profile) pAdv_.end(sc_);
@@ -1376,6 +1377,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
             l.blkBase = g_->csr->blkPtr[un.idLo];
         }
         l.joinTicks = o_.joinWindow;
+        l.nv = g_->num_nodes;
         launch_rounds(o_.algo, fmtOf(un), l, roundsGrid_, sc_);
         if (o_.profile) ev.end(sc_);
         return;

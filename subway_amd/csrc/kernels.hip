@@ -855,7 +855,8 @@ struct Codec<SplitEdge> {
 template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, double *delta, u64 count, u64 W, u64 w0,
-                                             u64 wend, u64 shift, u32 psh, u32 *err, u32 *claimCtr, u32 *s_next)
+                                             u64 wend, u64 shift, u32 psh, u32 *err, u32 *claimCtr, u32 *s_next,
+                                             u32 nv)
 {
     constexpr bool BLOCKED = std::is_same<E, BlockEdge>::value;
     __shared__ u32 s_pre[ADV_CHUNK + 1];
@@ -894,139 +895,186 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
     // for the memory operation right there) is needed.
     constexpr int PPT = ADV_CHUNK / ADV_BLOCK;
     static_assert(ADV_CHUNK % ADV_BLOCK == 0, "a chunk is a whole number of positions per thread");
-    u32 sv[PPT];
-    bool ok[PPT];
-    u32 dst[PPT], cv[PPT];
-    if constexpr (BLOCKED) {
-        // block layout (blocks.hpp): the descriptor of the position's block,
-        // then the words its offset and weight fields sit in (both words of
-        // each field are read; a field that ends in the first one ignores the
-        // second, and a buffer keeps two words of slack past its rows)
-        const u32 *W = reinterpret_cast<const u32 *>(edges);
-        const u32 wbits = psh & 63u, wbase = psh >> BLK_WFIELD;
-        constexpr bool WEIGHTS = ALGO == ALGO_SSSP || ALGO == ALGO_SSWP;
-        // per position: pay = first payload word of the row, jc = index in
-        // the block | edges of the block << 8, then the descriptor (da, dc)
-        u32 pay[PPT], jc[PPT], da[PPT], dc[PPT];
-#pragma unroll
-        for (int u = 0; u < PPT; u++) {
-            const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
-            ok[u] = p < pend;
-            if (ok[u]) {
-                const u32 pp = (u32)p;
-                int lo = 0, hi = nk - 1;
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= pp) lo = mid;
-                    else hi = mid - 1;
-                }
-                const u32 row = s_loc[lo], i = pp - s_pre[lo], d = s_deg[lo];
-                sv[u] = s_val[lo];
-                const u32 k = i >> BLK_SHIFT, left = d - (k << BLK_SHIFT);
-                pay[u] = row + 2u * ((d + BLK_EDGES - 1u) >> BLK_SHIFT);
-                jc[u] = (i & (BLK_EDGES - 1u)) | ((left < (u32)BLK_EDGES ? left : (u32)BLK_EDGES) << 8);
-                da[u] = W[row + 2u * k];
-                dc[u] = W[row + 2u * k + 1u];
-            }
-        }
-        // the offset field's two words (and the weight field's); sh packs the
-        // two bit shifts and the offset width
-        u32 f0[PPT], f1[PPT], g0[PPT], g1[PPT], sh[PPT];
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (ok[u]) {
-                const u32 jj = jc[u] & 0xFFu, cnt = jc[u] >> 8, wd = dc[u] & 63u;
-                const u32 P = pay[u] + (dc[u] >> BLK_WFIELD);
-                const u32 obit = jj ? (jj - 1u) * wd : 0u;
-                f0[u] = W[P + (obit >> 5)];
-                f1[u] = W[P + (obit >> 5) + 1u];
-                u32 gsh = 0;
-                if constexpr (WEIGHTS) {
-                    const u32 gbit = (cnt - 1u) * wd + jj * wbits;
-                    g0[u] = W[P + (gbit >> 5)];
-                    g1[u] = W[P + (gbit >> 5) + 1u];
-                    gsh = gbit & 31u;
-                }
-                sh[u] = (obit & 31u) | (gsh << 8) | ((jj ? wd : 0u) << 16);
-            }
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (ok[u]) {
-                dst[u] = da[u] + blk_extract(f0[u], f1[u], sh[u] & 31u, sh[u] >> 16);
-                if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
-                else if constexpr (WEIGHTS)
-                    cv[u] = Rule<ALGO>::cand(sv[u], wbase + blk_extract(g0[u], g1[u], (sh[u] >> 8) & 31u, wbits));
-                else cv[u] = Rule<ALGO>::cand(sv[u], 1u);
-            }
-        (void)shift;
-    } else {
-        typename Codec<E>::Raw raw[PPT];
-#pragma unroll
-        for (int u = 0; u < PPT; u++) {
-            const u64 p = pbeg + tid + (u64)u * ADV_BLOCK;
-            ok[u] = p < pend;
-            if (ok[u]) {
-                const u32 pp = (u32)p;
-                int lo = 0, hi = nk - 1;
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= pp) lo = mid;
-                    else hi = mid - 1;
-                }
-                const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
-                raw[u] = Codec<E>::load(edges, eidx, psh);
-                sv[u] = s_val[lo];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (ok[u]) {
-                const auto e = Codec<E>::decode(raw[u], psh);
-                dst[u] = e.end;
-                if constexpr (ALGO == ALGO_PR) cv[u] = sv[u];
-                else cv[u] = Rule<ALGO>::cand(sv[u], edge_weight(e));
-            }
-    }
+    // block units decode in two halves of the thread's positions (their
+    // decode holds more registers per position); plain formats in one step
+    constexpr int STEP = BLOCKED ? PPT / 2 : PPT;
     u32 updates = 0;
-    if constexpr (ALGO == ALGO_PR) {
-        // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
-        // FP64: an FP32 sum at a hub rounds away shares below half its ulp
 #pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (ok[u]) {
-                unsafeAtomicAdd(&delta[dst[u]], (double)__uint_as_float(cv[u]));
-                updates++;
+    for (int h = 0; h < PPT; h += STEP) {
+        u32 sv[STEP];
+        bool ok[STEP];
+        u32 dst[STEP], cv[STEP];
+        if constexpr (BLOCKED) {
+            // block layout (blocks.hpp): the descriptor of the position's
+            // block, then the words its gap and weight fields sit in (both
+            // words of each field are read; a field that ends in the first
+            // one ignores the second, and a buffer keeps two words of slack
+            // past its rows)
+            const u32 *W = reinterpret_cast<const u32 *>(edges);
+            const u32 wbits = psh & 63u, wbase = psh >> BLK_WFIELD;
+            constexpr bool WEIGHTS = ALGO == ALGO_SSSP || ALGO == ALGO_SSWP;
+            const int lane = lane_id();
+            // pay = first payload word of the row, jc = index in the block |
+            // edges of the block << 8, then the descriptor (da, dc)
+            u32 pay[STEP], jc[STEP], da[STEP], dc[STEP];
+#pragma unroll
+            for (int v = 0; v < STEP; v++) {
+                const u64 p = pbeg + tid + (u64)(h + v) * ADV_BLOCK;
+                ok[v] = p < pend;
+                if (ok[v]) {
+                    const u32 pp = (u32)p;
+                    int lo = 0, hi = nk - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_pre[mid] <= pp) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const u32 row = s_loc[lo], i = pp - s_pre[lo], d = s_deg[lo];
+                    sv[v] = s_val[lo];
+                    const u32 k = i >> BLK_SHIFT, left = d - (k << BLK_SHIFT);
+                    pay[v] = row + 2u * ((d + BLK_EDGES - 1u) >> BLK_SHIFT);
+                    jc[v] = (i & (BLK_EDGES - 1u)) | ((left < (u32)BLK_EDGES ? left : (u32)BLK_EDGES) << 8);
+                    da[v] = W[row + 2u * k];
+                    dc[v] = W[row + 2u * k + 1u];
+                }
             }
-        (void)value;
-        (void)lab;
-    } else {
-        u32 cur[PPT];
+            // the gap field's two words (and the weight field's); sh packs
+            // the two bit shifts and the gap width.  A wave's 64 lanes hold
+            // 64 consecutive positions, filled from lane 0 up; lanes
+            // 0 .. j0 - 2 also read gaps 1 .. j0 - 1 of the block lane 0
+            // starts in (j0 = lane 0's index in its block): the carry into
+            // the wave's first block
+            u32 f0[STEP], f1[STEP], g0[STEP], g1[STEP], sh[STEP], c0[STEP], c1[STEP];
 #pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (ok[u]) cur[u] = value[dst[u]];
-        // every test before the first atomic: a test between two atomics
-        // would wait for the earlier atomic as well
-        bool tried[PPT];
-#pragma unroll
-        for (int u = 0; u < PPT; u++) tried[u] = ok[u] && Rule<ALGO>::better(cv[u], cur[u]);
-        u32 old[PPT];
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (tried[u]) old[u] = Rule<ALGO>::atomic(&value[dst[u]], cv[u]);
-        // the returned values are taken only after every atomic is issued:
-        // the compiler would otherwise fold each improvement test into its
-        // atomic's branch and wait for that atomic there
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (tried[u]) asm volatile("" : "+v"(old[u]));
-        // a label is set only where the atomic's old value shows the improvement
-#pragma unroll
-        for (int u = 0; u < PPT; u++)
-            if (tried[u] && Rule<ALGO>::better(cv[u], old[u])) {
-                mst<COH>(&lab[dst[u]], (uint8_t)1);
-                updates++;
+            for (int v = 0; v < STEP; v++) {
+                if (ok[v]) {
+                    const u32 jj = jc[v] & 0xFFu, cnt = jc[v] >> 8, wd = dc[v] & 63u;
+                    const u32 P = pay[v] + (dc[v] >> BLK_WFIELD);
+                    const u32 obit = jj ? (jj - 1u) * wd : 0u;
+                    f0[v] = W[P + (obit >> 5)];
+                    f1[v] = W[P + (obit >> 5) + 1u];
+                    u32 gsh = 0;
+                    if constexpr (WEIGHTS) {
+                        const u32 gbit = (cnt - 1u) * wd + jj * wbits;
+                        g0[v] = W[P + (gbit >> 5)];
+                        g1[v] = W[P + (gbit >> 5) + 1u];
+                        gsh = gbit & 31u;
+                    }
+                    sh[v] = (obit & 31u) | (gsh << 8) | ((jj ? wd : 0u) << 16);
+                }
+                if (__builtin_amdgcn_readfirstlane(ok[v] ? 1u : 0u)) {
+                    const u32 j0 = __builtin_amdgcn_readfirstlane(jc[v] & 0xFFu);
+                    const u32 w0 = __builtin_amdgcn_readfirstlane(dc[v] & 63u);
+                    const u32 P0 = __builtin_amdgcn_readfirstlane(pay[v] + (dc[v] >> BLK_WFIELD));
+                    if ((u32)lane + 1u < j0) {
+                        const u32 cbit = (u32)lane * w0;
+                        c0[v] = W[P0 + (cbit >> 5)];
+                        c1[v] = W[P0 + (cbit >> 5) + 1u];
+                    }
+                }
             }
-        (void)delta;
+#pragma unroll
+            for (int v = 0; v < STEP; v++) {
+                if (!__builtin_amdgcn_readfirstlane(ok[v] ? 1u : 0u)) continue;   // uniform: no position in the wave
+                const u32 j0 = __builtin_amdgcn_readfirstlane(jc[v] & 0xFFu);
+                const u32 w0 = __builtin_amdgcn_readfirstlane(dc[v] & 63u);
+                const u32 shadow =
+                    ((u32)lane + 1u < j0) ? blk_extract(c0[v], c1[v], ((u32)lane * w0) & 31u, w0) : 0u;
+                const u32 carry = (u32)__builtin_amdgcn_readlane((int)wave_seg_incl_scan(shadow, lane, 0), 63);
+                const u32 jj = ok[v] ? (jc[v] & 0xFFu) : 0u;
+                const u32 gap = ok[v] && jj ? blk_extract(f0[v], f1[v], sh[v] & 31u, sh[v] >> 16) : 0u;
+                // segments: one block's positions in the wave, starting at
+                // lane 0 or where the index in the block is 0 (lanes without
+                // a position start their own)
+                const u64 heads = __ballot(lane == 0 || !ok[v] || jj == 0);
+                const u64 upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+                const int segStart = 63 - __clzll((long long)(heads & upTo));
+                const u32 S = wave_seg_incl_scan(gap, lane, segStart);
+                if (ok[v]) {
+                    dst[v] = da[v] + S + (segStart == 0 ? carry : 0u);
+                    if (dst[v] >= nv) {
+                        // a decoded destination outside the graph: the words
+                        // are not this unit's; report, never dereference
+                        (void)atomicCAS(err, 0u, WAIT_DATA);
+                        ok[v] = false;
+                        continue;
+                    }
+                    if constexpr (ALGO == ALGO_PR) cv[v] = sv[v];
+                    else if constexpr (WEIGHTS)
+                        cv[v] = Rule<ALGO>::cand(sv[v], wbase + blk_extract(g0[v], g1[v], (sh[v] >> 8) & 31u, wbits));
+                    else cv[v] = Rule<ALGO>::cand(sv[v], 1u);
+                }
+            }
+            (void)shift;
+        } else {
+            typename Codec<E>::Raw raw[STEP];
+#pragma unroll
+            for (int v = 0; v < STEP; v++) {
+                const u64 p = pbeg + tid + (u64)(h + v) * ADV_BLOCK;
+                ok[v] = p < pend;
+                if (ok[v]) {
+                    const u32 pp = (u32)p;
+                    int lo = 0, hi = nk - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_pre[mid] <= pp) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const u64 eidx = (u64)s_loc[lo] + (pp - s_pre[lo]) - shift;
+                    raw[v] = Codec<E>::load(edges, eidx, psh);
+                    sv[v] = s_val[lo];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (ok[v]) {
+                    const auto e = Codec<E>::decode(raw[v], psh);
+                    dst[v] = e.end;
+                    if constexpr (ALGO == ALGO_PR) cv[v] = sv[v];
+                    else cv[v] = Rule<ALGO>::cand(sv[v], edge_weight(e));
+                }
+            (void)nv;
+        }
+        if constexpr (ALGO == ALGO_PR) {
+            // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
+            // FP64: an FP32 sum at a hub rounds away shares below half its ulp
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (ok[v]) {
+                    unsafeAtomicAdd(&delta[dst[v]], (double)__uint_as_float(cv[v]));
+                    updates++;
+                }
+            (void)value;
+            (void)lab;
+        } else {
+            u32 cur[STEP];
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (ok[v]) cur[v] = value[dst[v]];
+            // every test before the first atomic: a test between two atomics
+            // would wait for the earlier atomic as well
+            bool tried[STEP];
+#pragma unroll
+            for (int v = 0; v < STEP; v++) tried[v] = ok[v] && Rule<ALGO>::better(cv[v], cur[v]);
+            u32 old[STEP];
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (tried[v]) old[v] = Rule<ALGO>::atomic(&value[dst[v]], cv[v]);
+            // the returned values are taken only after every atomic is issued:
+            // the compiler would otherwise fold each improvement test into its
+            // atomic's branch and wait for that atomic there
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (tried[v]) asm volatile("" : "+v"(old[v]));
+            // a label is set only where the atomic's old value shows the improvement
+#pragma unroll
+            for (int v = 0; v < STEP; v++)
+                if (tried[v] && Rule<ALGO>::better(cv[v], old[v])) {
+                    mst<COH>(&lab[dst[v]], (uint8_t)1);
+                    updates++;
+                }
+            (void)delta;
+        }
     }
     if (tid == 0) *s_next = next;
     __syncthreads();
@@ -1037,7 +1085,7 @@ template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries, const u32 *chunkMap,
                                                             const E *edges, u32 *value, uint8_t *lab,
                                                             double *delta, u64 w0, u64 w1, u64 shift,
-                                                            u32 psh, ScanCtl *ctl)
+                                                            u32 psh, ScanCtl *ctl, u32 nv)
 {
     __shared__ u32 s_chunk;
     const int tid = threadIdx.x;
@@ -1057,7 +1105,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
     __syncthreads();
     for (u64 c = s_chunk; c < nchunks; c = s_chunk)
         updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
-                                                 wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk);
+                                                 wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk, nv);
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) {
         atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -1223,6 +1271,7 @@ struct RoundsArgs {
     u32 packCode;         // EDGE_PACKED: destination bits
     const u64 *blk;        // EDGE_BLOCKS: row word offsets (null otherwise)
     u64 blkBase;           // ... of the buffer's first row
+    u32 nv;                // vertices of the graph (decoded destinations are checked against it)
 };
 
 // All async inner rounds of one resident partition in one persistent
@@ -1291,7 +1340,7 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
         for (u64 c = s_claim; c < nchunks; c = s_claim)
             updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
                                               a.delta, count, W, 0, W, 0, a.packCode, wt.err,
-                                              &a.pc->chunk_ctr[r & 1], &s_claim);
+                                              &a.pc->chunk_ctr[r & 1], &s_claim, a.nv);
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
@@ -2188,7 +2237,7 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 {
     hipLaunchKernelGGL((advance_kernel<ALGO, E>), dim3((unsigned)a.grid), dim3(ADV_BLOCK), 0, st,
                        a.entries, a.chunkMap, (const E *)a.edges, a.value, a.labelOut, a.delta,
-                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packCode, ctl);
+                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packCode, ctl, (u32)a.nv);
 }
 
 // Launch `L<ALGO, storage>` for the buffer's edge format; SSSP and SSWP need
@@ -2350,6 +2399,7 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.packCode = l.packCode;
     a.blk = (const u64 *)l.blk;
     a.blkBase = (u64)l.blkBase;
+    a.nv = l.nv;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
     // an ordinary launch: the membership protocol (join_launch) makes the

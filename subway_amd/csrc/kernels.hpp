@@ -198,6 +198,7 @@ struct AdvanceArgs {
     ull w0, w1;                 // work window [w0, w1); w1 = ~0 for all
     ull shift;                  // subtracted from edge indices
     int grid;                        // persistent workgroups
+    uint32_t nv;                     // vertices of the graph (block decode checks destinations against it)
 };
 void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st);
 
@@ -224,6 +225,7 @@ struct RoundsLaunch {
     uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
     const ull *blk;        // EDGE_BLOCKS buffers: row word offsets of the graph (else null)
     ull blkBase;           // ... of the buffer's first row
+    uint32_t nv;           // vertices of the graph (block decode checks destinations against it)
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
 // below full occupancy.

@@ -5,7 +5,7 @@ sort_rows(ptr, ends, w): every row in ascending destination order, equal
 destinations keeping their order (the device sort is a stable radix sort by
 destination, then by row).
 encode(ptr, ends, w): (row_ptr u64[n+1], words u32[T], wbits, wbase) of rows
-already sorted.
+already sorted (gaps between consecutive destinations of a block).
 decode(row_ptr, words, ptr, wbits, wbase, weighted): (ends, w) back.
 
 Plain Python integers do the bit packing, so it suits graphs up to about a
@@ -51,10 +51,11 @@ def encode(ptr, ends, w=None):
             e = a + BLK * k
             cnt = min(BLK, d - BLK * k)
             anchor = int(ends[e])
-            wd = _bits(int(ends[e + cnt - 1]) - anchor)
+            gaps = [int(ends[e + j]) - int(ends[e + j - 1]) for j in range(1, cnt)]
+            wd = _bits(max(gaps, default=0))
             val, pos = 0, 0
-            for j in range(1, cnt):
-                val |= (int(ends[e + j]) - anchor) << pos
+            for g in gaps:
+                val |= g << pos
                 pos += wd
             if w is not None:
                 for j in range(cnt):
@@ -93,7 +94,7 @@ def decode(row_ptr, words, ptr, wbits, wbase, weighted):
             anchor, c = W[r + 2 * k], W[r + 2 * k + 1]
             wd, woff = c & 63, c >> WFIELD
             p = r + 2 * nb + woff
-            ends[a + i] = anchor + (field(p, (j - 1) * wd, wd) if j else 0)
+            ends[a + i] = anchor + sum(field(p, (t - 1) * wd, wd) for t in range(1, j + 1))
             if weighted:
                 w[a + i] = wbase + field(p, (cnt - 1) * wd + j * wbits, wbits)
     return ends, w

@@ -11,21 +11,23 @@ from tests import _blocks as B
 
 def test_known_answer():
     # one row 5, 7, 12 with weights 3, 1, 2: wbase 1, wbits 2; anchor 5,
-    # offsets 2 and 7 at 3 bits (bits 0-2, 3-5), weights 2, 0, 1 at bits 6-7,
-    # 8-9, 10-11 -> 2 + 7*8 + 2*64 + 0*256 + 1*1024 = 1210
+    # gaps 2 and 5 at 3 bits (bits 0-2, 3-5), weights 2, 0, 1 at bits 6-7,
+    # 8-9, 10-11 -> 2 + 5*8 + 2*64 + 0*256 + 1*1024 = 1194
     ptr = np.array([0, 3], np.uint64)
     row_ptr, words, wbits, wbase = B.encode(ptr, np.array([5, 7, 12], np.uint32), np.array([3, 1, 2], np.uint32))
     assert (wbits, wbase) == (2, 1)
     assert row_ptr.tolist() == [0, 3]
-    assert words.tolist() == [5, 3, 1210]
-    # two blocks: 33 edges 0..32, unweighted: block 0 offsets 1..31 at 5 bits
-    # (155 bits -> 5 words), block 1 is the single edge 32 (no payload)
+    assert words.tolist() == [5, 3, 1194]
+    # two blocks: 33 edges 0, 2, 4, .., 64, unweighted: block 0 anchor 0 and
+    # 31 gaps of 2 at 2 bits (62 bits -> 2 words), block 1 the single edge 64
+    # (no payload, its payload offset 2 words)
     ptr = np.array([0, 0, 33], np.uint64)
-    row_ptr, words, _, _ = B.encode(ptr, np.arange(33, dtype=np.uint32))
-    assert row_ptr.tolist() == [0, 0, 4 + 5]
-    assert words[:4].tolist() == [0, (0 << 6) | 5, 32, (5 << 6) | 0]
-    bits = sum(int(x) << (32 * i) for i, x in enumerate(words[4:]))
-    assert [(bits >> (5 * j)) & 31 for j in range(31)] == list(range(1, 32))
+    row_ptr, words, _, _ = B.encode(ptr, np.arange(0, 66, 2, dtype=np.uint32))
+    assert row_ptr.tolist() == [0, 0, 4 + 2]
+    assert words[:4].tolist() == [0, (0 << 6) | 2, 64, (2 << 6) | 0]
+    bits = int(words[4]) | int(words[5]) << 32
+    assert [(bits >> (2 * j)) & 3 for j in range(31)] == [2] * 31
+    assert bits >> 62 == 0
 
 
 def random_graph(rng, n, degs, weighted, wmax=64, span=None):
@@ -56,14 +58,14 @@ def test_round_trip_row_lengths(weighted):
 
 def test_round_trip_wide_offsets_and_equal_weights():
     rng = np.random.default_rng(5)
-    ptr, ends, w = random_graph(rng, 50, [40, 3, 70], True, span=1 << 32)
+    ptr, ends, w = random_graph(rng, 50, [2, 40, 3, 70], True, span=1 << 32)
     ends[0] = 0
-    ends[1] = 0xFFFFFFFF                  # offset width 32 in block 0
+    ends[1] = 0xFFFFFFFF                  # row 0: one gap of 2^32 - 1, width 32
     w[:] = 9                              # weight width 0
     e, ww = B.sort_rows(ptr, ends, w)
     row_ptr, words, wbits, wbase = B.encode(ptr, e, ww)
     assert (wbits, wbase) == (0, 9)
-    assert words[1] & 63 == 32
+    assert words[1] & 63 == 32            # the gap 0 -> 0xFFFFFFFF needs 32 bits
     de, dw = B.decode(row_ptr, words, ptr, wbits, wbase, True)
     assert np.array_equal(de, e) and np.array_equal(dw, ww)
 
