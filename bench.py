@@ -579,7 +579,7 @@ def main():
             moved_bytes = min(moved_bytes, int(rowb[reach].sum()))
         bound_ms = moved_bytes / (peak * 1e9) * 1e3
         bound = {"edges_out_of_reached_vertices": moved, "fraction_of_edges": round(moved / g.m, 4),
-                 "edge_bytes": edge_bytes, "gb": round(moved_bytes / 1e9, 3),
+                 "bytes_per_edge": round(moved_bytes / max(moved, 1), 4), "gb": round(moved_bytes / 1e9, 3),
                  "ms_at_pinned_peak": round(bound_ms, 2),
                  "best_possible_speedup_over_plain": (round(results["plain"][0]["ms_per_step"] / bound_ms, 3)
                                                       if "plain" in results else None)}

@@ -1629,9 +1629,13 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
         const size_t nu = units_.size();
         const size_t ahead = (size_t)sub_->numBuffers - 1;
         size_t issued = 0;
+        // unit u is issued before its compute is enqueued, the units ahead
+        // after it: an issue that works on the host (a gather or a staged
+        // copy through the pinned slots) then runs while unit u computes
         for (size_t u = 0; u < nu; u++) {
-            while (issued < nu && issued <= u + ahead) issue(issued++);
+            while (issued < nu && issued <= u) issue(issued++);
             compute(u);
+            while (issued < nu && issued <= u + ahead) issue(issued++);
         }
         st_.partitions += (d_bounds_ && !onPlain_) ? (units_.empty() ? 0 : units_.back().part + 1)
                                                    : part_->numPartitions;
