@@ -153,7 +153,8 @@ def measure(sw, np, g, args, src, kw, names, params):
                 vals[name] = np.array(v, copy=True)
             log(f"{name} step {i}: {st.ms_total:.1f} ms e2e, {st.global_iterations} iters, {st.partitions} partitions, "
                 f"H2D {st.h2d_bytes / 1e9:.3f} GB, rounds {st.ms_rounds:.1f} ms ({st.rounds_launches} launches), "
-                f"transfer {st.ms_transfer:.1f} ms, in-place visits {st.resident_visits}")
+                f"transfer {st.ms_transfer:.1f} ms, in-place visits {st.resident_visits}, "
+                f"early read-back {st.early_readback} ({st.values_patched} values patched)")
     out = {}
     for name in names:
         ss = stats[name]
@@ -182,6 +183,10 @@ def measure(sw, np, g, args, src, kw, names, params):
                              "generate": round(float(np.mean([s.ms_generate for s in ss])), 3)},
             "mispredictions": ss[-1].mispredictions,
             "fallback_iteration": ss[-1].fallback_iteration,
+            # 1: results copied to the host during the last iterations, then
+            # the values that changed patched; 2: ... then copied in full
+            "early_readback": ss[-1].early_readback,
+            "values_patched": ss[-1].values_patched,
         }, ss, vals[name])
     for s in sessions.values():
         s.close()

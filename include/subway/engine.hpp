@@ -57,6 +57,7 @@ struct RunOptions {
     uint residentMinBytes = 0;   // learned schedule: least active edge bytes for an in-place visit (0: 8 MiB)
     int pushOnly = 0;       // BFS on a resident graph: 1 = top-down steps only (no direction switching)
     int noBlocks = 0;       // 1: whole id ranges move as plain edges even when the graph holds its block layout
+    int earlyReadback = 0;  // 0: early copy of the results on large graphs; -1: never; N > 0: always, patch list N
 };
 
 struct IterRecord {
@@ -118,6 +119,9 @@ struct RunStats {
     ull blkUpdates = 0;
     ull blockedEdges = 0;      // edges of the units moved in the block layout
     uint blockedUnits = 0;     // units moved in the block layout
+    uint earlyReadback = 0;    // 0: results copied at the end; 1: copied after the first iteration, then the
+                               // changed values patched; 2: ... then copied again in full (too many changed)
+    ull valuesPatched = 0;     // values that changed after the early copy
     uint globalIterations = 0;
     ull innerRounds = 0;       // frontier launches that selected work
     ull partitions = 0;

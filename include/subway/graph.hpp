@@ -167,6 +167,7 @@ public:
     std::string GetFileExtension(std::string fileName);
     void AssignW8(uint w8, uint index);
     void ReadGraph();
+    bool valueIsPinned() const { return valuePinned; }   // value is pinned host memory
 
 private:
     bool valuePinned = false;   // value came from hipHostMalloc
@@ -204,6 +205,7 @@ public:
     std::string GetFileExtension(std::string fileName);
     void AssignW8(uint w8, uint index);
     void ReadGraph();
+    bool valueIsPinned() const { return valuePinned; }   // value is pinned host memory
 
 private:
     bool valuePinned = false;   // value came from hipHostMalloc

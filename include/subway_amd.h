@@ -85,6 +85,12 @@ typedef struct subway_options {
                                bottom-up pull steps by frontier size (direction-optimizing BFS); 1 pushes only */
     int32_t no_blocks;      /* 1: a graph with its block layout (subway_graph_build_blocks) still moves its
                                plain edges; 0 (default) moves whole id ranges in the block layout */
+    int32_t early_readback; /* single-source runs: 0 (default) on graphs of at least 2^26 vertices, the
+                               results are copied to the host once the active edges have fallen to 1/64 of
+                               their peak, while the later iterations run, and only the values that change
+                               after that are copied at the end; -1: results are copied at the end only;
+                               N > 0: the early copy follows the first global iteration whatever the sizes,
+                               with a patch list of at most N values */
 } subway_options;
 
 typedef struct subway_stats {
@@ -141,6 +147,12 @@ typedef struct subway_stats {
     uint64_t blk_work_edges;
     uint64_t blk_work_entries;
     uint64_t blk_updates;
+    /* results: 0 copied to the host at the end of the run; 1 copied after the
+     * first global iteration (while the later ones ran), then the values that
+     * changed since were patched; 2 copied early, then again in full (more
+     * changed than the patch list holds).  values_patched: how many changed. */
+    uint64_t early_readback;
+    uint64_t values_patched;
 } subway_stats;
 
 typedef struct subway_iter_record {

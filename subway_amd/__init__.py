@@ -52,7 +52,7 @@ class _Options(ctypes.Structure):
         ("batch", ctypes.c_int32), ("batch_unroll", ctypes.c_int32), ("theta", ctypes.c_float),
         ("spin_limit", ctypes.c_uint32), ("unpacked_edges", ctypes.c_int32), ("join_window", ctypes.c_uint32),
         ("no_resident", ctypes.c_int32), ("resident_min_bytes", ctypes.c_uint32), ("push_only", ctypes.c_int32),
-        ("no_blocks", ctypes.c_int32),
+        ("no_blocks", ctypes.c_int32), ("early_readback", ctypes.c_int32),
     ]
 
 
@@ -76,6 +76,7 @@ class _Stats(ctypes.Structure):
         ("h2d_blocked_bytes", ctypes.c_uint64), ("blocked_edges", ctypes.c_uint64),
         ("ms_rounds_blocked", ctypes.c_double), ("rounds_launches_blocked", ctypes.c_uint64),
         ("blk_work_edges", ctypes.c_uint64), ("blk_work_entries", ctypes.c_uint64), ("blk_updates", ctypes.c_uint64),
+        ("early_readback", ctypes.c_uint64), ("values_patched", ctypes.c_uint64),
     ]
 
 
@@ -371,6 +372,8 @@ class RunStats:
     blk_work_edges: int = 0
     blk_work_entries: int = 0
     blk_updates: int = 0
+    early_readback: int = 0
+    values_patched: int = 0
     trace: list = field(default_factory=list)
 
 
@@ -389,7 +392,7 @@ def _set_policy(h, params):
 
 def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
              profile, rounds="persistent", rounds_grid=0, max_rounds=0, theta=0.5, spin_limit=0, packed=True,
-             join_window=0, resident=True, resident_min_bytes=0, push_only=False, blocks=True):
+             join_window=0, resident=True, resident_min_bytes=0, push_only=False, blocks=True, early_readback=0):
     o = _Options()
     lib.subway_default_options(ctypes.byref(o))
     o.algo = ALGOS[algo]
@@ -415,6 +418,7 @@ def _options(algo, mode, source, budget, buffers, threshold, init_pr, transfer, 
     o.resident_min_bytes = resident_min_bytes
     o.push_only = int(push_only)
     o.no_blocks = 0 if blocks else 1
+    o.early_readback = early_readback
     return o
 
 
@@ -440,18 +444,24 @@ class Session:
                  verbose: bool = False, advance_grid: int = 0, profile: bool = False, rounds: str = "persistent",
                  rounds_grid: int = 0, max_rounds: int = 0, theta: float = 0.5, spin_limit: int = 0,
                  packed: bool = True, params=None, join_window: int = 0, resident: bool = True,
-                 resident_min_bytes: int = 0, push_only: bool = False, blocks: bool = True):
+                 resident_min_bytes: int = 0, push_only: bool = False, blocks: bool = True,
+                 early_readback: int = 0):
         """``resident``: a graph whose edges fit one buffer stays there, and
         the learned schedule visits in place the ranges whose whole edge slice
         a buffer still holds, when they hold ``resident_min_bytes`` (0: 8 MiB)
         of active edges or more.  ``push_only``: BFS on a resident graph
         takes top-down steps only (no bottom-up steps for dense frontiers).
         ``blocks``: on a graph with its block layout (Graph.build_blocks),
-        whole id ranges move in that layout (False: as plain edges)."""
+        whole id ranges move in that layout (False: as plain edges).
+        ``early_readback``: 0 lets runs on graphs of 2^26 vertices or more
+        copy their results to the host while their last iterations run
+        (then patch what changed), -1 copies them at the end only, N > 0
+        copies them after the first iteration on any graph with a patch
+        list of N values (RunStats.early_readback says which happened)."""
         self.graph, self.algo = graph, algo
         o = _options(algo, mode, 0, budget, buffers, threshold, init_pr, transfer, policy, verbose, advance_grid,
                      profile, rounds, rounds_grid, max_rounds, theta, spin_limit, packed, join_window, resident,
-                     resident_min_bytes, push_only, blocks)
+                     resident_min_bytes, push_only, blocks, early_readback)
         h = ctypes.c_void_p()
         check(lib.subway_session_create(graph._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -337,6 +337,7 @@ static RunOptions to_options(const subway_options *o)
     r.residentMinBytes = o->resident_min_bytes;
     r.pushOnly = o->push_only;
     r.noBlocks = o->no_blocks;
+    r.earlyReadback = o->early_readback;
     if (r.algo < BFS || r.algo > PR) throw Error(ERR_ARG, "unknown algorithm");
     if (r.mode != SYNC && r.mode != ASYNC) throw Error(ERR_ARG, "unknown mode");
     return r;
@@ -413,6 +414,8 @@ static void fill_stats(const RunStats &s, subway_stats *stats, subway_iter_recor
     stats->blk_work_edges = s.blkWorkEdges;
     stats->blk_work_entries = s.blkWorkEntries;
     stats->blk_updates = s.blkUpdates;
+    stats->early_readback = s.earlyReadback;
+    stats->values_patched = s.valuesPatched;
 }
 
 extern "C" {

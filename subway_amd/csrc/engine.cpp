@@ -128,11 +128,14 @@ struct Unit {
     ull blkLo, blkWords;
 };
 
-// One compute and one copy stream per device, shared by every engine of the
-// process.  HIP maps streams onto a few hardware queues (four by default);
-// with two streams per session, a second session would put its copy and
-// compute streams on one queue and lose the transfer/compute overlap.
-// Runs of different sessions on one device therefore serialise on the pair.
+// One compute, one copy and one read-back stream per device, shared by
+// every engine of the process.  HIP maps streams onto a few hardware queues
+// (four by default); with streams per session, a second session would put
+// its copy and compute streams on one queue and lose the transfer/compute
+// overlap.  Runs of different sessions on one device therefore serialise on
+// the streams.  The read-back stream carries the early copy of a run's
+// results to the host (device to host, beside the copy stream's host to
+// device transfers).
 //
 // The pairs are destroyed explicitly, while the HIP runtime is still whole:
 // by release_shared_streams() (the C ABI's subway_shutdown, which the Python
@@ -143,7 +146,10 @@ struct Unit {
 // stream existed), so the runtime never has to reclaim live streams of this
 // library during process exit.
 static std::mutex g_streamMu;
-static std::map<int, std::pair<hipStream_t, hipStream_t>> *g_streamPairs = nullptr;
+struct DeviceStreams {
+    hipStream_t compute, copy, back;
+};
+static std::map<int, DeviceStreams> *g_streamPairs = nullptr;
 
 static void destroy_stream_pairs()
 {
@@ -153,36 +159,38 @@ static void destroy_stream_pairs()
         int cur = 0;
         if (hipGetDevice(&cur) != hipSuccess) cur = kv.first;
         (void)hipSetDevice(kv.first);
-        (void)hipStreamSynchronize(kv.second.first);
-        (void)hipStreamSynchronize(kv.second.second);
-        (void)hipStreamDestroy(kv.second.first);
-        (void)hipStreamDestroy(kv.second.second);
+        for (hipStream_t s : {kv.second.compute, kv.second.copy, kv.second.back}) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
         (void)hipSetDevice(cur);
     }
     delete g_streamPairs;
     g_streamPairs = nullptr;
 }
 
-static void shared_streams(hipStream_t &compute, hipStream_t &copy)
+static void shared_streams(hipStream_t &compute, hipStream_t &copy, hipStream_t &back)
 {
     static bool atexitRegistered = false;
     int dev = 0;
     SUBWAY_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_streamMu);
-    if (!g_streamPairs) g_streamPairs = new std::map<int, std::pair<hipStream_t, hipStream_t>>();
+    if (!g_streamPairs) g_streamPairs = new std::map<int, DeviceStreams>();
     auto it = g_streamPairs->find(dev);
     if (it == g_streamPairs->end()) {
-        hipStream_t a = nullptr, b = nullptr;
-        SUBWAY_CHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
-        SUBWAY_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
-        it = g_streamPairs->emplace(dev, std::make_pair(a, b)).first;
+        DeviceStreams s{};
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&s.compute, hipStreamNonBlocking));
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+        SUBWAY_CHECK(hipStreamCreateWithFlags(&s.back, hipStreamNonBlocking));
+        it = g_streamPairs->emplace(dev, s).first;
         if (!atexitRegistered) {
             std::atexit(destroy_stream_pairs);
             atexitRegistered = true;
         }
     }
-    compute = it->second.first;
-    copy = it->second.second;
+    compute = it->second.compute;
+    copy = it->second.copy;
+    back = it->second.back;
 }
 
 template <class E, class G>
@@ -207,7 +215,7 @@ public:
             sub_.reset(new Subgraph<E>(g_->num_nodes, g_->num_edges));
         gen_.reset(new SubgraphGenerator<E>(*g_));
         part_.reset(new Partitioner<E>());
-        shared_streams(sc_, sx_);
+        shared_streams(sc_, sx_, sd_);
         gen_->stream = sc_;
         part_->stream = sc_;
         for (int b = 0; b < 2; b++) {
@@ -251,6 +259,14 @@ public:
             SUBWAY_CHECK(hipMalloc(&d_blk_, bytes));
             SUBWAY_CHECK(hipMemcpy(d_blk_, g_->csr->blkPtr, bytes, hipMemcpyHostToDevice));
             SUBWAY_CHECK(hipDeviceSynchronize());   // a pageable copy may return before its DMA ends
+            void *dev = nullptr;
+            if (hipHostGetDevicePointer(&dev, g_->csr->blk, 0) == hipSuccess && dev) {
+                blkDev_ = (const uint32_t *)dev;
+                SUBWAY_CHECK(hipMalloc(&d_wpre_, bytes));
+                alloc_scan_scratch(rw_, ((ull)g_->num_nodes + GEN_TILE - 1) / GEN_TILE);
+            } else {
+                (void)hipGetLastError();   // not mapped: gathered units move their plain edges
+            }
         }
         // a graph that fits one buffer moves once per run and stays there
         resident_ = !o_.noResident && !o_.batch && g_->num_edges > 0 && g_->num_edges <= maxPart_;
@@ -287,6 +303,7 @@ public:
             SUBWAY_CHECK(hipMalloc(&d_unperm_, unpermRows_ * (o_.batch ? BATCH_LANES : 1) * sizeof(uint32_t)));
         }
         gen_->scanScratch().spinLimit = fr_.spinLimit;
+        if (!o_.batch) setupEarlyReadback();
         int dev = 0;
         SUBWAY_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
@@ -392,6 +409,11 @@ public:
         if (d_inPtr_) (void)hipFree(d_inPtr_);
         if (d_inAdj_) (void)hipFree(d_inAdj_);
         if (d_blk_) (void)hipFree(d_blk_);
+        if (d_wpre_) {
+            (void)hipFree(d_wpre_);
+            free_scan_scratch(rw_);
+        }
+        if (h_wb_) (void)hipHostFree(h_wb_);
         if (iterExec_) (void)hipGraphExecDestroy(iterExec_);
         if (pc_) (void)hipFree(pc_);
         for (int b = 0; b < 2; b++) {
@@ -401,6 +423,14 @@ public:
             if (stage_[b]) (void)hipHostFree(stage_[b]);
         }
         if (h_ids_) (void)hipHostFree(h_ids_);
+        if (sd_) (void)hipStreamSynchronize(sd_);
+        if (d_snap_) (void)hipFree(d_snap_);
+        if (d_chg_) (void)hipFree(d_chg_);
+        if (d_chgCount_) (void)hipFree(d_chgCount_);
+        if (h_chg_) (void)hipHostFree(h_chg_);
+        if (h_chgCount_) (void)hipHostFree(h_chgCount_);
+        if (evSnap_) (void)hipEventDestroy(evSnap_);
+        if (evBack_) (void)hipEventDestroy(evBack_);
         (void)hipEventDestroy(evX0_);
         (void)hipEventDestroy(evX1_);
         part_.reset();
@@ -419,7 +449,7 @@ private:
     std::unique_ptr<Subgraph<E>> sub_;
     std::unique_ptr<SubgraphGenerator<E>> gen_;
     std::unique_ptr<Partitioner<E>> part_;
-    hipStream_t sc_ = nullptr, sx_ = nullptr;
+    hipStream_t sc_ = nullptr, sx_ = nullptr, sd_ = nullptr;
     hipEvent_t evLoaded_[2], evFree_[2], evX0_, evX1_;
     ScanScratch fr_;
     Entry *entries_ = nullptr;
@@ -449,6 +479,18 @@ private:
     ull *d_blk_ = nullptr;
     bool blkOk_ = false;
     uint32_t blkCode_ = 0;
+    // gathered block units: the GPU reads the rows of a unit's active
+    // vertices from the graph's mapped block words (blkDev_) into the buffer
+    // in active-list order; d_wpre_[k] = words of the rows of active items
+    // before k (a scan per global iteration, when a unit needs it)
+    const uint32_t *blkDev_ = nullptr;
+    ull *d_wpre_ = nullptr;
+    ScanScratch rw_{};
+    bool wpreValid_ = false;
+    ull *h_wb_ = nullptr;        // pinned: the scan's values at the units' item bounds
+    size_t wbCap_ = 0;
+    static constexpr ull kGatherBlockedMinBytes = 16ull << 20;
+    void markGatheredBlocked();
     ull bufBytes_ = 0;
     bool bufBlocked_[2] = {false, false};
     // the GPU can read the host edges of the transfer format (mapped pinned
@@ -555,7 +597,119 @@ private:
     bool hostIdsValid_ = false;
     hipEvent_t evStaged_[2];
     ull deviceRounds();
+    // Early read-back of single-source results: once the active edges have
+    // fallen to 1/64 of their peak (and an iteration follows), the values (original order)
+    // are copied to d_snap_ and from there to the host result array on the
+    // read-back stream, while the later iterations run; at the end only the
+    // values that changed since then are copied (a list of at most chgCap_
+    // (vertex, value) pairs; more than that: the whole array again).
+    bool earlyOk_ = false;      // buffers allocated: the session may read back early
+    bool earlyRun_ = false;     // this run is a single-source run
+    bool early_ = false;        // this run's early copy was made
+    uint32_t *d_snap_ = nullptr, *d_chgCount_ = nullptr, *h_chgCount_ = nullptr;
+    uint2 *d_chg_ = nullptr, *h_chg_ = nullptr;
+    uint32_t chgCap_ = 0;
+    uint32_t *valueDev_ = nullptr;   // device address of the (mapped, pinned) host result array
+    hipEvent_t evSnap_ = nullptr, evBack_ = nullptr;
+    static constexpr ull kEarlyMinVertices = 1ull << 26;
+    static constexpr int kStoreGrid = 64;   // workgroups of the early copy's store kernel
+    void setupEarlyReadback();
+    void startEarlyReadback();
+    void finishValues();
 };
+
+template <class E, class G>
+void Engine<E, G>::setupEarlyReadback()
+{
+    const ull n = g_->num_nodes;
+    if constexpr (isPR) return;   // PR's values exist only after the last iteration
+    // small graphs copy their results in well under a millisecond; a result
+    // array in ordinary memory would make the early copy synchronous
+    if (o_.earlyReadback < 0 || n == 0 || !g_->valueIsPinned()) return;
+    if (o_.earlyReadback == 0 && n < kEarlyMinVertices) return;
+    chgCap_ = (uint32_t)std::min<ull>(n, o_.earlyReadback > 0 ? (ull)o_.earlyReadback : 1ull << 22);
+    if (hipHostMalloc((void **)&h_chg_, (size_t)chgCap_ * sizeof(uint2), 0) != hipSuccess ||
+        hipHostMalloc((void **)&h_chgCount_, sizeof(uint32_t), 0) != hipSuccess) {
+        (void)hipGetLastError();   // pinned memory is short: results are copied at the end
+        if (h_chg_) (void)hipHostFree(h_chg_);
+        h_chg_ = nullptr;
+        h_chgCount_ = nullptr;
+        return;
+    }
+    if (hipHostGetDevicePointer((void **)&valueDev_, g_->value, 0) != hipSuccess || !valueDev_ ||
+        ((uintptr_t)valueDev_ & 15u)) {
+        (void)hipGetLastError();   // the result array is not mapped: results are copied at the end
+        (void)hipHostFree(h_chg_);
+        (void)hipHostFree(h_chgCount_);
+        h_chg_ = nullptr;
+        h_chgCount_ = nullptr;
+        valueDev_ = nullptr;
+        return;
+    }
+    SUBWAY_CHECK(hipMalloc(&d_snap_, n * sizeof(uint32_t)));
+    SUBWAY_CHECK(hipMalloc(&d_chg_, (size_t)chgCap_ * sizeof(uint2)));
+    SUBWAY_CHECK(hipMalloc(&d_chgCount_, sizeof(uint32_t)));
+    SUBWAY_CHECK(hipEventCreateWithFlags(&evSnap_, hipEventDisableTiming));
+    SUBWAY_CHECK(hipEventCreateWithFlags(&evBack_, hipEventDisableTiming));
+    earlyOk_ = true;
+}
+
+// The values as they are now (the later iterations change few of them), in
+// the original order, to d_snap_ (compute stream), then to the host result
+// array by a store kernel on the read-back stream (not a copy engine: the
+// host's small read-backs of the later iterations must not queue behind
+// it).  Nothing writes d_snap_ until the next run.
+template <class E, class G>
+void Engine<E, G>::startEarlyReadback()
+{
+    const ull n = g_->num_nodes;
+    if (d_perm_) launch_unpermute((const uint32_t *)g_->d_value, d_perm_, 0, n, 1, d_snap_, sc_);
+    else SUBWAY_CHECK(hipMemcpyAsync(d_snap_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
+    SUBWAY_CHECK(hipEventRecord(evSnap_, sc_));
+    SUBWAY_CHECK(hipStreamWaitEvent(sd_, evSnap_, 0));
+    launch_store_host(d_snap_, valueDev_, n, kStoreGrid, sd_);
+    SUBWAY_CHECK(hipEventRecord(evBack_, sd_));
+    early_ = true;
+}
+
+// The run's values into the host result array: all of them, or after an
+// early copy the ones that changed since (patched on the host once the early
+// copy has landed).
+template <class E, class G>
+void Engine<E, G>::finishValues()
+{
+    if (!early_) {
+        copyValuesOut(g_->d_value, g_->value, 1);
+        return;
+    }
+    early_ = false;
+    const ull n = g_->num_nodes;
+    launch_fill_u32(d_chgCount_, 0u, 1, sc_);
+    launch_value_changes((const uint32_t *)g_->d_value, d_perm_, d_snap_, n, d_chgCount_, d_chg_, chgCap_, sc_);
+    SUBWAY_CHECK(hipMemcpyAsync(h_chgCount_, d_chgCount_, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
+    SUBWAY_CHECK(hipStreamSynchronize(sc_));
+    const uint32_t c = *h_chgCount_;
+    SUBWAY_CHECK(hipEventSynchronize(evBack_));   // the early copy has landed
+    st_.earlyReadback = 1;
+    st_.valuesPatched = c;
+    if (c > chgCap_) {
+        st_.earlyReadback = 2;
+        copyValuesOut(g_->d_value, g_->value, 1);
+        return;
+    }
+    if (c == 0) return;
+    SUBWAY_CHECK(hipMemcpyAsync(h_chg_, d_chg_, (size_t)c * sizeof(uint2), hipMemcpyDeviceToHost, sc_));
+    SUBWAY_CHECK(hipStreamSynchronize(sc_));
+    uint32_t *val = (uint32_t *)g_->value;
+    const uint2 *chg = h_chg_;
+    if (c < (1u << 16)) {
+        for (uint32_t i = 0; i < c; i++) val[chg[i].x] = chg[i].y;
+    } else {
+        parallel_for(c, [&](ull a, ull b, int) {
+            for (ull i = a; i < b; i++) val[chg[i].x] = chg[i].y;
+        });
+    }
+}
 
 template <class E, class G>
 void Engine<E, G>::initState()
@@ -582,6 +736,7 @@ template <class E, class G>
 void Engine<E, G>::generateAndPartition()
 {
     hostIdsValid_ = false;
+    wpreValid_ = false;
     if (batchK_ && !o_.profile && !d_bounds_) {
         // A batched iteration starts with the same launches and copies every
         // time (union generate, error words, partition table): recorded once
@@ -750,6 +905,58 @@ void Engine<E, G>::markBlocked(Unit &u) const
     u.blocked = true;
     u.blkLo = lo;
     u.blkWords = words;
+}
+
+// Units gathered from the active list (not a contiguous slice) move their
+// rows in the block layout when that takes fewer bytes than their plain
+// edges and fits a buffer: the row word counts of the active list are
+// scanned once per global iteration, and the scan's values at each unit's
+// item bounds come back to size the unit.
+template <class E, class G>
+void Engine<E, G>::markGatheredBlocked()
+{
+    if (!d_wpre_ || o_.transfer == TRANSFER_HOST) return;
+    std::vector<size_t> cand;
+    for (size_t i = 0; i < units_.size(); i++) {
+        const Unit &u = units_[i];
+        if (!u.dma && !u.hub && !u.blocked && u.count > 0 && u.nodes && u.kend > u.kbeg) cand.push_back(i);
+    }
+    if (cand.empty()) return;
+    if (edgesMapped_) {
+        // the plain edges are gathered by the GPU as well: the scan and its
+        // read-back pay only when the units are large (16 MiB, or an eighth
+        // of a buffer when buffers are smaller than 128 MiB)
+        ull plain = 0;
+        for (size_t i : cand) plain += units_[i].count * (ull)eb_;
+        if (plain < std::min<ull>(kGatherBlockedMinBytes, bufBytes_ / 8)) return;
+    }
+    const ull A = sub_->numActiveNodes;
+    if (!wpreValid_) {
+        launch_row_words(A, sub_->d_activeNodes, d_blk_, d_wpre_, rw_, sc_);
+        wpreValid_ = true;
+    }
+    if (2 * cand.size() > wbCap_) {
+        if (h_wb_) SUBWAY_CHECK(hipHostFree(h_wb_));
+        wbCap_ = std::max<size_t>(2 * cand.size(), 64);
+        SUBWAY_CHECK(hipHostMalloc((void **)&h_wb_, wbCap_ * sizeof(ull), 0));
+    }
+    for (size_t c = 0; c < cand.size(); c++) {
+        const Unit &u = units_[cand[c]];
+        if (u.kend > A) throw Error(ERR_INTERNAL, "gathered unit past the active list");
+        SUBWAY_CHECK(hipMemcpyAsync(&h_wb_[2 * c], d_wpre_ + u.kbeg, sizeof(ull), hipMemcpyDeviceToHost, sc_));
+        SUBWAY_CHECK(hipMemcpyAsync(&h_wb_[2 * c + 1], d_wpre_ + u.kend, sizeof(ull), hipMemcpyDeviceToHost, sc_));
+    }
+    SUBWAY_CHECK(hipStreamSynchronize(sc_));
+    for (size_t c = 0; c < cand.size(); c++) {
+        Unit &u = units_[cand[c]];
+        const ull lo = h_wb_[2 * c], hi = h_wb_[2 * c + 1];
+        if (hi < lo) throw Error(ERR_INTERNAL, "row word scan is not monotone");
+        const ull words = hi - lo;
+        if ((words + 2) * 4 > bufBytes_ || words + 2 >= (1ull << 32) || words * 4 >= u.count * (ull)eb_) continue;
+        u.blocked = true;
+        u.blkLo = lo;
+        u.blkWords = words;
+    }
 }
 
 // Learned policy, per global iteration, after the policy kernel: the
@@ -1110,7 +1317,18 @@ void Engine<E, G>::issue(size_t idx)
         const bool split = fmt_ == EDGE_SPLIT;
         // split format: destinations at the buffer's start, weights at byte 4 * capacity
         char *dstW = split ? dst + 4ull * maxPart_ : nullptr;
-        if (u.blocked) {
+        if (u.blocked && !u.dma) {
+            // the rows of the unit's active vertices, read by the GPU from
+            // the mapped block words, packed in active-list order
+            const size_t bb = (size_t)u.blkWords * 4;
+            launch_gather(4, sub_->d_activeNodes, d_wpre_, u.kbeg, u.kend, d_blk_, blkDev_, dst, u.blkLo, u.blkWords,
+                          sx_);
+            st_.h2dGatherBytes += bb;
+            st_.h2dBlockedBytes += bb;
+            st_.blockedEdges += u.count;
+            st_.blockedUnits++;
+            st_.h2dBytes += bb;
+        } else if (u.blocked) {
             const size_t bb = (size_t)u.blkWords * 4;
             const hipError_t ce = hipMemcpyAsync(dst, g_->csr->blk + u.blkLo, bb, hipMemcpyHostToDevice, sx_);
             if (ce != hipSuccess) {
@@ -1293,8 +1511,10 @@ void Engine<E, G>::frontier(int first, uint8_t *labA, uint8_t *labB, const Unit 
     a.chunkMap = chunkMap_;
     a.first = first;
     if (un.blocked) {
-        a.blk = d_blk_;
-        a.blkBase = g_->csr->blkPtr[un.idLo];
+        // DMA units: row offsets by vertex id; gathered units: by item
+        a.blkByItem = un.dma ? 0u : 1u;
+        a.blk = un.dma ? d_blk_ : d_wpre_ + un.kbeg;
+        a.blkBase = un.blkLo;
     }
     if (o_.profile) pFr_.begin(sc_);
     launch_frontier(a, fr_, sc_);
@@ -1373,8 +1593,9 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         if (o_.profile) ev.begin(sc_);
         l.packCode = codeOf(un);
         if (un.blocked) {
-            l.blk = d_blk_;
-            l.blkBase = g_->csr->blkPtr[un.idLo];
+            l.blkByItem = un.dma ? 0u : 1u;
+            l.blk = un.dma ? d_blk_ : d_wpre_ + un.kbeg;
+            l.blkBase = un.blkLo;
         }
         l.joinTicks = o_.joinWindow;
         l.nv = g_->num_nodes;
@@ -1584,6 +1805,12 @@ template <class E, class G>
 void Engine<E, G>::resetRun()
 {
     if (broken_) throw Error(ERR_INTERNAL, "a device wait of an earlier run gave up; the session must be rebuilt");
+    if (early_) {
+        // an earlier run ended by an error after its early copy: it must
+        // land before this run writes the result array
+        SUBWAY_CHECK(hipEventSynchronize(evBack_));
+        early_ = false;
+    }
     st_ = RunStats();
     residentLoaded_ = false;
     bufCount_[0] = bufCount_[1] = 0;   // every run moves its edges at least once
@@ -1615,7 +1842,9 @@ template <class E, class G>
 RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
 {
     (void)t0;
+    ull peakActiveEdges = 0;
     while (sub_->numActiveNodes > 0) {
+        peakActiveEdges = std::max<ull>(peakActiveEdges, sub_->numActiveEdges);
         const auto ti = clk::now();
         st_.globalIterations++;
         IterRecord rec{};
@@ -1626,6 +1855,7 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
         const ull h2d0 = st_.h2dBytes;
         if (d_bounds_) applyPolicy(rec);
         buildUnits();
+        markGatheredBlocked();
         const size_t nu = units_.size();
         const size_t ahead = (size_t)sub_->numBuffers - 1;
         size_t issued = 0;
@@ -1641,6 +1871,9 @@ RunStats Engine<E, G>::loop(const std::chrono::steady_clock::time_point &t0)
                                                    : part_->numPartitions;
         rec.h2dBytes = st_.h2dBytes - h2d0;
         generateAndPartition();
+        if (earlyRun_ && !early_ && sub_->numActiveNodes > 0 &&
+            (o_.earlyReadback > 0 || sub_->numActiveEdges * 64 <= peakActiveEdges))
+            startEarlyReadback();
         rec.msIteration = ms_since(ti);
         st_.trace.push_back(rec);
     }
@@ -1697,9 +1930,11 @@ RunStats Engine<E, G>::run(uint source)
             SUBWAY_CHECK(hipMemcpyAsync((uint8_t *)g_->d_label1 + src_, &one, 1, hipMemcpyHostToDevice, sc_));
         }
     }
+    earlyRun_ = earlyOk_;
     loop(t0);
+    earlyRun_ = false;
     if constexpr (isPR) launch_finish_pr(n, g_->d_value, d_sum_, g_->d_delta, d_resid_, sc_);
-    copyValuesOut(g_->d_value, g_->value, 1);
+    finishValues();
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     st_.msTotal = ms_since(t0);
     finishStats();

@@ -539,6 +539,7 @@ struct FrontierLabelOp {
     u32 idBase;    // items are ids idBase + i when nodes is null (a whole id range)
     const u64 *blk;   // block layout in the buffer: row word offsets (null: plain edges)
     u64 blkBase;      // ... of the buffer's first row
+    u32 blkByItem;    // blk is indexed by item (rows gathered in item order), else by vertex id
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -556,7 +557,7 @@ struct FrontierLabelOp {
                     if (p1 > p0) {
                         mask |= 1u << j;
                         w[j] = p1 - p0;
-                        it[j].loc = blk ? (u32)(blk[id] - blkBase) : (u32)(p0 - fromEdge);
+                        it[j].loc = blk ? (u32)(blk[blkByItem ? i : (u64)id] - blkBase) : (u32)(p0 - fromEdge);
                         it[j].val = mld<COH>(&value[id]);
                     }
                 }
@@ -592,6 +593,7 @@ struct FrontierPrOp {
     u32 idBase;
     const u64 *blk;   // block layout in the buffer: row word offsets (null: plain edges)
     u64 blkBase;
+    u32 blkByItem;    // blk is indexed by item (rows gathered in item order), else by vertex id
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *it, u64 *w, u32 &mask) const
     {
@@ -609,7 +611,7 @@ struct FrontierPrOp {
                         const float share = (float)(d / (double)(p1 - p0) * 0.85);
                         mask |= 1u << j;
                         w[j] = p1 - p0;
-                        it[j].loc = blk ? (u32)(blk[id] - blkBase) : (u32)(p0 - fromEdge);
+                        it[j].loc = blk ? (u32)(blk[blkByItem ? i : (u64)id] - blkBase) : (u32)(p0 - fromEdge);
                         it[j].val = __float_as_uint(share);
                     }
                 }
@@ -1271,6 +1273,7 @@ struct RoundsArgs {
     u32 packCode;         // EDGE_PACKED: destination bits
     const u64 *blk;        // EDGE_BLOCKS: row word offsets (null otherwise)
     u64 blkBase;           // ... of the buffer's first row
+    u32 blkByItem;         // blk indexed by item, else by vertex id
     u32 nv;                // vertices of the graph (decoded destinations are checked against it)
 };
 
@@ -1308,13 +1311,13 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
             if (t >= numTiles) break;
             if constexpr (ALGO == ALGO_PR) {
                 FrontierPrOp<true> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc,
-                                      a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase};
+                                      a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase, a.blkByItem};
                 if (!scan_tile<FrontierPrOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td, a.sc,
                                                                  wt))
                     break;
             } else {
                 FrontierLabelOp<true> op{a.nodes, a.ptr, a.fromEdge, labIn, rg == 0 ? a.l2 : nullptr, a.value,
-                                   a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase};
+                                   a.entries, a.chunkMap, a.idBase, a.blk, a.blkBase, a.blkByItem};
                 if (!scan_tile<FrontierLabelOp<true>, SCAN_BLOCK, FR_IPT>(op, a.n, t, epoch0 + r, numTiles - 1, a.td,
                                                                     a.sc, wt))
                     break;
@@ -1990,6 +1993,54 @@ __global__ void unpermute_kernel(const u32 *in, const u32 *perm, u64 v0, u64 cou
     }
 }
 
+// The values that differ from an earlier copy of them: snap[v] holds vertex
+// v's value (original order) as it was copied; cur is in[perm[v]] (in[v]
+// without a renumbering).  Every changed vertex appends (v, cur) to `list`
+// (its first `cap` entries are written) and counts in *count, one atomic per
+// wave.
+__global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const u32 *perm, const u32 *snap, u64 n,
+                                                            u32 *count, uint2 *list, u32 cap)
+{
+    const int lane = lane_id();
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    // every lane of a wave runs the same number of trips (the ballot below)
+    const u64 trips = (n + stride - 1) / stride;
+    u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (u64 t = 0; t < trips; t++, v += stride) {
+        bool changed = false;
+        u32 cur = 0;
+        if (v < n) {
+            cur = in[perm ? (u64)perm[v] : v];
+            changed = cur != snap[v];
+        }
+        const u64 mask = __ballot(changed);
+        if (mask == 0) continue;
+        u32 base = 0;
+        if (lane == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(count, (u32)__popcll(mask));
+        base = (u32)__shfl((int)base, __ffsll((unsigned long long)mask) - 1);
+        if (changed) {
+            const u32 at = base + (u32)__popcll(mask & ((1ull << lane) - 1ull));
+            if (at < cap) list[at] = make_uint2((u32)v, cur);
+        }
+    }
+}
+
+// n words from device memory to mapped pinned host memory, written by the
+// CUs (16-byte nontemporal stores) rather than by a copy engine: a copy
+// engine serves device-to-host copies in order, so one long copy would hold
+// back the small read-backs the host waits for between iterations.
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void store_host_kernel(const u32 *src, u32 *dst, u64 n)
+{
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 n4 = n >> 2;
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+    u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
+    for (u64 i = t0; i < n4; i += stride) __builtin_nontemporal_store(s4[i], &d4[i]);
+    for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[i];
+}
+
 // FP32 result values from the FP64 running sums
 __global__ void finish_pr_kernel(u64 n, float *value, const double *sum, float *deltaOut, const double *resid)
 {
@@ -2138,6 +2189,38 @@ static void run_generator(const Op &op, ull n, ScanScratch &s, hipStream_t st)
     SUBWAY_CHECK(hipGetLastError());
 }
 
+// Word offsets of gathered block rows: item k of the active list is vertex
+// ids[k], whose row takes rowPtr[ids[k] + 1] - rowPtr[ids[k]] words;
+// wpre[k] = the words of items 0 .. k - 1, wpre[n] = all of them.  Every
+// item is selected, so the generator's scan is a plain exclusive scan.
+struct RowWordsOp {
+    typedef Empty Item;
+    static constexpr bool TALLY = true;
+    const u32 *ids;
+    const u64 *rowPtr;
+    u64 *wpre;
+
+    __device__ __forceinline__ void load(u64 i0, u64 n, Item *, u64 *w, u32 &mask) const
+    {
+        for (int j = 0; j < GEN_IPT; j++) {
+            const u64 i = i0 + j;
+            if (i < n) {
+                const u32 v = ids[i];
+                mask |= 1u << j;
+                w[j] = rowPtr[(u64)v + 1] - rowPtr[v];
+            }
+        }
+    }
+    __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &, u64) const { wpre[k] = pre; }
+    __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *) const { wpre[cnt] = sum; }
+};
+
+void launch_row_words(ull n, const uint32_t *ids, const ull *rowPtr, ull *wpre, ScanScratch &s, hipStream_t st)
+{
+    RowWordsOp op{ids, (const u64 *)rowPtr, (u64 *)wpre};
+    run_generator(op, n, s, st);
+}
+
 void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
                             const uint32_t *outDegree, uint32_t *activeNodes,
                             ull *activeNodesPointer, ScanScratch &s, hipStream_t st)
@@ -2218,13 +2301,13 @@ void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st)
     check_tiles(s, tiles, st);
     if (a.algo == ALGO_PR) {
         FrontierPrOp<false> op{a.nodes, a.ptr, a.fromEdge, a.sum, a.delta, a.acc, a.entries,
-                               a.chunkMap, a.idBase, (const u64 *)a.blk, (u64)a.blkBase};
+                               a.chunkMap, a.idBase, (const u64 *)a.blk, (u64)a.blkBase, a.blkByItem};
         hipLaunchKernelGGL((select_scan_kernel<FrontierPrOp<false>, SCAN_BLOCK, FR_IPT>), dim3((unsigned)tiles),
                            dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td, a.first,
                            s.spinLimit);
     } else {
         FrontierLabelOp<false> op{a.nodes, a.ptr, a.fromEdge, a.labA, a.labB, a.value, a.entries, a.chunkMap,
-                                  a.idBase, (const u64 *)a.blk, (u64)a.blkBase};
+                                  a.idBase, (const u64 *)a.blk, (u64)a.blkBase, a.blkByItem};
         hipLaunchKernelGGL((select_scan_kernel<FrontierLabelOp<false>, SCAN_BLOCK, FR_IPT>),
                            dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, op, (u64)a.n, s.ctl, s.td,
                            a.first, s.spinLimit);
@@ -2294,6 +2377,22 @@ void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull coun
     if (count == 0) return;
     hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(count * width, 256, 16384)), dim3(256), 0, st, in, perm,
                        (u64)v0, (u64)count, width, out);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(store_host_kernel, dim3(grid), dim3(256), 0, st, src, dstMapped, (u64)n);
+    SUBWAY_CHECK(hipGetLastError());
+}
+
+void launch_value_changes(const uint32_t *in, const uint32_t *perm, const uint32_t *snap, ull n, uint32_t *count,
+                          uint2 *list, uint32_t cap, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(value_changes_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, in, perm, snap, (u64)n,
+                       count, list, cap);
     SUBWAY_CHECK(hipGetLastError());
 }
 
@@ -2399,6 +2498,7 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.packCode = l.packCode;
     a.blk = (const u64 *)l.blk;
     a.blkBase = (u64)l.blkBase;
+    a.blkByItem = l.blkByItem;
     a.nv = l.nv;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};

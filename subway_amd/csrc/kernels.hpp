@@ -102,6 +102,9 @@ void alloc_scan_scratch(ScanScratch &s, ull maxTiles);
 void free_scan_scratch(ScanScratch &s);
 
 // ---- subgraph generator (flags -> scan -> compaction, one pass)
+// Exclusive scan of the block-row word counts of the active list (n items,
+// ids), into wpre[0 .. n] (uses the generator's scan scratch).
+void launch_row_words(ull n, const uint32_t *ids, const ull *rowPtr, ull *wpre, ScanScratch &s, hipStream_t st);
 void launch_generate_labels(ull n, const uint8_t *l1, const uint8_t *l2,
                             const uint32_t *outDegree, uint32_t *activeNodes,
                             ull *activeNodesPointer, ScanScratch &s, hipStream_t st);
@@ -181,6 +184,7 @@ struct FrontierArgs {
     uint32_t idBase;                 // nodes == null: the items are ids idBase .. idBase + n - 1
     const ull *blk;                  // EDGE_BLOCKS buffers: row word offsets of the graph (else null)
     ull blkBase;                     // ... of the buffer's first row
+    uint32_t blkByItem;              // 1: blk is indexed by item (rows gathered in item order), 0: by vertex id
 };
 void launch_frontier(const FrontierArgs &a, ScanScratch &s, hipStream_t st);
 
@@ -225,6 +229,7 @@ struct RoundsLaunch {
     uint32_t packCode;    // EDGE_PACKED: shift | weight base << 5
     const ull *blk;        // EDGE_BLOCKS buffers: row word offsets of the graph (else null)
     ull blkBase;           // ... of the buffer's first row
+    uint32_t blkByItem;    // 1: blk is indexed by item (rows gathered in item order), 0: by vertex id
     uint32_t nv;           // vertices of the graph (block decode checks destinations against it)
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
@@ -277,6 +282,14 @@ void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_
 // batched rows) of a renumbered graph in the original order.
 void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull count, uint32_t width, uint32_t *out,
                       hipStream_t st);
+// n words into mapped pinned host memory (dstMapped: its device address),
+// stored by `grid` workgroups; src and dstMapped 16-byte aligned.
+void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st);
+// The vertices whose value in[perm[v]] (in[v] when perm is null) differs
+// from snap[v]: (v, value) pairs into list (the first cap of them), their
+// number added to *count (which the caller zeroes).
+void launch_value_changes(const uint32_t *in, const uint32_t *perm, const uint32_t *snap, ull n, uint32_t *count,
+                          uint2 *list, uint32_t cap, hipStream_t st);
 // PR: the running sum and the residual are FP64 (an FP32 residual at a hub
 // rounds away shares below half its ulp).
 void launch_init_pr_state(ull n, double *sum, double *delta, double init, hipStream_t st);

@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_library(sw):
     got = [ctypes.c_uint32() for _ in range(3)]
     _lib.lib.subway_struct_sizes(*(ctypes.cast(ctypes.byref(x), ctypes.POINTER(ctypes.c_uint32)) for x in got))
     assert [x.value for x in got] == [ctypes.sizeof(_Options), ctypes.sizeof(_Stats), ctypes.sizeof(_IterRecord)]
-    assert ctypes.sizeof(_Stats) == _Stats.blk_updates.offset + 8
+    assert ctypes.sizeof(_Stats) == _Stats.values_patched.offset + 8
 
 
 def test_missing_file_is_an_io_error(sw, tmp_path):

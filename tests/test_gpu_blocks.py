@@ -250,3 +250,31 @@ def test_row_offsets_end_at_the_word_count(sw, gpu):
         sub_rp = np.array([0, r1 - r0], np.uint64)
         de, dw = B.decode(sub_rp, words[r0:r1], sub_ptr, wbits, wbase, True)
         assert np.array_equal(de, ends[a:b]) and np.array_equal(dw, w[a:b]), v
+
+
+@pytest.mark.parametrize("algo,mode", APPS)
+def test_gathered_units_move_block_rows(sw, gpu, algo, mode):
+    """Units gathered from the active list: the GPU reads the block rows of
+    their active vertices from the mapped block words (transfer="zerocopy":
+    no DMA at all, so every blocked byte was gathered), with the oracle's
+    answers, and the same values as plain gathers of the edges."""
+    weighted = algo in ("sssp", "sswp")
+    g = sw.Graph.rmat(14, 16, seed=zlib.crc32(f"g{algo}".encode()) & 0xFFFF, weighted=weighted,
+                      symmetrize=(algo == "cc"))
+    ptr, adj, w = g.csr()
+    src = g.max_degree_vertex if algo != "cc" else 0
+    exp = expected(algo, ptr, adj, w, src)
+    g.build_blocks()
+    kw = {"threshold": 1e-6} if algo == "pr" else {}
+    for transfer in ("zerocopy", "auto"):
+        s = sw.Session(g, algo, mode=mode, budget=96 << 10, transfer=transfer, **kw)
+        for _ in range(2):
+            v, st = s.run(src)
+            check(algo, v, exp)
+            if transfer == "zerocopy":
+                assert st.h2d_dma_bytes == 0
+                assert st.blocked_units > 0 and 0 < st.h2d_blocked_bytes <= st.h2d_gather_bytes
+        s.close()
+    vp, _ = sw.Session(g, algo, mode=mode, budget=96 << 10, transfer="zerocopy", blocks=False, **kw).run(src)
+    if algo != "pr":
+        assert np.array_equal(v, vp)
