@@ -425,6 +425,7 @@ public:
         if (h_ids_) (void)hipHostFree(h_ids_);
         if (sd_) (void)hipStreamSynchronize(sd_);
         if (d_snap_) (void)hipFree(d_snap_);
+        if (d_snapInt_) (void)hipFree(d_snapInt_);
         if (d_chg_) (void)hipFree(d_chg_);
         if (d_chgCount_) (void)hipFree(d_chgCount_);
         if (h_chg_) (void)hipHostFree(h_chg_);
@@ -606,7 +607,7 @@ private:
     bool earlyOk_ = false;      // buffers allocated: the session may read back early
     bool earlyRun_ = false;     // this run is a single-source run
     bool early_ = false;        // this run's early copy was made
-    uint32_t *d_snap_ = nullptr, *d_chgCount_ = nullptr, *h_chgCount_ = nullptr;
+    uint32_t *d_snap_ = nullptr, *d_snapInt_ = nullptr, *d_chgCount_ = nullptr, *h_chgCount_ = nullptr;
     uint2 *d_chg_ = nullptr, *h_chg_ = nullptr;
     uint32_t chgCap_ = 0;
     uint32_t *valueDev_ = nullptr;   // device address of the (mapped, pinned) host result array
@@ -647,6 +648,16 @@ void Engine<E, G>::setupEarlyReadback()
         return;
     }
     SUBWAY_CHECK(hipMalloc(&d_snap_, n * sizeof(uint32_t)));
+    if (d_perm_) {
+        // renumbered: the changes are found in the internal order (a second
+        // copy of the values) and named through the inverse map
+        SUBWAY_CHECK(hipMalloc(&d_snapInt_, n * sizeof(uint32_t)));
+        if (!d_iperm_) {
+            SUBWAY_CHECK(hipMalloc(&d_iperm_, n * sizeof(uint32_t)));
+            SUBWAY_CHECK(hipMemcpy(d_iperm_, g_->csr->iperm, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+            SUBWAY_CHECK(hipDeviceSynchronize());   // a pageable copy may return before its DMA ends
+        }
+    }
     SUBWAY_CHECK(hipMalloc(&d_chg_, (size_t)chgCap_ * sizeof(uint2)));
     SUBWAY_CHECK(hipMalloc(&d_chgCount_, sizeof(uint32_t)));
     SUBWAY_CHECK(hipEventCreateWithFlags(&evSnap_, hipEventDisableTiming));
@@ -663,8 +674,12 @@ template <class E, class G>
 void Engine<E, G>::startEarlyReadback()
 {
     const ull n = g_->num_nodes;
-    if (d_perm_) launch_unpermute((const uint32_t *)g_->d_value, d_perm_, 0, n, 1, d_snap_, sc_);
-    else SUBWAY_CHECK(hipMemcpyAsync(d_snap_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
+    if (d_perm_) {
+        SUBWAY_CHECK(hipMemcpyAsync(d_snapInt_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
+        launch_unpermute((const uint32_t *)g_->d_value, d_perm_, 0, n, 1, d_snap_, sc_);
+    } else {
+        SUBWAY_CHECK(hipMemcpyAsync(d_snap_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
+    }
     SUBWAY_CHECK(hipEventRecord(evSnap_, sc_));
     SUBWAY_CHECK(hipStreamWaitEvent(sd_, evSnap_, 0));
     launch_store_host(d_snap_, valueDev_, n, kStoreGrid, sd_);
@@ -685,7 +700,8 @@ void Engine<E, G>::finishValues()
     early_ = false;
     const ull n = g_->num_nodes;
     launch_fill_u32(d_chgCount_, 0u, 1, sc_);
-    launch_value_changes((const uint32_t *)g_->d_value, d_perm_, d_snap_, n, d_chgCount_, d_chg_, chgCap_, sc_);
+    launch_value_changes((const uint32_t *)g_->d_value, d_perm_ ? d_iperm_ : nullptr, d_perm_ ? d_snapInt_ : d_snap_, n,
+                         d_chgCount_, d_chg_, chgCap_, sc_);
     SUBWAY_CHECK(hipMemcpyAsync(h_chgCount_, d_chgCount_, sizeof(uint32_t), hipMemcpyDeviceToHost, sc_));
     SUBWAY_CHECK(hipStreamSynchronize(sc_));
     const uint32_t c = *h_chgCount_;

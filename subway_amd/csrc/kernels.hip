@@ -1993,12 +1993,12 @@ __global__ void unpermute_kernel(const u32 *in, const u32 *perm, u64 v0, u64 cou
     }
 }
 
-// The values that differ from an earlier copy of them: snap[v] holds vertex
-// v's value (original order) as it was copied; cur is in[perm[v]] (in[v]
-// without a renumbering).  Every changed vertex appends (v, cur) to `list`
-// (its first `cap` entries are written) and counts in *count, one atomic per
-// wave.
-__global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const u32 *perm, const u32 *snap, u64 n,
+// The values that differ from an earlier copy of them: snap[v] holds
+// in[v] as it was copied.  Every changed v appends (name[v], in[v]) to
+// `list` (name = the original id of a renumbered graph's vertex v; null:
+// v itself; the first `cap` entries are written) and counts in *count, one
+// atomic per wave.
+__global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const u32 *name, const u32 *snap, u64 n,
                                                             u32 *count, uint2 *list, u32 cap)
 {
     const int lane = lane_id();
@@ -2010,7 +2010,7 @@ __global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const
         bool changed = false;
         u32 cur = 0;
         if (v < n) {
-            cur = in[perm ? (u64)perm[v] : v];
+            cur = in[v];
             changed = cur != snap[v];
         }
         const u64 mask = __ballot(changed);
@@ -2020,7 +2020,7 @@ __global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const
         base = (u32)__shfl((int)base, __ffsll((unsigned long long)mask) - 1);
         if (changed) {
             const u32 at = base + (u32)__popcll(mask & ((1ull << lane) - 1ull));
-            if (at < cap) list[at] = make_uint2((u32)v, cur);
+            if (at < cap) list[at] = make_uint2(name ? name[v] : (u32)v, cur);
         }
     }
 }
@@ -2191,33 +2191,45 @@ static void run_generator(const Op &op, ull n, ScanScratch &s, hipStream_t st)
 
 // Word offsets of gathered block rows: item k of the active list is vertex
 // ids[k], whose row takes rowPtr[ids[k] + 1] - rowPtr[ids[k]] words;
-// wpre[k] = the words of items 0 .. k - 1, wpre[n] = all of them.  Every
-// item is selected, so the generator's scan is a plain exclusive scan.
-struct RowWordsOp {
+// wpre[k] = the words of items 0 .. k - 1, wpre[n] = all of them.  The row
+// lengths are read by one thread per item (the reads are random), then the
+// generator's scan runs over them in place: every item is selected, its
+// load reads a[i] before its emit writes a[i], and items are disjoint
+// between threads.
+__global__ void row_words_kernel(const u32 *ids, const u64 *rowPtr, u64 n, u64 *out)
+{
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
+        const u32 v = ids[k];
+        out[k] = rowPtr[(u64)v + 1] - rowPtr[v];
+    }
+}
+
+struct ScanInPlaceOp {
     typedef Empty Item;
     static constexpr bool TALLY = true;
-    const u32 *ids;
-    const u64 *rowPtr;
-    u64 *wpre;
+    u64 *a;
 
     __device__ __forceinline__ void load(u64 i0, u64 n, Item *, u64 *w, u32 &mask) const
     {
         for (int j = 0; j < GEN_IPT; j++) {
             const u64 i = i0 + j;
             if (i < n) {
-                const u32 v = ids[i];
                 mask |= 1u << j;
-                w[j] = rowPtr[(u64)v + 1] - rowPtr[v];
+                w[j] = a[i];
             }
         }
     }
-    __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &, u64) const { wpre[k] = pre; }
-    __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *) const { wpre[cnt] = sum; }
+    __device__ __forceinline__ void emit(u64 k, u64 pre, u64, const Item &, u64) const { a[k] = pre; }
+    __device__ __forceinline__ void finish(u64 cnt, u64 sum, ScanCtl *) const { a[cnt] = sum; }
 };
 
 void launch_row_words(ull n, const uint32_t *ids, const ull *rowPtr, ull *wpre, ScanScratch &s, hipStream_t st)
 {
-    RowWordsOp op{ids, (const u64 *)rowPtr, (u64 *)wpre};
+    if (n == 0) return;
+    hipLaunchKernelGGL(row_words_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, ids, (const u64 *)rowPtr,
+                       (u64)n, (u64 *)wpre);
+    SUBWAY_CHECK(hipGetLastError());
+    ScanInPlaceOp op{(u64 *)wpre};
     run_generator(op, n, s, st);
 }
 
@@ -2387,11 +2399,11 @@ void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid
     SUBWAY_CHECK(hipGetLastError());
 }
 
-void launch_value_changes(const uint32_t *in, const uint32_t *perm, const uint32_t *snap, ull n, uint32_t *count,
+void launch_value_changes(const uint32_t *in, const uint32_t *name, const uint32_t *snap, ull n, uint32_t *count,
                           uint2 *list, uint32_t cap, hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(value_changes_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, in, perm, snap, (u64)n,
+    hipLaunchKernelGGL(value_changes_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, in, name, snap, (u64)n,
                        count, list, cap);
     SUBWAY_CHECK(hipGetLastError());
 }

@@ -285,10 +285,10 @@ void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull coun
 // n words into mapped pinned host memory (dstMapped: its device address),
 // stored by `grid` workgroups; src and dstMapped 16-byte aligned.
 void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st);
-// The vertices whose value in[perm[v]] (in[v] when perm is null) differs
-// from snap[v]: (v, value) pairs into list (the first cap of them), their
-// number added to *count (which the caller zeroes).
-void launch_value_changes(const uint32_t *in, const uint32_t *perm, const uint32_t *snap, ull n, uint32_t *count,
+// The vertices v whose value in[v] differs from snap[v]: (name[v], value)
+// pairs (name null: v) into list (the first cap of them), their number
+// added to *count (which the caller zeroes).
+void launch_value_changes(const uint32_t *in, const uint32_t *name, const uint32_t *snap, ull n, uint32_t *count,
                           uint2 *list, uint32_t cap, hipStream_t st);
 // PR: the running sum and the residual are FP64 (an FP32 residual at a hub
 // rounds away shares below half its ulp).
