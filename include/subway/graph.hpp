@@ -103,6 +103,13 @@ public:
     ull blkWords = 0;
     uint32_t blkCode = 0;
     bool rowsSorted = false;
+    // Split arrays in ordinary memory beside the block layout: the edges
+    // [tailEdge, m) of the id suffix whose rows take nearly as many bytes in
+    // the block layout as in the split format (low-degree rows) are also kept
+    // pinned here, so whole ranges there move by DMA without staging.
+    uint32_t *tailDst = nullptr;
+    uint8_t *tailW = nullptr;
+    ull tailEdge = 0;
 };
 
 // Renumbers the vertices in descending in-degree order (ties by ascending

@@ -260,6 +260,11 @@ int32_t subway_graph_build_blocks(subway_graph *g, int32_t flags);
  * while the graph lives.  SUBWAY_ERR_ARG when the graph has none. */
 int32_t subway_graph_blocks_view(const subway_graph *g, const uint64_t **row_ptr, const uint32_t **words,
                                  uint64_t *num_words, uint32_t *weight_code);
+/* A graph whose split arrays left pinned memory keeps the split edges of its
+ * low-degree id suffix pinned (whole ranges there move in the split format
+ * by DMA): *tail_edge = the first edge of that copy, m when there is none.
+ * No counterpart in the reference. */
+int32_t subway_graph_split_tail(const subway_graph *g, uint64_t *tail_edge);
 /* perm[v] = internal id of original vertex v, iperm[p] = original id of
  * internal vertex p (n entries each, valid while the graph lives); both NULL
  * for a graph that keeps its own numbering. */

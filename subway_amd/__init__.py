@@ -250,6 +250,13 @@ class Graph:
         threads.  Call it before creating any session on the graph."""
         check(lib.subway_graph_build_blocks(self._h, 1 if unpin_split else 0))
 
+    def split_tail(self) -> int:
+        """First edge of the pinned copy of the low-degree id suffix's split
+        edges (kept when the split arrays left pinned memory), m if none."""
+        t = ctypes.c_uint64()
+        check(lib.subway_graph_split_tail(self._h, ctypes.byref(t)))
+        return t.value
+
     def blocks_view(self):
         """(row_ptr u64[n+1], words u32[num_words], weight_bits, weight_base)
         of the block layout, read-only zero-copy views that keep the graph

@@ -56,6 +56,7 @@ _SIGS = {
     "subway_graph_order": (_i32, [_vp, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p)]),
     "subway_graph_build_blocks": (_i32, [_vp, _i32]),
     "subway_graph_blocks_view": (_i32, [_vp, ctypes.POINTER(_u64p), ctypes.POINTER(_u32p), _u64p, _u32p]),
+    "subway_graph_split_tail": (_i32, [_vp, _u64p]),
     "subway_graph_save": (_i32, [_vp, ctypes.c_char_p]),
     "subway_graph_info": (_i32, [_vp, _u64p, _u64p, _i32p, _u32p, _u32p]),
     "subway_graph_export": (_i32, [_vp, _u64p, _u32p, _u32p]),

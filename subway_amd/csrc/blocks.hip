@@ -518,6 +518,43 @@ void build(HostCsr<E> &csr, DevArr<u32> &dst, DevArr<W> &w, bool hasW, bool unpi
     csr.blkCode = blk_code(wbits, wbase);
 }
 
+// Split arrays in ordinary memory: a pinned copy of the edges of the id
+// suffix [v, n) for the smallest v at which the suffix's rows take at least
+// their split bytes in the block layout (there whole ranges move in the
+// split format, through this copy), at most a quarter of the edges and 2^31
+// of them; none when pinned memory runs short.
+template <class E>
+void pin_split_tail(HostCsr<E> &csr)
+{
+    const u64 n = csr.n, m = csr.m;
+    if (!csr.splitDst || csr.splitPinned || !csr.blkPtr || m == 0) return;
+    const ull *np = csr.ptr, *bp = csr.blkPtr;
+    const u64 maxTail = std::min<u64>(m / 4, 1ull << 31);
+    u64 start = n;
+    for (u64 v = n; v-- > 0;) {
+        const u64 edges = np[n] - np[v];
+        if (edges > maxTail) break;
+        if ((bp[n] - bp[v]) * 4 >= edges * 5) start = v;   // 4 bytes a word, 5 an edge
+    }
+    const u64 t0 = np[start], cnt = m - t0;
+    if (cnt == 0) return;
+    uint32_t *d = nullptr;
+    uint8_t *w = nullptr;
+    if (hipHostMalloc((void **)&d, cnt * sizeof(uint32_t), hipHostMallocPortable) != hipSuccess ||
+        hipHostMalloc((void **)&w, cnt, hipHostMallocPortable) != hipSuccess) {
+        (void)hipGetLastError();
+        if (d) (void)hipHostFree(d);
+        return;
+    }
+    parallel_for(cnt, [&](u64 b, u64 e, int) {
+        std::memcpy(d + b, csr.splitDst + t0 + b, (e - b) * sizeof(uint32_t));
+        std::memcpy(w + b, csr.splitW + t0 + b, e - b);
+    });
+    csr.tailDst = d;
+    csr.tailW = w;
+    csr.tailEdge = t0;
+}
+
 }  // namespace
 
 template <class E>
@@ -567,6 +604,7 @@ void build_blocks(HostCsr<E> &csr, bool unpinSplit)
         DevArr<uint8_t> none;
         build<E, uint8_t>(csr, dst, none, false, unpinSplit, st);
     }
+    pin_split_tail(csr);
 }
 
 template void build_blocks<OutEdge>(HostCsr<OutEdge> &, bool);

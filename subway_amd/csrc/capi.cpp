@@ -226,6 +226,15 @@ int32_t subway_graph_blocks_view(const subway_graph *g, const uint64_t **row_ptr
     });
 }
 
+int32_t subway_graph_split_tail(const subway_graph *g, uint64_t *tail_edge)
+{
+    return guard([&] {
+        if (!g || !tail_edge) throw Error(ERR_ARG, "null argument");
+        if (g->weighted) *tail_edge = g->w->tailDst ? g->w->tailEdge : g->w->m;
+        else *tail_edge = g->u->tailDst ? g->u->tailEdge : g->u->m;
+    });
+}
+
 int32_t subway_graph_order(const subway_graph *g, const uint32_t **perm, const uint32_t **iperm)
 {
     return guard([&] {

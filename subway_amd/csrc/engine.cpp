@@ -1107,19 +1107,23 @@ void Engine<E, G>::buildUnitsLearned()
     ull gLo = 0, gHi = 0, gEdges = 0, gBase = 0;
     bool gFull = true;
     uint gFirstRange = 0, gLastRange = 0;
-    // open whole group: vertex ids [wB0, wB1), at most wCap edges
+    // open whole group: vertex ids [wB0, wB1), at most wCap edges, its last
+    // range wLast
     bool wOpen = false;
     uint wB0 = 0, wB1 = 0;
     ull wEdges = 0, wCap = 0;
+    size_t wLast = 0;
     // edges of the whole ranges from range i on (the ranges this iteration
-    // moves whole), for the geometric tail of whole units
-    std::vector<ull> wholeLeft(P + 1, 0);
+    // moves whole), for the geometric tail of whole units, and their cost
+    // (edges + kVertexCost * vertices) for the pieces of low-degree units
+    std::vector<ull> wholeLeft(P + 1, 0), wholeCostLeft(P + 1, 0);
     for (size_t i = P; i-- > 0;) {
         const ull lo = h_rstats_[4 * i], hi = h_rstats_[4 * i + 1], ea = h_rstats_[4 * i + 2];
         const ull re = np[bounds_[i + 1]] - np[bounds_[i]];
         const bool hub = (bounds_[i + 1] - bounds_[i] == 1) && re > maxPart_;
         const bool whole = !done[i] && hi > lo && !hub && re > 0 && (double)ea >= theta_ * (double)re;
         wholeLeft[i] = wholeLeft[i + 1] + (whole ? re : 0);
+        wholeCostLeft[i] = wholeCostLeft[i + 1] + (whole ? re + kVertexCost * (ull)(bounds_[i + 1] - bounds_[i]) : 0);
     }
     auto flush = [&]() {
         if (!open) return;
@@ -1141,24 +1145,39 @@ void Engine<E, G>::buildUnitsLearned()
         units_.push_back(u);
         open = false;
     };
-    auto flushWhole = [&]() {
-        if (!wOpen) return;
+    auto emitWhole = [&](uint a, uint b) {
         Unit u{};
         u.part = part++;
         u.dma = true;
-        u.hostEdge = np[wB0];
-        u.count = wEdges;
+        u.hostEdge = np[a];
+        u.count = np[b] - np[a];
         u.whole = true;
-        u.n = wB1 - wB0;
+        u.n = b - a;
         u.nodes = nullptr;
-        u.ptr = g_->d_nodePointer + wB0;
-        u.fromEdge = np[wB0];
-        u.idBase = wB0;
-        u.idLo = wB0;
-        u.idHi = wB1;
+        u.ptr = g_->d_nodePointer + a;
+        u.fromEdge = np[a];
+        u.idBase = a;
+        u.idLo = a;
+        u.idHi = b;
         markBlocked(u);
         units_.push_back(u);
+    };
+    std::vector<uint32_t> pieceEnds;
+    auto flushWhole = [&]() {
+        if (!wOpen) return;
         wOpen = false;
+        if (kVertexCost * (ull)(wB1 - wB0) <= wEdges) {
+            emitWhole(wB0, wB1);
+            return;
+        }
+        // low-degree rows: pieces by cost, shrinking towards the iteration's end
+        pieceEnds.clear();
+        split_by_cost(wB0, wB1, np, wholeCostLeft[wLast + 1], kTailMinEdges, pieceEnds);
+        uint a = wB0;
+        for (uint32_t e : pieceEnds) {
+            emitWhole(a, e);
+            a = e;
+        }
     };
     for (size_t i = 0; i < P; i++) {
         if (done[i]) {
@@ -1199,6 +1218,7 @@ void Engine<E, G>::buildUnitsLearned()
                 wEdges = re;
                 wCap = std::min<ull>(maxPart_, whole_unit_cap(maxPart_, re, wholeLeft[i]));
             }
+            wLast = i;
             continue;
         }
         flushWhole();
@@ -1371,10 +1391,17 @@ void Engine<E, G>::issue(size_t idx)
             st_.blockedUnits++;
             st_.h2dBytes += bb;
         } else if (u.dma) {
-            if (split && !edgesMapped_) {
+            const HostCsr<E> &c = *g_->csr;
+            if (split && !edgesMapped_ && c.tailDst && u.hostEdge >= c.tailEdge) {
+                // the pinned copy of the low-degree id suffix
+                SUBWAY_CHECK(hipMemcpyAsync(dst, c.tailDst + (u.hostEdge - c.tailEdge), 4ull * u.count,
+                                            hipMemcpyHostToDevice, sx_));
+                SUBWAY_CHECK(hipMemcpyAsync(dstW, c.tailW + (u.hostEdge - c.tailEdge), u.count,
+                                            hipMemcpyHostToDevice, sx_));
+            } else if (split && !edgesMapped_) {
                 // split arrays in ordinary memory: through the pinned staging slots
-                stagedCopy((const char *)(g_->csr->splitDst + u.hostEdge), dst, 4ull * u.count);
-                stagedCopy((const char *)(g_->csr->splitW + u.hostEdge), dstW, u.count);
+                stagedCopy((const char *)(c.splitDst + u.hostEdge), dst, 4ull * u.count);
+                stagedCopy((const char *)(c.splitW + u.hostEdge), dstW, u.count);
             } else if (split) {
                 SUBWAY_CHECK(hipMemcpyAsync(dst, g_->csr->splitDst + u.hostEdge, 4ull * u.count,
                                             hipMemcpyHostToDevice, sx_));

@@ -65,6 +65,8 @@ HostCsr<E>::~HostCsr()
     delete[] iperm;
     if (blk) (void)hipHostFree(blk);
     delete[] blkPtr;
+    if (tailDst) (void)hipHostFree(tailDst);
+    if (tailW) (void)hipHostFree(tailW);
 }
 
 template <class E>

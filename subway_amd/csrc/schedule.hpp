@@ -65,6 +65,38 @@ inline unsigned long long whole_unit_cap(unsigned long long bufferEdges, unsigne
     return std::max(firstRange, std::min(bufferEdges, std::max(remaining / 2, minEdges)));
 }
 
+// A whole unit of low-degree rows costs its rounds more per vertex than per
+// edge: every round scans all of its vertices.  Its cost is counted as
+// edges + kVertexCost * vertices (about three edges of rounds work a vertex
+// on the low-degree ranges of a scale-28 R-MAT), and a unit whose vertices
+// dominate that cost is cut into pieces: each piece takes half of the cost
+// still to come in the iteration (the rest of the unit plus `after`, the
+// cost of the whole ranges behind it), not less than minCost, so towards the
+// end of an iteration the pieces shrink and the last one, whose rounds no
+// transfer overlaps, is short.  Writes the piece ends (ids, ascending, the
+// last one b) to `ends`.  np: node pointers (np[v] .. np[v + 1] = row v).
+constexpr unsigned long long kVertexCost = 3;
+template <class NP, class Out>
+inline void split_by_cost(uint32_t a, uint32_t b, const NP *np, unsigned long long after,
+                          unsigned long long minCost, Out &ends)
+{
+    auto cost = [&](uint32_t x, uint32_t y) {
+        return (unsigned long long)(np[y] - np[x]) + kVertexCost * (unsigned long long)(y - x);
+    };
+    while (a < b) {
+        const unsigned long long cap = std::max(minCost, (cost(a, b) + after) / 2);
+        // the largest end e in (a, b] with cost(a, e) <= cap, at least a + 1
+        uint32_t lo = a + 1, hi = b;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo + 1) / 2;
+            if (cost(a, mid) <= cap) lo = mid;
+            else hi = mid - 1;
+        }
+        ends.push_back(lo);
+        a = lo;
+    }
+}
+
 }  // namespace subway
 
 #endif  // SUBWAY_SCHEDULE_HPP

@@ -104,3 +104,43 @@ def test_tail_floor_keeps_small_units_whole(tail_cap):
     assert tail_cap([(80, 10, 30, 50)]) == [50]
     assert tail_cap([(80, 90, 30, 1)]) == [90]
     assert tail_cap([(80, 10, 1000, 1)]) == [80]
+
+
+@pytest.fixture(scope="module")
+def split_cost(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("split") / "split_cost"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "subway_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "split_cost.cpp"), "-o", str(exe)], check=True)
+
+    def run(degrees, after, min_cost):
+        np_ = [0]
+        for d in degrees:
+            np_.append(np_[-1] + d)
+        line = f"{len(degrees)} {after} {min_cost}\n" + " ".join(map(str, np_)) + "\n"
+        out = subprocess.run([str(exe)], input=line, capture_output=True, text=True, check=True).stdout
+        return [int(x) for x in out.split()]
+    return run
+
+
+def test_low_degree_unit_pieces_halve_to_the_floor(split_cost):
+    """1000 vertices of degree 1 (cost 4 each: the edge plus three for the
+    vertex), nothing behind them: pieces take half of the cost left, 2000,
+    1000, 500, 250, then the floor of 200 (50 vertices) ends the halving."""
+    ends = split_cost([1] * 1000, 0, 200)
+    sizes = [b - a for a, b in zip([0] + ends, ends)]
+    assert sizes[:4] == [500, 250, 125, 62] and sum(sizes) == 1000 and ends[-1] == 1000
+    assert all(s >= 50 for s in sizes[:-1]) and max(sizes[4:]) <= 62
+
+
+def test_work_behind_a_unit_keeps_it_whole(split_cost):
+    # half of the cost still to come exceeds the unit: one piece
+    assert split_cost([1] * 100, 10_000, 1) == [100]
+
+
+def test_pieces_follow_cost_not_vertex_count(split_cost):
+    # a heavy vertex (degree 400, cost 403) then 100 of degree 1 (cost 4):
+    # the first half of the cost (503 of 803 / 2 = 401, at least one vertex)
+    # is the heavy vertex alone
+    ends = split_cost([400] + [1] * 100, 0, 1)
+    assert ends[0] == 1 and ends[-1] == 101
+    assert all(b > a for a, b in zip([0] + ends, ends))
