@@ -251,18 +251,17 @@ __global__ __launch_bounds__(BB) void block_words_kernel(const u64 *ptr, const u
     __syncthreads();
     for (u64 b = b0 + threadIdx.x; b < b1; b += BB) {
         const BlockSpot s = block_spot(ptr, bidx, b, s_p0, s_p1);
-        pw[b] = blk_payload_words(s.cnt, block_width(dst + s.e, s.cnt), wbits);
+        pw[b] = blk_block_words(s.cnt, block_width(dst + s.e, s.cnt), wbits, s.nb == 1);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) pw[B] = 0;
 }
 
+// pay: exclusive prefix of the blocks' words (descriptors included), so a
+// row starts where its first block does
 __global__ __launch_bounds__(BB) void row_ptr_kernel(const u64 *bidx, const u64 *pay, u64 n, u64 *rowPtr)
 {
     const u64 step = (u64)gridDim.x * BB;
-    for (u64 v = (u64)blockIdx.x * BB + threadIdx.x; v <= n; v += step) {
-        const u64 b = bidx[v];
-        rowPtr[v] = 2 * b + pay[b];
-    }
+    for (u64 v = (u64)blockIdx.x * BB + threadIdx.x; v <= n; v += step) rowPtr[v] = pay[bidx[v]];
 }
 
 template <class W>
@@ -282,17 +281,26 @@ __global__ __launch_bounds__(BB) void encode_kernel(const u64 *ptr, const u64 *b
         const BlockSpot s = block_spot(ptr, bidx, b, s_p0, s_p1);
         const u32 anchor = dst[s.e];
         const u32 wd = block_width(dst + s.e, s.cnt);
-        const u64 woff = pay[b] - pay[bidx[s.v]];
-        if (woff + (pay[b + 1] - pay[b]) > (u64)BLK_MAX_ROW_WORDS) {
-            atomicOr(err, 1u);
-            continue;
-        }
         const u64 row = rowPtr[s.v];
-        out[row + 2 * s.k] = anchor;
-        out[row + 2 * s.k + 1] = ((u32)woff << BLK_WFIELD) | wd;
-        u64 at = row + 2ull * s.nb + woff;
-        u64 acc = 0;
+        u64 at, acc = 0;
         u32 used = 0;
+        if (s.nb == 1) {
+            // one block: the anchor, then the payload led by the width
+            out[row] = anchor;
+            at = row + 1;
+            acc = wd;
+            used = BLK_WFIELD;
+        } else {
+            // words of the row's earlier blocks, less their descriptors
+            const u64 woff = pay[b] - pay[bidx[s.v]] - 2 * s.k;
+            if (woff + (pay[b + 1] - pay[b] - 2) > (u64)BLK_MAX_ROW_WORDS) {
+                atomicOr(err, 1u);
+                continue;
+            }
+            out[row + 2 * s.k] = anchor;
+            out[row + 2 * s.k + 1] = ((u32)woff << BLK_WFIELD) | wd;
+            at = row + 2ull * s.nb + woff;
+        }
         auto put = [&](u32 x, u32 width) {
             if (width == 0) return;
             acc |= (u64)x << used;

@@ -12,12 +12,17 @@
 // a wave computes with a segmented scan): no scan over earlier blocks.
 //
 // Layout, in 32-bit words.  Row v (degree d > 0, nb = ceil(d / BLK_EDGES)
-// blocks) starts at word rowPtr[v]:
+// blocks) starts at word rowPtr[v].  With nb > 1:
 //   words [0, 2 nb)           descriptors, two words per block k:
 //                               anchor_k
 //                               woff_k << BLK_WFIELD | w_k   (w_k <= 32)
 //   words [2 nb, ...)         payloads; block k's starts woff_k words after
 //                             the first payload word of the row
+// With nb = 1 (d <= BLK_EDGES, most rows of a skewed graph) the width moves
+// into the payload and the row takes one descriptor word:
+//   word 0                    anchor
+//   words [1, ...)            the payload, led by w in its first BLK_WFIELD
+//                             bits
 // Payload of block k (cnt edges, cnt = min(BLK_EDGES, d - k BLK_EDGES)):
 //   (cnt - 1) gaps of w_k bits (gap j = destination j - destination j - 1,
 //   j = 1 .. cnt - 1; destination j = anchor + gaps 1 .. j), then
@@ -44,10 +49,17 @@ constexpr uint32_t BLK_MAX_WBASE = 1u << (32 - BLK_WFIELD);   // weight base lim
 // bits needed for x (0 for 0)
 __host__ __device__ inline uint32_t blk_bits(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
 
-// payload words of a block of cnt edges, offset width w, weight width wbits
-__host__ __device__ inline uint32_t blk_payload_words(uint32_t cnt, uint32_t w, uint32_t wbits)
+// payload words of a block of cnt edges, gap width w, weight width wbits
+// (head: the bits in front of the gaps, BLK_WFIELD for a row of one block)
+__host__ __device__ inline uint32_t blk_payload_words(uint32_t cnt, uint32_t w, uint32_t wbits, uint32_t head = 0)
 {
-    return (uint32_t)(((uint64_t)(cnt - 1) * w + (uint64_t)cnt * wbits + 31) >> 5);
+    return (uint32_t)(((uint64_t)head + (uint64_t)(cnt - 1) * w + (uint64_t)cnt * wbits + 31) >> 5);
+}
+
+// all words of block b of a row: its descriptor word(s) and its payload
+__host__ __device__ inline uint32_t blk_block_words(uint32_t cnt, uint32_t w, uint32_t wbits, bool oneBlock)
+{
+    return oneBlock ? 1u + blk_payload_words(cnt, w, wbits, BLK_WFIELD) : 2u + blk_payload_words(cnt, w, wbits);
 }
 
 // The weight code kernels take: wbits | wbase << BLK_WFIELD.

@@ -1585,6 +1585,7 @@ void Engine<E, G>::advance(uint8_t *labOut, ull w0, ull w1, ull shift, int buf, 
     a.shift = shift;
     a.grid = advGrid_;
     a.nv = g_->num_nodes;
+    a.wlim = (uint32_t)std::min<ull>(bufBytes_ / 4, 0xFFFFFFFFull);
     if (o_.profile) pAdv_.begin(sc_);
     launch_advance(a, fr_.ctl, sc_);
     if (o_.profile) pAdv_.end(sc_);
@@ -1642,6 +1643,7 @@ void Engine<E, G>::asyncRounds(const Unit &un, int buf)
         }
         l.joinTicks = o_.joinWindow;
         l.nv = g_->num_nodes;
+        l.wlim = (uint32_t)std::min<ull>(bufBytes_ / 4, 0xFFFFFFFFull);
         launch_rounds(o_.algo, fmtOf(un), l, roundsGrid_, sc_);
         if (o_.profile) ev.end(sc_);
         return;

@@ -858,7 +858,7 @@ template <int ALGO, class E, bool COH>
 __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const u32 *chunkMap, const E *edges,
                                              u32 *value, uint8_t *lab, double *delta, u64 count, u64 W, u64 w0,
                                              u64 wend, u64 shift, u32 psh, u32 *err, u32 *claimCtr, u32 *s_next,
-                                             u32 nv)
+                                             u32 nv, u32 wlim)
 {
     constexpr bool BLOCKED = std::is_same<E, BlockEdge>::value;
     __shared__ u32 s_pre[ADV_CHUNK + 1];
@@ -917,7 +917,10 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
             constexpr bool WEIGHTS = ALGO == ALGO_SSSP || ALGO == ALGO_SSWP;
             const int lane = lane_id();
             // pay = first payload word of the row, jc = index in the block |
-            // edges of the block << 8, then the descriptor (da, dc)
+            // edges of the block << 8 | (row of one block) << 16, then the
+            // descriptor (da, dc); a row of one block has no second
+            // descriptor word, dc is then its first payload word, whose low
+            // BLK_WFIELD bits hold the gap width
             u32 pay[STEP], jc[STEP], da[STEP], dc[STEP];
 #pragma unroll
             for (int v = 0; v < STEP; v++) {
@@ -934,8 +937,16 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
                     const u32 row = s_loc[lo], i = pp - s_pre[lo], d = s_deg[lo];
                     sv[v] = s_val[lo];
                     const u32 k = i >> BLK_SHIFT, left = d - (k << BLK_SHIFT);
-                    pay[v] = row + 2u * ((d + BLK_EDGES - 1u) >> BLK_SHIFT);
-                    jc[v] = (i & (BLK_EDGES - 1u)) | ((left < (u32)BLK_EDGES ? left : (u32)BLK_EDGES) << 8);
+                    const u32 nbk = (d + BLK_EDGES - 1u) >> BLK_SHIFT;
+                    pay[v] = nbk == 1u ? row + 1u : row + 2u * nbk;
+                    jc[v] = (i & (BLK_EDGES - 1u)) | ((left < (u32)BLK_EDGES ? left : (u32)BLK_EDGES) << 8) |
+                            (nbk == 1u ? 1u << 16 : 0u);
+                    if (row + 2u * k + 1u >= wlim) {
+                        // the row's words are not in this buffer: report, never read
+                        (void)atomicCAS(err, 0u, WAIT_DATA);
+                        ok[v] = false;
+                        continue;
+                    }
                     da[v] = W[row + 2u * k];
                     dc[v] = W[row + 2u * k + 1u];
                 }
@@ -950,28 +961,43 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
 #pragma unroll
             for (int v = 0; v < STEP; v++) {
                 if (ok[v]) {
-                    const u32 jj = jc[v] & 0xFFu, cnt = jc[v] >> 8, wd = dc[v] & 63u;
-                    const u32 P = pay[v] + (dc[v] >> BLK_WFIELD);
-                    const u32 obit = jj ? (jj - 1u) * wd : 0u;
-                    f0[v] = W[P + (obit >> 5)];
-                    f1[v] = W[P + (obit >> 5) + 1u];
-                    u32 gsh = 0;
-                    if constexpr (WEIGHTS) {
-                        const u32 gbit = (cnt - 1u) * wd + jj * wbits;
-                        g0[v] = W[P + (gbit >> 5)];
-                        g1[v] = W[P + (gbit >> 5) + 1u];
-                        gsh = gbit & 31u;
+                    const u32 jj = jc[v] & 0xFFu, cnt = (jc[v] >> 8) & 0xFFu, one = jc[v] >> 16, wd = dc[v] & 63u;
+                    // P = the block's first payload word, base = the bits
+                    // in front of its gaps
+                    const u32 P = one ? pay[v] : pay[v] + (dc[v] >> BLK_WFIELD);
+                    const u32 base = one ? (u32)BLK_WFIELD : 0u;
+                    const u32 obit = base + (jj ? (jj - 1u) * wd : 0u);
+                    const u32 gbit = base + (cnt - 1u) * wd + jj * wbits;
+                    if (P + ((WEIGHTS ? gbit : obit) >> 5) + 1u >= wlim) {
+                        (void)atomicCAS(err, 0u, WAIT_DATA);
+                        ok[v] = false;
+                    } else {
+                        f0[v] = W[P + (obit >> 5)];
+                        f1[v] = W[P + (obit >> 5) + 1u];
+                        u32 gsh = 0;
+                        if constexpr (WEIGHTS) {
+                            g0[v] = W[P + (gbit >> 5)];
+                            g1[v] = W[P + (gbit >> 5) + 1u];
+                            gsh = gbit & 31u;
+                        }
+                        sh[v] = (obit & 31u) | (gsh << 8) | ((jj ? wd : 0u) << 16);
                     }
-                    sh[v] = (obit & 31u) | (gsh << 8) | ((jj ? wd : 0u) << 16);
                 }
                 if (__builtin_amdgcn_readfirstlane(ok[v] ? 1u : 0u)) {
                     const u32 j0 = __builtin_amdgcn_readfirstlane(jc[v] & 0xFFu);
                     const u32 w0 = __builtin_amdgcn_readfirstlane(dc[v] & 63u);
-                    const u32 P0 = __builtin_amdgcn_readfirstlane(pay[v] + (dc[v] >> BLK_WFIELD));
+                    const u32 one0 = __builtin_amdgcn_readfirstlane(jc[v] >> 16);
+                    const u32 P0 = __builtin_amdgcn_readfirstlane(one0 ? pay[v] : pay[v] + (dc[v] >> BLK_WFIELD));
+                    const u32 base0 = one0 ? (u32)BLK_WFIELD : 0u;
                     if ((u32)lane + 1u < j0) {
-                        const u32 cbit = (u32)lane * w0;
-                        c0[v] = W[P0 + (cbit >> 5)];
-                        c1[v] = W[P0 + (cbit >> 5) + 1u];
+                        const u32 cbit = base0 + (u32)lane * w0;
+                        if (P0 + (cbit >> 5) + 1u < wlim) {
+                            c0[v] = W[P0 + (cbit >> 5)];
+                            c1[v] = W[P0 + (cbit >> 5) + 1u];
+                        } else {
+                            (void)atomicCAS(err, 0u, WAIT_DATA);
+                            c0[v] = c1[v] = 0u;
+                        }
                     }
                 }
             }
@@ -980,8 +1006,9 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
                 if (!__builtin_amdgcn_readfirstlane(ok[v] ? 1u : 0u)) continue;   // uniform: no position in the wave
                 const u32 j0 = __builtin_amdgcn_readfirstlane(jc[v] & 0xFFu);
                 const u32 w0 = __builtin_amdgcn_readfirstlane(dc[v] & 63u);
+                const u32 base0 = __builtin_amdgcn_readfirstlane(jc[v] >> 16) ? (u32)BLK_WFIELD : 0u;
                 const u32 shadow =
-                    ((u32)lane + 1u < j0) ? blk_extract(c0[v], c1[v], ((u32)lane * w0) & 31u, w0) : 0u;
+                    ((u32)lane + 1u < j0) ? blk_extract(c0[v], c1[v], (base0 + (u32)lane * w0) & 31u, w0) : 0u;
                 const u32 carry = (u32)__builtin_amdgcn_readlane((int)wave_seg_incl_scan(shadow, lane, 0), 63);
                 const u32 jj = ok[v] ? (jc[v] & 0xFFu) : 0u;
                 const u32 gap = ok[v] && jj ? blk_extract(f0[v], f1[v], sh[v] & 31u, sh[v] >> 16) : 0u;
@@ -1036,6 +1063,7 @@ __device__ __forceinline__ u32 advance_chunk(u64 j, const Entry *entries, const 
                     else cv[v] = Rule<ALGO>::cand(sv[v], edge_weight(e));
                 }
             (void)nv;
+            (void)wlim;
         }
         if constexpr (ALGO == ALGO_PR) {
             // delta[v] += share (shared/gpu_kernels.cu:241-244); the residual is
@@ -1087,7 +1115,7 @@ template <int ALGO, class E>
 __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries, const u32 *chunkMap,
                                                             const E *edges, u32 *value, uint8_t *lab,
                                                             double *delta, u64 w0, u64 w1, u64 shift,
-                                                            u32 psh, ScanCtl *ctl, u32 nv)
+                                                            u32 psh, ScanCtl *ctl, u32 nv, u32 wlim)
 {
     __shared__ u32 s_chunk;
     const int tid = threadIdx.x;
@@ -1107,7 +1135,7 @@ __global__ __launch_bounds__(ADV_BLOCK) void advance_kernel(const Entry *entries
     __syncthreads();
     for (u64 c = s_chunk; c < nchunks; c = s_chunk)
         updates += advance_chunk<ALGO, E, false>(j0 + c, entries, chunkMap, edges, value, lab, delta, count, W, w0,
-                                                 wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk, nv);
+                                                 wend, shift, psh, &ctl->error, &ctl->adv_counter, &s_chunk, nv, wlim);
     u32 wsum = wave_reduce(updates);
     if (lane_id() == 0 && wsum) {
         atomicAdd((unsigned long long *)&ctl->updates, (unsigned long long)wsum);
@@ -1275,6 +1303,7 @@ struct RoundsArgs {
     u64 blkBase;           // ... of the buffer's first row
     u32 blkByItem;         // blk indexed by item, else by vertex id
     u32 nv;                // vertices of the graph (decoded destinations are checked against it)
+    u32 wlim;              // EDGE_BLOCKS: words of the buffer (no word at or past it is read)
 };
 
 // All async inner rounds of one resident partition in one persistent
@@ -1343,7 +1372,7 @@ __device__ __forceinline__ void rounds_body(const RoundsArgs &a, u32 me, u32 mem
         for (u64 c = s_claim; c < nchunks; c = s_claim)
             updates += advance_chunk<ALGO, E, true>(c, a.entries, a.chunkMap, (const E *)a.edges, a.value, labOut,
                                               a.delta, count, W, 0, W, 0, a.packCode, wt.err,
-                                              &a.pc->chunk_ctr[r & 1], &s_claim, a.nv);
+                                              &a.pc->chunk_ctr[r & 1], &s_claim, a.nv, a.wlim);
         if (!grid_sync(a.pc, gen, a.spinLimit, (r << 1) | 1u, me, members)) return;
     }
     u32 wsum = wave_reduce(updates);
@@ -2332,7 +2361,7 @@ static void adv(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st)
 {
     hipLaunchKernelGGL((advance_kernel<ALGO, E>), dim3((unsigned)a.grid), dim3(ADV_BLOCK), 0, st,
                        a.entries, a.chunkMap, (const E *)a.edges, a.value, a.labelOut, a.delta,
-                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packCode, ctl, (u32)a.nv);
+                       (u64)a.w0, (u64)a.w1, (u64)a.shift, (u32)a.packCode, ctl, (u32)a.nv, (u32)a.wlim);
 }
 
 // Launch `L<ALGO, storage>` for the buffer's edge format; SSSP and SSWP need
@@ -2512,6 +2541,7 @@ void launch_rounds(int algo, int edgeFmt, const RoundsLaunch &l, int grid, hipSt
     a.blkBase = (u64)l.blkBase;
     a.blkByItem = l.blkByItem;
     a.nv = l.nv;
+    a.wlim = l.wlim;
     a.joinTicks = l.joinTicks ? l.joinTicks : (u32)JOIN_WINDOW_TICKS;
     void *params[] = {&a};
     // an ordinary launch: the membership protocol (join_launch) makes the

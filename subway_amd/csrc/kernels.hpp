@@ -203,6 +203,7 @@ struct AdvanceArgs {
     ull shift;                  // subtracted from edge indices
     int grid;                        // persistent workgroups
     uint32_t nv;                     // vertices of the graph (block decode checks destinations against it)
+    uint32_t wlim;                   // EDGE_BLOCKS: words of the buffer (the decode reads none at or past it)
 };
 void launch_advance(const AdvanceArgs &a, ScanCtl *ctl, hipStream_t st);
 
@@ -231,6 +232,7 @@ struct RoundsLaunch {
     ull blkBase;           // ... of the buffer's first row
     uint32_t blkByItem;    // 1: blk is indexed by item (rows gathered in item order), 0: by vertex id
     uint32_t nv;           // vertices of the graph (block decode checks destinations against it)
+    uint32_t wlim;         // EDGE_BLOCKS: words of the buffer (the decode reads none at or past it)
 };
 // Largest co-resident grid for the rounds kernel, one workgroup slot per CU
 // below full occupancy.

@@ -5,7 +5,9 @@ sort_rows(ptr, ends, w): every row in ascending destination order, equal
 destinations keeping their order (the device sort is a stable radix sort by
 destination, then by row).
 encode(ptr, ends, w): (row_ptr u64[n+1], words u32[T], wbits, wbase) of rows
-already sorted (gaps between consecutive destinations of a block).
+already sorted (gaps between consecutive destinations of a block; a row of
+one block keeps its gap width in the first 6 bits of its payload instead of
+a second descriptor word).
 decode(row_ptr, words, ptr, wbits, wbase, weighted): (ends, w) back.
 
 Plain Python integers do the bit packing, so it suits graphs up to about a
@@ -53,7 +55,7 @@ def encode(ptr, ends, w=None):
             anchor = int(ends[e])
             gaps = [int(ends[e + j]) - int(ends[e + j - 1]) for j in range(1, cnt)]
             wd = _bits(max(gaps, default=0))
-            val, pos = 0, 0
+            val, pos = (wd, WFIELD) if nb == 1 else (0, 0)
             for g in gaps:
                 val |= g << pos
                 pos += wd
@@ -62,7 +64,7 @@ def encode(ptr, ends, w=None):
                     val |= (int(w[e + j]) - wbase) << pos
                     pos += wbits
             words = (pos + 31) // 32
-            desc += [anchor, (len(pay) << WFIELD) | wd]
+            desc += [anchor] if nb == 1 else [anchor, (len(pay) << WFIELD) | wd]
             pay += [(val >> (32 * i)) & 0xFFFFFFFF for i in range(words)]
         out += desc + pay
     row_ptr[n] = len(out)
@@ -92,9 +94,12 @@ def decode(row_ptr, words, ptr, wbits, wbase, weighted):
             k, j = i // BLK, i % BLK
             cnt = min(BLK, d - BLK * k)
             anchor, c = W[r + 2 * k], W[r + 2 * k + 1]
-            wd, woff = c & 63, c >> WFIELD
-            p = r + 2 * nb + woff
-            ends[a + i] = anchor + sum(field(p, (t - 1) * wd, wd) for t in range(1, j + 1))
+            if nb == 1:
+                # one block: the payload starts at word 1 with the 6-bit width
+                wd, p, base = c & 63, r + 1, WFIELD
+            else:
+                wd, p, base = c & 63, r + 2 * nb + (c >> WFIELD), 0
+            ends[a + i] = anchor + sum(field(p, base + (t - 1) * wd, wd) for t in range(1, j + 1))
             if weighted:
-                w[a + i] = wbase + field(p, (cnt - 1) * wd + j * wbits, wbits)
+                w[a + i] = wbase + field(p, base + (cnt - 1) * wd + j * wbits, wbits)
     return ends, w

@@ -10,14 +10,18 @@ from tests import _blocks as B
 
 
 def test_known_answer():
-    # one row 5, 7, 12 with weights 3, 1, 2: wbase 1, wbits 2; anchor 5,
-    # gaps 2 and 5 at 3 bits (bits 0-2, 3-5), weights 2, 0, 1 at bits 6-7,
-    # 8-9, 10-11 -> 2 + 5*8 + 2*64 + 0*256 + 1*1024 = 1194
+    # one row 5, 7, 12 with weights 3, 1, 2: wbase 1, wbits 2; one block, so
+    # the anchor 5 and then the payload: the gap width 3 in bits 0-5, gaps 2
+    # and 5 at 3 bits (bits 6-8, 9-11), weights 2, 0, 1 at bits 12-13, 14-15,
+    # 16-17 -> 3 + 2*64 + 5*512 + 2*4096 + 0*16384 + 1*65536 = 76419
     ptr = np.array([0, 3], np.uint64)
     row_ptr, words, wbits, wbase = B.encode(ptr, np.array([5, 7, 12], np.uint32), np.array([3, 1, 2], np.uint32))
     assert (wbits, wbase) == (2, 1)
-    assert row_ptr.tolist() == [0, 3]
-    assert words.tolist() == [5, 3, 1194]
+    assert row_ptr.tolist() == [0, 2]
+    assert words.tolist() == [5, 76419]
+    # a 1-edge unweighted row: the anchor and a payload word holding width 0
+    row_ptr, words, _, _ = B.encode(np.array([0, 1], np.uint64), np.array([9], np.uint32))
+    assert row_ptr.tolist() == [0, 2] and words.tolist() == [9, 0]
     # two blocks: 33 edges 0, 2, 4, .., 64, unweighted: block 0 anchor 0 and
     # 31 gaps of 2 at 2 bits (62 bits -> 2 words), block 1 the single edge 64
     # (no payload, its payload offset 2 words)
@@ -65,7 +69,8 @@ def test_round_trip_wide_offsets_and_equal_weights():
     e, ww = B.sort_rows(ptr, ends, w)
     row_ptr, words, wbits, wbase = B.encode(ptr, e, ww)
     assert (wbits, wbase) == (0, 9)
-    assert words[1] & 63 == 32            # the gap 0 -> 0xFFFFFFFF needs 32 bits
+    assert words[1] & 63 == 32            # the gap 0 -> 0xFFFFFFFF needs 32 bits (row 0: one block, width in its payload)
+    assert words[1] == (32 | (0xFFFFFFFF << 6)) & 0xFFFFFFFF and words[2] == 0xFFFFFFFF >> 26
     de, dw = B.decode(row_ptr, words, ptr, wbits, wbase, True)
     assert np.array_equal(de, e) and np.array_equal(dw, ww)
 
