@@ -8,7 +8,7 @@
 // graph's weight width.  Rows of one id range are contiguous, so a range
 // moves as one DMA of fewer bytes than its plain edges, and any edge decodes
 // from its row's offset, its index in the row and the row's degree plus the
-// gaps before it in its own block (a prefix sum over at most 31 gaps, which
+// gaps before it in its own block (a prefix sum over at most 63 gaps, which
 // a wave computes with a segmented scan): no scan over earlier blocks.
 //
 // Layout, in 32-bit words.  Row v (degree d > 0, nb = ceil(d / BLK_EDGES)
@@ -40,8 +40,8 @@
 
 namespace subway {
 
-constexpr int BLK_EDGES = 32;                       // edges per block
-constexpr int BLK_SHIFT = 5;                        // log2(BLK_EDGES)
+constexpr int BLK_EDGES = 64;                       // edges per block (a wave's positions)
+constexpr int BLK_SHIFT = 6;                        // log2(BLK_EDGES)
 constexpr int BLK_WFIELD = 6;                       // width bits in a descriptor's second word
 constexpr uint32_t BLK_MAX_ROW_WORDS = 1u << (32 - BLK_WFIELD);
 constexpr uint32_t BLK_MAX_WBASE = 1u << (32 - BLK_WFIELD);   // weight base limit (weight code word)

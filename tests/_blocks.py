@@ -14,7 +14,7 @@ Plain Python integers do the bit packing, so it suits graphs up to about a
 million edges."""
 import numpy as np
 
-BLK = 32
+BLK = 64
 WFIELD = 6
 
 

@@ -1,7 +1,7 @@
 """The block-compressed row layout, on the CPU: a hand-encoded known answer
 pins the Python restatement (tests/_blocks.py), and encode -> decode gives
-back every edge of random graphs, including rows of 0, 1, 31, 32, 33 and 65
-edges, duplicate destinations (offset width 0), a width of 32 bits, equal
+back every edge of random graphs, including rows of 0, 1, 31 .. 33, 63 .. 65,
+128, 129 and 197 edges, duplicate destinations (offset width 0), a width of 32 bits, equal
 weights (weight width 0) and unweighted rows."""
 import numpy as np
 import pytest
@@ -22,16 +22,16 @@ def test_known_answer():
     # a 1-edge unweighted row: the anchor and a payload word holding width 0
     row_ptr, words, _, _ = B.encode(np.array([0, 1], np.uint64), np.array([9], np.uint32))
     assert row_ptr.tolist() == [0, 2] and words.tolist() == [9, 0]
-    # two blocks: 33 edges 0, 2, 4, .., 64, unweighted: block 0 anchor 0 and
-    # 31 gaps of 2 at 2 bits (62 bits -> 2 words), block 1 the single edge 64
-    # (no payload, its payload offset 2 words)
-    ptr = np.array([0, 0, 33], np.uint64)
-    row_ptr, words, _, _ = B.encode(ptr, np.arange(0, 66, 2, dtype=np.uint32))
-    assert row_ptr.tolist() == [0, 0, 4 + 2]
-    assert words[:4].tolist() == [0, (0 << 6) | 2, 64, (2 << 6) | 0]
-    bits = int(words[4]) | int(words[5]) << 32
-    assert [(bits >> (2 * j)) & 3 for j in range(31)] == [2] * 31
-    assert bits >> 62 == 0
+    # two blocks: 65 edges 0, 2, 4, .., 128, unweighted: block 0 anchor 0 and
+    # 63 gaps of 2 at 2 bits (126 bits -> 4 words), block 1 the single edge
+    # 128 (no payload, its payload offset 4 words)
+    ptr = np.array([0, 0, 65], np.uint64)
+    row_ptr, words, _, _ = B.encode(ptr, np.arange(0, 130, 2, dtype=np.uint32))
+    assert row_ptr.tolist() == [0, 0, 4 + 4]
+    assert words[:4].tolist() == [0, (0 << 6) | 2, 128, (4 << 6) | 0]
+    bits = sum(int(words[4 + i]) << (32 * i) for i in range(4))
+    assert [(bits >> (2 * j)) & 3 for j in range(63)] == [2] * 63
+    assert bits >> 126 == 0
 
 
 def random_graph(rng, n, degs, weighted, wmax=64, span=None):
@@ -49,7 +49,7 @@ def random_graph(rng, n, degs, weighted, wmax=64, span=None):
 @pytest.mark.parametrize("weighted", [False, True])
 def test_round_trip_row_lengths(weighted):
     rng = np.random.default_rng(3)
-    degs = [0, 1, 31, 32, 33, 65, 0, 2, 64, 97, 5]
+    degs = [0, 1, 31, 32, 33, 63, 64, 65, 0, 2, 128, 129, 197, 5]
     ptr, ends, w = random_graph(rng, 200, degs, weighted)
     ends[ptr[3]:ptr[3] + 10] = 7          # duplicates: a block of equal destinations
     e, ww = B.sort_rows(ptr, ends, w)
