@@ -239,10 +239,11 @@ int32_t subway_graph_split_view(const subway_graph *g, const uint64_t **node_poi
  * of free device memory.  No counterpart in the reference. */
 int32_t subway_graph_reorder(subway_graph *g, int32_t order);
 /* Sorts every row by destination and builds the block-compressed copy of
- * the rows (32-edge blocks: an anchor destination, offsets from it at the
- * block's bit width, weights at the graph's weight width; layout in
- * subway_amd/csrc/blocks.hpp), on the GPU, in place.  Sessions on the graph
- * then move whole id ranges in that layout (fewer bytes over the host link)
+ * the rows (64-edge blocks: an anchor destination, the gaps between
+ * consecutive destinations at the block's bit width, weights at the graph's
+ * weight width; layout in subway_amd/csrc/blocks.hpp), on the GPU, in
+ * place.  Sessions on the graph then move whole id ranges (and compacted
+ * units, when smaller) in that layout (fewer bytes over the host link)
  * unless options.no_blocks is set; every result is unchanged (a vertex's
  * out-edges keep their multiset).  The accessors (export, split_view, save)
  * see the sorted rows.  Call it before any session is created on the graph
