@@ -51,6 +51,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # the OpenMP runtime reads these when the baseline library loads
 os.environ.setdefault("OMP_PROC_BIND", "close")
+# the CPUs this process may run on, read before any OpenMP region binds the
+# main thread to one of them
+AFFINITY_CPUS = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "end-to-end GTEPS + H2D GB moved; learned vs plain policy, same exact result"
@@ -334,7 +337,7 @@ def cpu_baseline(np, g, args, src, gpu_vals):
         same = bool(np.max(np.abs(a - vals) / np.maximum(np.abs(a), 1e-30)) < 1e-3)
     else:
         same = bool(np.array_equal(vals, gpu_vals))
-    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    aff = AFFINITY_CPUS
     return {
         "value": round(g.m / med / 1e9, 4), "unit": "GTEPS", "cores": O.omp_threads(), "kind": "port",
         "sample": f"the full workload: one {args.algo} run of the OpenMP build on the same {g.m}-edge CSR and source, "
