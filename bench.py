@@ -307,7 +307,7 @@ def to_original(np, g, algo, vals):
 def cpu_baseline(np, g, args, src, gpu_vals):
     """OpenMP build of the same rules on the same CSR (the graph's own host
     arrays, so a renumbered graph's internal numbering, its results mapped
-    back), BASELINE protocol: one warm-up, median of 3."""
+    back), timed as one warm-up then the median of 3."""
     from tests import _oracle as O
     o = g.order()
     if o is not None:
