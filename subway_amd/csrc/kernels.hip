@@ -2026,31 +2026,47 @@ __global__ void unpermute_kernel(const u32 *in, const u32 *perm, u64 v0, u64 cou
 // in[v] as it was copied.  Every changed v appends (name[v], in[v]) to
 // `list` (name = the original id of a renumbered graph's vertex v; null:
 // v itself; the first `cap` entries are written) and counts in *count, one
-// atomic per wave.
+// atomic per wave.  A lane compares four consecutive values (16-byte loads;
+// in and snap 16-byte aligned).
 __global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const u32 *name, const u32 *snap, u64 n,
                                                             u32 *count, uint2 *list, u32 cap)
 {
     const int lane = lane_id();
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    // every lane of a wave runs the same number of trips (the ballot below)
-    const u64 trips = (n + stride - 1) / stride;
-    u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    for (u64 t = 0; t < trips; t++, v += stride) {
-        bool changed = false;
-        u32 cur = 0;
-        if (v < n) {
-            cur = in[v];
-            changed = cur != snap[v];
+    const u64 n4 = (n + 3) >> 2;
+    // every lane of a wave runs the same number of trips (the wave scan below)
+    const u64 trips = (n4 + stride - 1) / stride;
+    u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (u64 t = 0; t < trips; t++, g += stride) {
+        u32 cur[4] = {0u, 0u, 0u, 0u}, chg = 0;
+        const u64 v0 = g << 2;
+        if (v0 + 4 <= n) {
+            const uint4 a = reinterpret_cast<const uint4 *>(in)[g];
+            const uint4 b = reinterpret_cast<const uint4 *>(snap)[g];
+            cur[0] = a.x;
+            cur[1] = a.y;
+            cur[2] = a.z;
+            cur[3] = a.w;
+            chg = (a.x != b.x ? 1u : 0u) | (a.y != b.y ? 2u : 0u) | (a.z != b.z ? 4u : 0u) | (a.w != b.w ? 8u : 0u);
+        } else {
+            for (u32 j = 0; j < 4; j++)
+                if (v0 + j < n) {
+                    cur[j] = in[v0 + j];
+                    if (cur[j] != snap[v0 + j]) chg |= 1u << j;
+                }
         }
-        const u64 mask = __ballot(changed);
-        if (mask == 0) continue;
+        const u32 nc = (u32)__popc(chg);
+        if (!__any(nc != 0)) continue;
+        const u32 incl = wave_incl_scan(nc);
+        const u32 total = (u32)__shfl((int)incl, WAVE - 1);
         u32 base = 0;
-        if (lane == __ffsll((unsigned long long)mask) - 1) base = atomicAdd(count, (u32)__popcll(mask));
-        base = (u32)__shfl((int)base, __ffsll((unsigned long long)mask) - 1);
-        if (changed) {
-            const u32 at = base + (u32)__popcll(mask & ((1ull << lane) - 1ull));
-            if (at < cap) list[at] = make_uint2(name ? name[v] : (u32)v, cur);
-        }
+        if (lane == 0) base = atomicAdd(count, total);
+        base = (u32)__shfl((int)base, 0) + incl - nc;
+        for (u32 j = 0; j < 4; j++)
+            if ((chg >> j) & 1u) {
+                if (base < cap) list[base] = make_uint2(name ? name[v0 + j] : (u32)(v0 + j), cur[j]);
+                base++;
+            }
     }
 }
 
@@ -2058,16 +2074,32 @@ __global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const
 // CUs (16-byte nontemporal stores) rather than by a copy engine: a copy
 // engine serves device-to-host copies in order, so one long copy would hold
 // back the small read-backs the host waits for between iterations.
+// dst[i] = src[perm[i]] (src[i] when perm is null): with perm, the values
+// of a renumbered graph in the original order, gathered on the way.
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void store_host_kernel(const u32 *src, u32 *dst, u64 n)
+__global__ __launch_bounds__(256) void store_host_kernel(const u32 *src, const u32 *perm, u32 *dst, u64 n)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     const u64 n4 = n >> 2;
-    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
     u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
-    for (u64 i = t0; i < n4; i += stride) __builtin_nontemporal_store(s4[i], &d4[i]);
-    for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[i];
+    if (perm) {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(perm);
+        for (u64 i = t0; i < n4; i += stride) {
+            const uint4 p = p4[i];
+            u32x4 x;
+            x.x = src[p.x];
+            x.y = src[p.y];
+            x.z = src[p.z];
+            x.w = src[p.w];
+            __builtin_nontemporal_store(x, &d4[i]);
+        }
+        for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[perm[i]];
+    } else {
+        const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+        for (u64 i = t0; i < n4; i += stride) __builtin_nontemporal_store(s4[i], &d4[i]);
+        for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[i];
+    }
 }
 
 // FP32 result values from the FP64 running sums
@@ -2421,10 +2453,11 @@ void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull coun
     SUBWAY_CHECK(hipGetLastError());
 }
 
-void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st)
+void launch_store_host(const uint32_t *src, const uint32_t *perm, uint32_t *dstMapped, ull n, int grid,
+                       hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(store_host_kernel, dim3(grid), dim3(256), 0, st, src, dstMapped, (u64)n);
+    hipLaunchKernelGGL(store_host_kernel, dim3(grid), dim3(256), 0, st, src, perm, dstMapped, (u64)n);
     SUBWAY_CHECK(hipGetLastError());
 }
 
