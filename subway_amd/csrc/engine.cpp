@@ -647,17 +647,16 @@ void Engine<E, G>::setupEarlyReadback()
         valueDev_ = nullptr;
         return;
     }
+    SUBWAY_CHECK(hipMalloc(&d_snap_, n * sizeof(uint32_t)));
     if (d_perm_) {
-        // renumbered: the changes are found in the internal order and named
-        // through the inverse map
+        // renumbered: the changes are found in the internal order (a second
+        // copy of the values) and named through the inverse map
         SUBWAY_CHECK(hipMalloc(&d_snapInt_, n * sizeof(uint32_t)));
         if (!d_iperm_) {
             SUBWAY_CHECK(hipMalloc(&d_iperm_, n * sizeof(uint32_t)));
             SUBWAY_CHECK(hipMemcpy(d_iperm_, g_->csr->iperm, n * sizeof(uint32_t), hipMemcpyHostToDevice));
             SUBWAY_CHECK(hipDeviceSynchronize());   // a pageable copy may return before its DMA ends
         }
-    } else {
-        SUBWAY_CHECK(hipMalloc(&d_snap_, n * sizeof(uint32_t)));
     }
     SUBWAY_CHECK(hipMalloc(&d_chg_, (size_t)chgCap_ * sizeof(uint2)));
     SUBWAY_CHECK(hipMalloc(&d_chgCount_, sizeof(uint32_t)));
@@ -676,21 +675,18 @@ void Engine<E, G>::startEarlyReadback()
 {
     const ull n = g_->num_nodes;
     if (d_perm_) {
-        // the snapshot the end compares against, in the internal order; the
-        // store then gathers the live values through perm: a value that
-        // changes while it runs changes monotonically (min for BFS, SSSP and
-        // CC, max for SSWP), so a vertex whose stored value can differ from
-        // its final one differs from the snapshot too, and is patched
+        // the internal-order copy the end compares against, and the values
+        // in the original order for the store (a gather through perm inside
+        // the store, by its 64 workgroups, took 3 ms longer at scale 28 than
+        // this full-device pass)
         SUBWAY_CHECK(hipMemcpyAsync(d_snapInt_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
-        SUBWAY_CHECK(hipEventRecord(evSnap_, sc_));
-        SUBWAY_CHECK(hipStreamWaitEvent(sd_, evSnap_, 0));
-        launch_store_host((const uint32_t *)g_->d_value, d_perm_, valueDev_, n, kStoreGrid, sd_);
+        launch_unpermute((const uint32_t *)g_->d_value, d_perm_, 0, n, 1, d_snap_, sc_);
     } else {
         SUBWAY_CHECK(hipMemcpyAsync(d_snap_, g_->d_value, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, sc_));
-        SUBWAY_CHECK(hipEventRecord(evSnap_, sc_));
-        SUBWAY_CHECK(hipStreamWaitEvent(sd_, evSnap_, 0));
-        launch_store_host(d_snap_, nullptr, valueDev_, n, kStoreGrid, sd_);
     }
+    SUBWAY_CHECK(hipEventRecord(evSnap_, sc_));
+    SUBWAY_CHECK(hipStreamWaitEvent(sd_, evSnap_, 0));
+    launch_store_host(d_snap_, valueDev_, n, kStoreGrid, sd_);
     SUBWAY_CHECK(hipEventRecord(evBack_, sd_));
     early_ = true;
 }

@@ -2074,32 +2074,16 @@ __global__ __launch_bounds__(256) void value_changes_kernel(const u32 *in, const
 // CUs (16-byte nontemporal stores) rather than by a copy engine: a copy
 // engine serves device-to-host copies in order, so one long copy would hold
 // back the small read-backs the host waits for between iterations.
-// dst[i] = src[perm[i]] (src[i] when perm is null): with perm, the values
-// of a renumbered graph in the original order, gathered on the way.
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void store_host_kernel(const u32 *src, const u32 *perm, u32 *dst, u64 n)
+__global__ __launch_bounds__(256) void store_host_kernel(const u32 *src, u32 *dst, u64 n)
 {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     const u64 n4 = n >> 2;
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
     u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
-    if (perm) {
-        const uint4 *p4 = reinterpret_cast<const uint4 *>(perm);
-        for (u64 i = t0; i < n4; i += stride) {
-            const uint4 p = p4[i];
-            u32x4 x;
-            x.x = src[p.x];
-            x.y = src[p.y];
-            x.z = src[p.z];
-            x.w = src[p.w];
-            __builtin_nontemporal_store(x, &d4[i]);
-        }
-        for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[perm[i]];
-    } else {
-        const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
-        for (u64 i = t0; i < n4; i += stride) __builtin_nontemporal_store(s4[i], &d4[i]);
-        for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[i];
-    }
+    for (u64 i = t0; i < n4; i += stride) __builtin_nontemporal_store(s4[i], &d4[i]);
+    for (u64 i = (n4 << 2) + t0; i < n; i += stride) dst[i] = src[i];
 }
 
 // FP32 result values from the FP64 running sums
@@ -2453,11 +2437,10 @@ void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull coun
     SUBWAY_CHECK(hipGetLastError());
 }
 
-void launch_store_host(const uint32_t *src, const uint32_t *perm, uint32_t *dstMapped, ull n, int grid,
-                       hipStream_t st)
+void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(store_host_kernel, dim3(grid), dim3(256), 0, st, src, perm, dstMapped, (u64)n);
+    hipLaunchKernelGGL(store_host_kernel, dim3(grid), dim3(256), 0, st, src, dstMapped, (u64)n);
     SUBWAY_CHECK(hipGetLastError());
 }
 

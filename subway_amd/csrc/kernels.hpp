@@ -285,10 +285,8 @@ void launch_init_label_state(int algo, int async, ull n, uint32_t *value, uint8_
 void launch_unpermute(const uint32_t *in, const uint32_t *perm, ull v0, ull count, uint32_t width, uint32_t *out,
                       hipStream_t st);
 // n words into mapped pinned host memory (dstMapped: its device address),
-// dstMapped[i] = src[perm[i]] (src[i] when perm is null), stored by `grid`
-// workgroups; perm and dstMapped 16-byte aligned (src too without perm).
-void launch_store_host(const uint32_t *src, const uint32_t *perm, uint32_t *dstMapped, ull n, int grid,
-                       hipStream_t st);
+// stored by `grid` workgroups; src and dstMapped 16-byte aligned.
+void launch_store_host(const uint32_t *src, uint32_t *dstMapped, ull n, int grid, hipStream_t st);
 // The vertices v whose value in[v] differs from snap[v]: (name[v], value)
 // pairs (name null: v) into list (the first cap of them), their number
 // added to *count (which the caller zeroes).
